@@ -1,0 +1,294 @@
+// C ABI: context management, argument validation and the Shape trait mirror.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "internal.h"
+
+namespace cth {
+
+static thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+int check_hip(hipError_t e, const char* what) {
+    if (e == hipSuccess) return CTH_OK;
+    return fail(e == hipErrorOutOfMemory ? CTH_E_NOMEM : CTH_E_DEVICE,
+                std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// `Mandelbulb::new` asserts max_iters >= 1 (src/shape/mandelbulb.rs:20); the
+// power of `Mandelbulb<P>` is a u8 const generic (:13).
+int resolve_mandel(const cth_shape& s, MandelUniform& u, int& family) {
+    if (s.max_iters == 0) return fail(CTH_E_INVALID, "max_iters must be >= 1");
+    u.max_iters = s.max_iters;
+    u.bailout = s.bailout;
+    u.power = s.power;
+    if (s.kind == CTH_SHAPE_MANDELBULB) {
+        float p = s.power;
+        if (!(p >= 1.0f && p <= 255.0f) || std::floor(p) != p)
+            return fail(CTH_E_INVALID, "Mandelbulb power must be an integer in [1, 255]");
+        u.ipower = (int)p;
+        u.integral = 1;
+        family = (u.ipower == 8) ? FAM_P8 : FAM_TRIG;
+        return CTH_OK;
+    }
+    if (s.kind == CTH_SHAPE_MANDELBULB_TRIG) {
+        if (!std::isfinite(s.power)) return fail(CTH_E_INVALID, "power must be finite");
+        u.ipower = 0;
+        u.integral = 0;
+        family = FAM_TRIG;
+        return CTH_OK;
+    }
+    return fail(CTH_E_INVALID, "not a Mandelbulb shape");
+}
+
+static int check_shape(const cth_shape* s) {
+    if (!s) return fail(CTH_E_INVALID, "null shape");
+    if (s->kind == CTH_SHAPE_SPHERE) return CTH_OK;
+    MandelUniform u;
+    int fam;
+    return resolve_mandel(*s, u, fam);
+}
+
+static hipStream_t pick_stream(Ctx* c, void* stream) { return stream ? (hipStream_t)stream : c->stream; }
+
+// RAII device buffer for the host-pointer entry points.
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    int alloc(size_t bytes) {
+        if (bytes == 0) bytes = 4;
+        return check_hip(hipMalloc(&p, bytes), "hipMalloc");
+    }
+};
+
+}  // namespace cth
+
+using namespace cth;
+
+struct cth_ctx : public Ctx {};
+
+extern "C" {
+
+const char* cth_last_error(void) { return g_last_error.c_str(); }
+
+int cth_ctx_create(int device, cth_ctx** out) {
+    if (!out) return fail(CTH_E_INVALID, "null output pointer");
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count <= 0) return fail(CTH_E_DEVICE, "no HIP device available");
+    if (device < 0 || device >= count) return fail(CTH_E_INVALID, "device index out of range");
+    hipDeviceProp_t prop;
+    if (int rc = check_hip(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties")) return rc;
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(CTH_E_DEVICE, std::string("library is built for gfx950, device is ") + prop.gcnArchName);
+    if (int rc = check_hip(hipSetDevice(device), "hipSetDevice")) return rc;
+    cth_ctx* c = new cth_ctx();
+    c->device = device;
+    c->num_cus = prop.multiProcessorCount;
+    int rc = check_hip(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+    if (rc == CTH_OK) rc = check_hip(hipMalloc((void**)&c->counters, 2 * COUNTER_RING * sizeof(uint32_t)), "hipMalloc");
+    if (rc == CTH_OK)
+        rc = check_hip(hipMemset(c->counters, 0, 2 * COUNTER_RING * sizeof(uint32_t)), "hipMemset counters");
+    if (rc == CTH_OK) rc = check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    if (rc != CTH_OK) {
+        cth_ctx_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return CTH_OK;
+}
+
+void cth_ctx_destroy(cth_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->counters) (void)hipFree(c->counters);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int cth_ctx_set_mode(cth_ctx* c, int mode) {
+    if (!c) return fail(CTH_E_INVALID, "null context");
+    if (mode != CTH_MODE_PARITY && mode != CTH_MODE_FAST) return fail(CTH_E_INVALID, "unknown mode");
+    c->mode = mode;
+    return CTH_OK;
+}
+
+int cth_ctx_get_mode(const cth_ctx* c) { return c ? c->mode : -1; }
+
+void* cth_ctx_stream(cth_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int cth_ctx_num_cus(const cth_ctx* c) { return c ? c->num_cus : 0; }
+
+int cth_synchronize(cth_ctx* c) {
+    if (!c) return fail(CTH_E_INVALID, "null context");
+    return check_hip(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+}
+
+int cth_bounding_box(const cth_shape* s, cth_span* out) {
+    if (!s || !out) return fail(CTH_E_INVALID, "null argument");
+    if (s->kind == CTH_SHAPE_SPHERE) {
+        // center + -off .. center + off  (src/shape/sphere.rs:27-30)
+        for (int a = 0; a < 3; ++a) {
+            out->start[a] = s->center[a] + -s->radius;
+            out->end[a] = s->center[a] + s->radius;
+        }
+        return CTH_OK;
+    }
+    if (int rc = check_shape(s)) return rc;
+    for (int a = 0; a < 3; ++a) {
+        out->start[a] = -1.2f;
+        out->end[a] = 1.2f;
+    }
+    return CTH_OK;
+}
+
+int cth_min_distance_dev(cth_ctx* c, const cth_shape* s, const float* pts, const float* powers, size_t n, float* out,
+                         uint32_t* iters, void* stream) {
+    if (!c) return fail(CTH_E_INVALID, "null context");
+    if (int rc = check_shape(s)) return rc;
+    if (n && (!pts || !out)) return fail(CTH_E_INVALID, "null buffer");
+    hipStream_t st = pick_stream(c, stream);
+    const size_t B = size_t(1) << 30;
+    for (size_t off = 0; off < n; off += B) {
+        uint32_t m = (uint32_t)((n - off) < B ? (n - off) : B);
+        int rc = launch_de_points(*c, *s, pts + 3 * off, powers ? powers + off : nullptr, m, out + off,
+                                  iters ? iters + off : nullptr, nullptr, nullptr, st);
+        if (rc) return rc;
+    }
+    return CTH_OK;
+}
+
+int cth_de_state_dev(cth_ctx* c, const cth_shape* s, const float* pts, const float* powers, size_t n, float* r,
+                     float* dr, uint32_t* iters, void* stream) {
+    if (!c) return fail(CTH_E_INVALID, "null context");
+    if (int rc = check_shape(s)) return rc;
+    if (s->kind == CTH_SHAPE_SPHERE) return fail(CTH_E_UNSUPPORTED, "a sphere has no iteration state");
+    if (n && (!pts || !r || !dr || !iters)) return fail(CTH_E_INVALID, "null buffer");
+    if (n >= (size_t(1) << 31)) return fail(CTH_E_INVALID, "batch larger than 2^31");
+    return launch_de_points(*c, *s, pts, powers, (uint32_t)n, nullptr, iters, r, dr, pick_stream(c, stream));
+}
+
+int cth_de_grid_dev(cth_ctx* c, const cth_shape* s, const float start[3], const float end[3], uint32_t n,
+                    float divisor, float* out, uint32_t* iters, void* stream) {
+    if (!c) return fail(CTH_E_INVALID, "null context");
+    if (int rc = check_shape(s)) return rc;
+    if (!start || !end || (n && !out)) return fail(CTH_E_INVALID, "null buffer");
+    if (!(divisor > 0.0f)) return fail(CTH_E_INVALID, "divisor must be positive");
+    return launch_de_grid(*c, *s, start, end, n, divisor, out, iters, pick_stream(c, stream));
+}
+
+int cth_sample_spans_dev(cth_ctx* c, const cth_shape* s, const float* spans, size_t nspans, uint32_t res,
+                         float* dists, uint64_t* cross_bits, void* stream) {
+    if (!c) return fail(CTH_E_INVALID, "null context");
+    if (int rc = check_shape(s)) return rc;
+    if (nspans && (!spans || !dists)) return fail(CTH_E_INVALID, "null buffer");
+    if (res == 0 || (res & (res - 1)) != 0) return fail(CTH_E_INVALID, "resolution must be a power of two");
+    if (res > 512) return fail(CTH_E_INVALID, "resolution above 512");
+    hipStream_t st = pick_stream(c, stream);
+    size_t per = size_t(res + 1) * (res + 1) * (res + 1);
+    size_t words = (size_t(res) * res * res + 63) / 64;
+    size_t batch = ((size_t(1) << 30) / per);
+    if (batch == 0) batch = 1;
+    for (size_t off = 0; off < nspans; off += batch) {
+        size_t m = (nspans - off) < batch ? (nspans - off) : batch;
+        int rc = launch_sample_spans(*c, *s, spans + 6 * off, (uint32_t)m, res, dists + off * per,
+                                     cross_bits ? cross_bits + off * words : nullptr, st);
+        if (rc) return rc;
+    }
+    return CTH_OK;
+}
+
+int cth_sample_spans(cth_ctx* c, const cth_shape* s, const cth_span* spans, size_t nspans, uint32_t res,
+                     float* dists, uint64_t* cross_bits) {
+    if (!c) return fail(CTH_E_INVALID, "null context");
+    if (int rc = check_shape(s)) return rc;
+    if (nspans == 0) return CTH_OK;
+    if (!spans || !dists) return fail(CTH_E_INVALID, "null buffer");
+    if (res == 0 || (res & (res - 1)) != 0) return fail(CTH_E_INVALID, "resolution must be a power of two");
+    if (res > 512) return fail(CTH_E_INVALID, "resolution above 512");
+    size_t per = size_t(res + 1) * (res + 1) * (res + 1);
+    size_t words = (size_t(res) * res * res + 63) / 64;
+    DevBuf dsp, dd, dc;
+    int rc = dsp.alloc(nspans * sizeof(cth_span));
+    if (!rc) rc = dd.alloc(nspans * per * sizeof(float));
+    if (!rc && cross_bits) rc = dc.alloc(nspans * words * sizeof(uint64_t));
+    if (rc) return rc;
+    rc = check_hip(hipMemcpyAsync(dsp.p, spans, nspans * sizeof(cth_span), hipMemcpyHostToDevice, c->stream), "H2D");
+    if (!rc)
+        rc = cth_sample_spans_dev(c, s, (const float*)dsp.p, nspans, res, (float*)dd.p,
+                                  cross_bits ? (uint64_t*)dc.p : nullptr, c->stream);
+    if (!rc)
+        rc = check_hip(hipMemcpyAsync(dists, dd.p, nspans * per * sizeof(float), hipMemcpyDeviceToHost, c->stream),
+                       "D2H");
+    if (!rc && cross_bits)
+        rc = check_hip(hipMemcpyAsync(cross_bits, dc.p, nspans * words * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                                      c->stream),
+                       "D2H");
+    int rs = check_hip(hipStreamSynchronize(c->stream), "sync");
+    return rc ? rc : rs;
+}
+
+// Host-pointer Shape-trait queries: copy in, run, copy out.
+enum Query { Q_MIN, Q_MAX, Q_BOUNDED, Q_CONTAINS };
+
+static int host_query(cth_ctx* c, const cth_shape* s, const cth_point3* pts, size_t n, void* out, Query q) {
+    if (!c) return fail(CTH_E_INVALID, "null context");
+    if (int rc = check_shape(s)) return rc;
+    if ((q == Q_MAX || q == Q_BOUNDED) && s->kind != CTH_SHAPE_SPHERE)
+        return fail(CTH_E_UNSUPPORTED, "this shape has no upper distance bound (max_distance_from is None)");
+    if (n == 0) return CTH_OK;
+    if (!pts || !out) return fail(CTH_E_INVALID, "null buffer");
+    if (n >= (size_t(1) << 31)) return fail(CTH_E_INVALID, "batch larger than 2^31");
+    uint32_t m = (uint32_t)n;
+    DevBuf dp, dd, dout;
+    int rc = dp.alloc(n * 12);
+    if (!rc) rc = dd.alloc(n * 4);
+    size_t out_bytes = (q == Q_BOUNDED) ? n * 8 : (q == Q_CONTAINS ? n : n * 4);
+    if (!rc && (q == Q_BOUNDED || q == Q_CONTAINS)) rc = dout.alloc(out_bytes);
+    if (rc) return rc;
+    rc = check_hip(hipMemcpyAsync(dp.p, pts, n * 12, hipMemcpyHostToDevice, c->stream), "H2D");
+    if (!rc) {
+        if (q == Q_CONTAINS && s->kind == CTH_SHAPE_SPHERE) {
+            rc = launch_contains_sphere(*c, *s, (const float*)dp.p, m, (uint8_t*)dout.p, c->stream);
+        } else {
+            rc = launch_de_points(*c, *s, (const float*)dp.p, nullptr, m, (float*)dd.p, nullptr, nullptr, nullptr,
+                                  c->stream);
+            if (!rc && q == Q_CONTAINS) rc = launch_less_than_zero(*c, (const float*)dd.p, m, (uint8_t*)dout.p, c->stream);
+            if (!rc && q == Q_BOUNDED) rc = launch_dup_pairs(*c, (const float*)dd.p, m, (float*)dout.p, c->stream);
+        }
+    }
+    const void* src = (q == Q_MIN || q == Q_MAX) ? dd.p : dout.p;
+    if (!rc) rc = check_hip(hipMemcpyAsync(out, src, out_bytes, hipMemcpyDeviceToHost, c->stream), "D2H");
+    int rs = check_hip(hipStreamSynchronize(c->stream), "sync");
+    return rc ? rc : rs;
+}
+
+int cth_min_distance_batch(cth_ctx* c, const cth_shape* s, const cth_point3* pts, size_t n, float* out) {
+    return host_query(c, s, pts, n, out, Q_MIN);
+}
+
+int cth_max_distance_batch(cth_ctx* c, const cth_shape* s, const cth_point3* pts, size_t n, float* out) {
+    return host_query(c, s, pts, n, out, Q_MAX);
+}
+
+int cth_bounded_distance_batch(cth_ctx* c, const cth_shape* s, const cth_point3* pts, size_t n, float* out_pairs) {
+    return host_query(c, s, pts, n, out_pairs, Q_BOUNDED);
+}
+
+int cth_contains_batch(cth_ctx* c, const cth_shape* s, const cth_point3* pts, size_t n, uint8_t* out) {
+    return host_query(c, s, pts, n, out, Q_CONTAINS);
+}
+
+}  // extern "C"

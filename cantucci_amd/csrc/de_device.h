@@ -1,0 +1,244 @@
+// Device-side Mandelbulb / sphere distance estimators for gfx950.
+//
+// Two arithmetic modes share one per-lane interface (start / mag / body /
+// finish), so the lane-refill work loop in work_loop.h can drive any of them:
+//
+//   * Exact evaluators (parity mode) reproduce the reference's f32 operation
+//     sequence (src/shape/mandelbulb.rs:59-79, 96-200): no FMA contraction
+//     (the library is compiled with -ffp-contract=off), IEEE-correct division
+//     and square root, denormals kept, the same association order.  The loop
+//     state (r, dr, iteration count) is therefore bit-identical to the
+//     reference; only the final `ln` (ocml instead of glibc) may differ by an
+//     ulp.  NaN results are canonicalised to the x86 default NaN (sign bit set)
+//     so that `is_sign_positive` classifies them exactly as the reference does.
+//
+//   * Fast evaluators (perf mode) compute the same map with explicit FMAs,
+//     shared subexpressions, hardware reciprocal / square root and the
+//     squared-radius bailout test.  They are not bit-exact (the chaotic tail of
+//     non-escaping points moves by more than 1e-5); their error is measured
+//     and reported, never used for parity claims.
+//
+// Packed-FP32 instructions are disabled for the whole library at compile time
+// (-Xclang -target-feature -Xclang -packed-fp32-ops).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "cantucci_hip.h"
+
+namespace cth {
+
+// ---------------------------------------------------------------------------
+// Bit-level helpers
+// ---------------------------------------------------------------------------
+
+// `Vec3::is_on_z_axis` (src/shape/mandelbulb.rs:421-426): x and y both +-0.
+__device__ __forceinline__ bool on_z_axis(float x, float y) {
+    return ((__float_as_uint(x) | __float_as_uint(y)) & 0x7fffffffu) == 0u;
+}
+
+// Every NaN the reference produces is the x86 default NaN 0xFFC00000 or a
+// propagation of it, so its sign bit is set ("inside" for is_sign_positive).
+__device__ __forceinline__ float canon_nan(float d) { return (d != d) ? __uint_as_float(0xffc00000u) : d; }
+
+// `f32::powi` with the LLVM ExpandPowI ladder: multiply the running square
+// into the result at each set bit, lowest first.  n >= 0.
+__device__ __forceinline__ float powi_ladder(float x, int n) {
+    if (n == 0) return 1.0f;
+    float cur = x, res = 1.0f;
+    bool have = false;
+    unsigned v = (unsigned)n;
+    while (v) {
+        if (v & 1u) {
+            res = have ? res * cur : cur;
+            have = true;
+        }
+        v >>= 1;
+        if (v) cur = cur * cur;
+    }
+    return res;
+}
+
+// ---------------------------------------------------------------------------
+// Evaluator parameters (uniform across a launch)
+// ---------------------------------------------------------------------------
+struct MandelUniform {
+    uint32_t max_iters;
+    float bailout;
+    float power;    // used when no per-point power is given
+    int ipower;     // integer power when `integral`
+    int integral;   // 1: every point uses ipower (reference Mandelbulb<P>);
+                    // 0: real power per point (powf unless integral and < 64)
+};
+
+// ---------------------------------------------------------------------------
+// Exact P8 polynomial evaluator (reference `Mandelbulb<8>`)
+// ---------------------------------------------------------------------------
+struct P8Exact {
+    float px, py, pz, x, y, z, dr;
+    float x2, y2, w2, z2;  // squares of the last magnitude, reused by rotate
+
+    __device__ __forceinline__ void start(float ax, float ay, float az, float, const MandelUniform&) {
+        px = x = ax;
+        py = y = ay;
+        pz = z = az;
+        dr = 1.0f;
+    }
+    // r = sqrt((x*x + y*y) + z*z) (:412-418).  x*x, y*y, x*x + y*y are the same
+    // IEEE operations as the rotate's x2, y2, rxy2 (:153-168), so they are
+    // computed once.
+    __device__ __forceinline__ float mag() {
+        x2 = x * x;
+        y2 = y * y;
+        w2 = x2 + y2;
+        z2 = z * z;
+        return __fsqrt_rn(w2 + z2);
+    }
+    __device__ __forceinline__ void body(float r, const MandelUniform&) {
+        // dr = r.powi(7) * 8.0 * dr + 1.0      (:73)
+        float r2 = r * r;
+        float r4 = r2 * r2;
+        float r7 = (r * r2) * r4;
+        dr = ((r7 * 8.0f) * dr) + 1.0f;
+        float nx, ny, nz;
+        if (on_z_axis(x, y)) {
+            // rotate_on_z_axis::<8> (:114-126)
+            float th = acosf(__fdiv_rn(z, r));
+            float r8 = r4 * r4;
+            nx = 0.0f;
+            ny = 0.0f;
+            nz = r8 * cosf(th * 8.0f);
+        } else {
+            // rotate_inner_p8_scalar (:148-200)
+            float x4 = x2 * x2, x6 = x4 * x2, x8 = x4 * x4;
+            float y4 = y2 * y2, y6 = y4 * y2, y8 = y4 * y4;
+            float z4 = z2 * z2, z6 = z4 * z2, z8 = z4 * z4;
+            float w4 = w2 * w2, w6 = w2 * w4, w8 = w4 * w4;
+            float num = ((z8 - (28.0f * z6) * w2) + (70.0f * z4) * w4) - (28.0f * z2) * w6;
+            float a = 1.0f + __fdiv_rn(num, w8);
+            nx = a * ((((x8 - (28.0f * x6) * y2) + (70.0f * x4) * y4) - (28.0f * x2) * y6) - y8);
+            ny = (((8.0f * a) * x) * y) * (((x6 - (7.0f * x4) * y2) + (7.0f * x2) * y4) - y6);
+            nz = (((8.0f * z) * __fsqrt_rn(w2)) * (z2 - w2)) * ((z4 - (6.0f * z2) * w2) + w4);
+        }
+        x = nx + px;
+        y = ny + py;
+        z = nz + pz;
+    }
+    // 0.5 * (r.ln() * r) / dr   (:77-78)
+    __device__ __forceinline__ float finish(float r) const { return canon_nan(__fdiv_rn(0.5f * (logf(r) * r), dr)); }
+};
+
+// ---------------------------------------------------------------------------
+// Exact trig evaluator (reference `Mandelbulb<P != 8>`, and the real-power
+// extension with powf).  Transcendentals are ocml's.
+// ---------------------------------------------------------------------------
+struct TrigExact {
+    float px, py, pz, x, y, z, dr, P;
+    int Pi;  // integer power for the powi ladder, or -1 for powf
+
+    // Integral powers use the powi ladder (the reference's const-generic u8
+    // power); the real-power kind uses powf unless the point's power happens
+    // to be an integer below 64 (the oracle applies the same rule).
+    __device__ __forceinline__ void start(float ax, float ay, float az, float power, const MandelUniform& u) {
+        px = x = ax;
+        py = y = ay;
+        pz = z = az;
+        dr = 1.0f;
+        P = power;
+        if (u.integral)
+            Pi = u.ipower;
+        else
+            Pi = (floorf(power) == power && fabsf(power) < 64.0f) ? (int)power : -1;
+    }
+    __device__ __forceinline__ float mag() const { return __fsqrt_rn((x * x + y * y) + z * z); }
+    __device__ __forceinline__ void body(float r, const MandelUniform&) {
+        float rp = (Pi >= 0) ? powi_ladder(r, Pi - 1) : powf(r, P - 1.0f);
+        dr = ((rp * P) * dr) + 1.0f;
+        float R = (Pi >= 0) ? powi_ladder(r, Pi) : powf(r, P);
+        float th = acosf(__fdiv_rn(z, r));
+        float nx, ny, nz;
+        if (on_z_axis(x, y)) {
+            nx = 0.0f;
+            ny = 0.0f;
+            nz = R * cosf(th * P);
+        } else {
+            float ph = atan2f(y, x);
+            th = th * P;
+            ph = ph * P;
+            float st = sinf(th), ct = cosf(th), sp = sinf(ph), cp = cosf(ph);
+            nx = (st * cp) * R;
+            ny = (sp * st) * R;
+            nz = ct * R;
+        }
+        x = nx + px;
+        y = ny + py;
+        z = nz + pz;
+    }
+    __device__ __forceinline__ float finish(float r) const { return canon_nan(__fdiv_rn(0.5f * (logf(r) * r), dr)); }
+};
+
+// ---------------------------------------------------------------------------
+// Fast P8 evaluator (perf mode).  Same map; FMAs, reciprocal, hardware sqrt.
+// ---------------------------------------------------------------------------
+struct P8Fast {
+    float px, py, pz, x, y, z, dr;
+    float x2, y2, w2, z2;
+
+    __device__ __forceinline__ void start(float ax, float ay, float az, float, const MandelUniform&) {
+        px = x = ax;
+        py = y = ay;
+        pz = z = az;
+        dr = 1.0f;
+    }
+    __device__ __forceinline__ float mag() {
+        x2 = x * x;
+        y2 = y * y;
+        w2 = x2 + y2;
+        z2 = z * z;
+        return __builtin_amdgcn_sqrtf(w2 + z2);
+    }
+    __device__ __forceinline__ void body(float r, const MandelUniform&) {
+        float r2 = r * r;
+        float r4 = r2 * r2;
+        float r7 = (r * r2) * r4;
+        dr = __builtin_fmaf(r7 * 8.0f, dr, 1.0f);
+        float nx, ny, nz;
+        if (on_z_axis(x, y)) {
+            float r8 = r4 * r4;
+            nx = 0.0f;
+            ny = 0.0f;
+            // z on the axis: cos(8 acos(+-1)) == 1 and z / r == +-1 unless r == 0.
+            nz = (z == 0.0f) ? __uint_as_float(0xffc00000u) : r8;
+        } else {
+            float x4 = x2 * x2, x6 = x4 * x2, x8 = x4 * x4;
+            float y4 = y2 * y2, y6 = y4 * y2, y8 = y4 * y4;
+            float z4 = z2 * z2, z6 = z4 * z2, z8 = z4 * z4;
+            float w4 = w2 * w2, w6 = w2 * w4, w8 = w4 * w4;
+            float num = __builtin_fmaf(-28.0f * z2, w6, __builtin_fmaf(70.0f * z4, w4, __builtin_fmaf(-28.0f * z6, w2, z8)));
+            float a = __builtin_fmaf(num, __builtin_amdgcn_rcpf(w8), 1.0f);
+            float px8 = __builtin_fmaf(-28.0f * x2, y6, __builtin_fmaf(70.0f * x4, y4, __builtin_fmaf(-28.0f * x6, y2, x8))) - y8;
+            float py6 = __builtin_fmaf(7.0f * x2, y4, __builtin_fmaf(-7.0f * x4, y2, x6)) - y6;
+            float pz4 = __builtin_fmaf(-6.0f * z2, w2, z4) + w4;
+            nx = a * px8;
+            ny = ((8.0f * a) * (x * y)) * py6;
+            nz = ((8.0f * z) * __builtin_amdgcn_sqrtf(w2)) * ((z2 - w2) * pz4);
+        }
+        x = nx + px;
+        y = ny + py;
+        z = nz + pz;
+    }
+    __device__ __forceinline__ float finish(float r) const {
+        return canon_nan((0.5f * (__logf(r) * r)) * __builtin_amdgcn_rcpf(dr));
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Sphere (src/shape/sphere.rs:33-35): |center - p| - radius.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float sphere_de(float px, float py, float pz, const float* c, float radius) {
+    float dx = c[0] - px, dy = c[1] - py, dz = c[2] - pz;
+    return __fsqrt_rn((dx * dx + dy * dy) + dz * dz) - radius;
+}
+
+}  // namespace cth

@@ -1,0 +1,64 @@
+// Internal host-side declarations shared by the C ABI and the kernel units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "cantucci_hip.h"
+#include "de_device.h"
+
+namespace cth {
+
+// Work counters: a ring of {next, done} pairs, zeroed at context creation.
+// Each launch takes the next pair, and the launch's last wave zeroes it again
+// (work_loop.h), so launches on different streams never share a pair unless
+// more than COUNTER_RING launches are in flight at once.
+constexpr uint32_t COUNTER_RING = 4096;
+
+struct LaunchShape {
+    int block = 256;        // threads per workgroup (4 waves)
+    int blocks_per_cu = 8;  // persistent workgroups per CU
+    uint32_t chunk = 256;   // indices claimed per wave atomic
+};
+
+struct Ctx {
+    int device = 0;
+    int num_cus = 0;
+    int mode = CTH_MODE_PARITY;
+    hipStream_t stream = nullptr;
+    uint32_t* counters = nullptr;  // 2 * COUNTER_RING
+    uint32_t ring_pos = 0;
+    LaunchShape shape_de;
+    // Next counter pair of the ring.
+    uint32_t* take_counters() {
+        uint32_t i = __atomic_fetch_add(&ring_pos, 1u, __ATOMIC_RELAXED) % COUNTER_RING;
+        return counters + 2 * i;
+    }
+    LaunchShape shape_trace;
+    LaunchShape shape_adj;
+};
+
+// Status helpers (capi.hip)
+int fail(int code, const std::string& msg);
+int check_hip(hipError_t e, const char* what);
+
+// Resolve a shape descriptor into kernel uniforms.  Returns CTH_OK or an error.
+int resolve_mandel(const cth_shape& s, MandelUniform& u, int& family);
+
+// Family codes used by the launchers.
+enum Family { FAM_P8 = 0, FAM_TRIG = 1, FAM_SPHERE = 2 };
+
+// Kernel launchers (kernels_de.hip).  All indices per launch are < 2^31.
+int launch_de_points(Ctx& c, const cth_shape& s, const float* pts, const float* powers, uint32_t n, float* out,
+                     uint32_t* iters, float* r_out, float* dr_out, hipStream_t st);
+int launch_de_grid(Ctx& c, const cth_shape& s, const float* start, const float* end, uint32_t n, float divisor,
+                   float* out, uint32_t* iters, hipStream_t st);
+int launch_sample_spans(Ctx& c, const cth_shape& s, const float* spans, uint32_t nspans, uint32_t res,
+                        float* dists, uint64_t* cross, hipStream_t st);
+int launch_contains_sphere(Ctx& c, const cth_shape& s, const float* pts, uint32_t n, uint8_t* out, hipStream_t st);
+int launch_less_than_zero(Ctx& c, const float* d, uint32_t n, uint8_t* out, hipStream_t st);
+int launch_dup_pairs(Ctx& c, const float* d, uint32_t n, float* pairs, hipStream_t st);
+
+}  // namespace cth

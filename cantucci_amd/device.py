@@ -1,0 +1,83 @@
+"""Device-tensor entry points: torch supplies HBM buffers and the stream, the
+C ABI's `_dev` functions do the work (asynchronously on torch's current
+stream)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+from . import _lib
+from ._lib import check, lib
+from .shape import Context, Shape
+
+
+def _ptr(t: torch.Tensor | None) -> int | None:
+    if t is None:
+        return None
+    if not t.is_cuda or not t.is_contiguous():
+        raise ValueError("expected a contiguous device tensor")
+    return t.data_ptr()
+
+
+def _stream() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _f32(t: torch.Tensor) -> torch.Tensor:
+    if t.dtype != torch.float32:
+        raise TypeError("expected float32")
+    return t
+
+
+def de_points(shape: Shape, pts: torch.Tensor, powers: torch.Tensor | None = None, with_iters: bool = False,
+              ctx: Context | None = None):
+    """Distances of (n, 3) f32 points; optionally the per-point iteration count."""
+    ctx = ctx or shape.ctx
+    pts = _f32(pts).contiguous()
+    n = pts.shape[0]
+    out = torch.empty(n, dtype=torch.float32, device=pts.device)
+    iters = torch.empty(n, dtype=torch.int32, device=pts.device) if with_iters else None
+    check(lib.cth_min_distance_dev(ctx.handle, C.byref(shape.desc), _ptr(pts), _ptr(powers), n, _ptr(out),
+                                   _ptr(iters), _stream()))
+    return (out, iters) if with_iters else out
+
+
+def de_state(shape: Shape, pts: torch.Tensor, powers: torch.Tensor | None = None, ctx: Context | None = None):
+    """(r, dr, iters): the loop state the distance is formed from."""
+    ctx = ctx or shape.ctx
+    pts = _f32(pts).contiguous()
+    n = pts.shape[0]
+    r = torch.empty(n, dtype=torch.float32, device=pts.device)
+    dr = torch.empty_like(r)
+    it = torch.empty(n, dtype=torch.int32, device=pts.device)
+    check(lib.cth_de_state_dev(ctx.handle, C.byref(shape.desc), _ptr(pts), _ptr(powers), n, _ptr(r), _ptr(dr),
+                               _ptr(it), _stream()))
+    return r, dr, it
+
+
+def de_grid(shape: Shape, start, end, n: int, divisor: float, with_iters: bool = False, device="cuda",
+            ctx: Context | None = None):
+    """n^3 grid in GridTable order (x slowest, z fastest)."""
+    ctx = ctx or shape.ctx
+    out = torch.empty(n * n * n, dtype=torch.float32, device=device)
+    iters = torch.empty(n * n * n, dtype=torch.int32, device=device) if with_iters else None
+    s = (C.c_float * 3)(*start)
+    e = (C.c_float * 3)(*end)
+    check(lib.cth_de_grid_dev(ctx.handle, C.byref(shape.desc), s, e, n, float(divisor), _ptr(out), _ptr(iters),
+                              _stream()))
+    return (out, iters) if with_iters else out
+
+
+def sample_spans(shape: Shape, spans: torch.Tensor, res: int, with_cross: bool = True, ctx: Context | None = None):
+    """Corner distances ((n, (res+1)^3) f32) and crossing words ((n, ceil(res^3/64)) int64)."""
+    ctx = ctx or shape.ctx
+    spans = _f32(spans).contiguous()
+    m = spans.shape[0]
+    per = (res + 1) ** 3
+    words = (res ** 3 + 63) // 64
+    dists = torch.empty((m, per), dtype=torch.float32, device=spans.device)
+    cross = torch.empty((m, words), dtype=torch.int64, device=spans.device) if with_cross else None
+    check(lib.cth_sample_spans_dev(ctx.handle, C.byref(shape.desc), _ptr(spans), m, res, _ptr(dists), _ptr(cross),
+                                   _stream()))
+    return dists, cross

@@ -1,0 +1,168 @@
+/*
+ * cantucci_hip.h -- C ABI of the MI355X (gfx950) Mandelbulb distance-estimation
+ * library.
+ *
+ * The entry points replace cantucci's `Shape` trait surface and the hot loops
+ * of its callers (citations are path:line in the cantucci source tree):
+ *
+ *   Shape::min_distance_from / batch_min_distance_from  src/shape/mod.rs:37,89
+ *   Shape::max_distance_from / batch_max_distance_from  src/shape/mod.rs:62-64,96
+ *   Shape::bounded_distance_from / batch_bounded_...    src/shape/mod.rs:73-75,103
+ *   Shape::contains                                     src/shape/mod.rs:78-80
+ *   Shape::bounding_box                                 src/shape/mod.rs:39
+ *   MeshBuffer step 1 (corner sampling) + sign test     src/mesh/buffer.rs:63-147
+ *   ShapeMesh::get_focii (sphere tracing)               src/mesh/mod.rs:205-243
+ *
+ * Conventions
+ *   - Plain C types only.  Host-pointer entry points are synchronous; `_dev`
+ *     entry points take device pointers and are asynchronous on `stream`
+ *     (a hipStream_t passed as void*; NULL selects the context's stream).
+ *   - Every call returns a cth_status; on failure cth_last_error() returns a
+ *     thread-local message.  Nothing aborts the process.  Where the reference
+ *     panics (e.g. `batch_max_distance_from` on a Mandelbulb,
+ *     src/shape/util.rs:8) the call returns CTH_E_UNSUPPORTED.
+ *   - The caller owns every buffer.  The library never frees caller memory.
+ *   - A context owns one HIP stream and small device work counters; calls on
+ *     one context must not overlap in time (use one context per host thread,
+ *     the analogue of the reference sharing `Arc<dyn Shape>` across its pool).
+ *   - There is no CPU fallback: without a usable gfx950 device,
+ *     cth_ctx_create fails with CTH_E_DEVICE.
+ */
+#ifndef CANTUCCI_HIP_H
+#define CANTUCCI_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum cth_status {
+    CTH_OK = 0,
+    CTH_E_INVALID = 1,     /* bad argument (null pointer, max_iters == 0, ...) */
+    CTH_E_UNSUPPORTED = 2, /* the reference would panic / has no such bound */
+    CTH_E_DEVICE = 3,      /* HIP runtime error */
+    CTH_E_NOMEM = 4        /* device allocation failed */
+} cth_status;
+
+/* Shape kinds.
+ *   CTH_SHAPE_MANDELBULB       reference `Mandelbulb<P>` (src/shape/mandelbulb.rs:13):
+ *                              `power` must be an integer in [1, 255]; P == 8 uses
+ *                              the closed-form polynomial (:148-200), any other P
+ *                              the trigonometric map with integer powers (:128-146).
+ *   CTH_SHAPE_MANDELBULB_TRIG  the trigonometric map with a real power (powf); the
+ *                              per-point power arrays of the gradient entry points
+ *                              apply to this kind.
+ *   CTH_SHAPE_SPHERE           reference `Sphere` (src/shape/sphere.rs:6-52). */
+enum {
+    CTH_SHAPE_MANDELBULB = 0,
+    CTH_SHAPE_MANDELBULB_TRIG = 1,
+    CTH_SHAPE_SPHERE = 2
+};
+
+/* Arithmetic modes of a context.
+ *   CTH_MODE_PARITY  the reference's f32 operation sequence (no FMA, IEEE div/sqrt):
+ *                    loop state bit-identical, final ln within an ulp.
+ *   CTH_MODE_FAST    FMAs, hardware reciprocal/sqrt; not bit-exact. */
+enum { CTH_MODE_PARITY = 0, CTH_MODE_FAST = 1 };
+
+/* Shape descriptor (the reference's type-level parameters made runtime):
+ * `Mandelbulb::new(max_iters, bailout)` (src/shape/mandelbulb.rs:19-26, asserts
+ * max_iters >= 1) and `Sphere::new(center, radius)` (src/shape/sphere.rs:13-19). */
+typedef struct cth_shape {
+    int32_t kind;
+    float power;
+    uint32_t max_iters;
+    float bailout;
+    float center[3];
+    float radius;
+} cth_shape;
+
+/* cgmath `Point3<f32>`: 12-byte AoS. */
+typedef struct cth_point3 {
+    float x, y, z;
+} cth_point3;
+
+/* octree `Span = Range<Point3<f32>>` (src/octree/mod.rs:13). */
+typedef struct cth_span {
+    float start[3];
+    float end[3];
+} cth_span;
+
+typedef struct cth_ctx cth_ctx;
+
+/* ---- context ------------------------------------------------------------ */
+int cth_ctx_create(int device, cth_ctx** out);
+void cth_ctx_destroy(cth_ctx* ctx);
+const char* cth_last_error(void);
+int cth_ctx_set_mode(cth_ctx* ctx, int mode);
+int cth_ctx_get_mode(const cth_ctx* ctx);
+void* cth_ctx_stream(cth_ctx* ctx);
+int cth_synchronize(cth_ctx* ctx);
+/* Number of compute units of the context's device. */
+int cth_ctx_num_cus(const cth_ctx* ctx);
+
+/* ---- Shape trait mirror (host buffers, synchronous) ------------------------ */
+
+/* Shape::bounding_box (src/shape/mod.rs:39): Mandelbulb [-1.2,1.2]^3
+ * (src/shape/mandelbulb.rs:53-57), Sphere center +- radius (src/shape/sphere.rs:27-30). */
+int cth_bounding_box(const cth_shape* shape, cth_span* out);
+
+/* Shape::batch_min_distance_from (src/shape/mod.rs:89, src/shape/util.rs:3-5). */
+int cth_min_distance_batch(cth_ctx* ctx, const cth_shape* shape, const cth_point3* pts, size_t n, float* out);
+
+/* Shape::batch_max_distance_from (src/shape/mod.rs:96, src/shape/util.rs:7-9):
+ * Sphere returns its exact distance (src/shape/sphere.rs:37-39); Mandelbulb has
+ * no upper bound (src/shape/mod.rs:62-64) -> CTH_E_UNSUPPORTED. */
+int cth_max_distance_batch(cth_ctx* ctx, const cth_shape* shape, const cth_point3* pts, size_t n, float* out);
+
+/* Shape::batch_bounded_distance_from (src/shape/mod.rs:103, src/shape/util.rs:11-16):
+ * out holds (min, max) pairs; CTH_E_UNSUPPORTED for Mandelbulb. */
+int cth_bounded_distance_batch(cth_ctx* ctx, const cth_shape* shape, const cth_point3* pts, size_t n,
+                               float* out_pairs);
+
+/* Shape::contains (src/shape/mod.rs:78-80: DE < 0; Sphere overrides with
+ * |c - p|^2 <= r^2, src/shape/sphere.rs:23-26).  One byte (0/1) per point. */
+int cth_contains_batch(cth_ctx* ctx, const cth_shape* shape, const cth_point3* pts, size_t n, uint8_t* out);
+
+/* ---- device-pointer entry points (asynchronous on `stream`) ------------------ */
+
+/* Distances of n AoS points (3 floats each).  `powers` (nullable) gives a
+ * per-point power for CTH_SHAPE_MANDELBULB_TRIG.  `iters` (nullable) receives
+ * the number of loop bodies each point executed (src/shape/mandelbulb.rs:65-75). */
+int cth_min_distance_dev(cth_ctx* ctx, const cth_shape* shape, const float* pts, const float* powers, size_t n,
+                         float* out, uint32_t* iters, void* stream);
+
+/* The loop state the distance is formed from: final r, dr and iteration count
+ * (src/shape/mandelbulb.rs:61-78).  Used to check bit-exactness of the loop. */
+int cth_de_state_dev(cth_ctx* ctx, const cth_shape* shape, const float* pts, const float* powers, size_t n,
+                     float* r, float* dr, uint32_t* iters, void* stream);
+
+/* Regular n^3 grid in GridTable order (x slowest, z fastest; src/util/grid.rs:22-51):
+ *   p(i,j,k) = start + (end - start) * ((i, j, k) / divisor)
+ * which is the reference's corner formula (src/mesh/buffer.rs:78-80) with
+ * `divisor` = resolution.  `iters` nullable. */
+int cth_de_grid_dev(cth_ctx* ctx, const cth_shape* shape, const float start[3], const float end[3], uint32_t n,
+                    float divisor, float* out, uint32_t* iters, void* stream);
+
+/* Octree-leaf span sampling + sign classification: naive surface nets step 1
+ * and its crossing test (src/mesh/buffer.rs:63-83, 113-147).  For each span the
+ * adjusted span s' = s - (e - s)/res, e' = e + (e - s)/res is sampled at
+ * (res+1)^3 corners (GridTable order) into dists[span*(res+1)^3 ...].  If
+ * cross_bits is non-null, bit c of span word block w = ceil(res^3/64) is set
+ * when cell c = (x*res + y)*res + z has corners of both IEEE signs.  `res`
+ * must be a power of two (src/mesh/buffer.rs:38-39).  `spans` holds 6 floats
+ * per span (start xyz, end xyz). */
+int cth_sample_spans_dev(cth_ctx* ctx, const cth_shape* shape, const float* spans, size_t nspans, uint32_t res,
+                         float* dists, uint64_t* cross_bits, void* stream);
+
+/* Host-buffer variant of cth_sample_spans_dev (synchronous). */
+int cth_sample_spans(cth_ctx* ctx, const cth_shape* shape, const cth_span* spans, size_t nspans, uint32_t res,
+                     float* dists, uint64_t* cross_bits);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CANTUCCI_HIP_H */

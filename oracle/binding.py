@@ -1,0 +1,248 @@
+"""ctypes binding of oracle/liboracle.so (test infrastructure only).
+
+Each wrapper names the reference function it restates; the C++ source
+(oracle.cpp) cites file:line for every operation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+K_MANDELBULB = 0
+K_MANDELBULB_TRIG = 1
+K_SPHERE = 2
+
+__all__ = [
+    "OrcShape", "mandelbulb", "mandelbulb_trig", "sphere", "de_points", "de_points_f64", "de_grad_f64",
+    "de_grid", "sample_spans", "surface_nets", "create_spans", "refine", "ray_dirs", "sphere_trace", "fd_diff",
+    "powi", "build", "K_MANDELBULB", "K_MANDELBULB_TRIG", "K_SPHERE", "default_threads",
+]
+
+
+class OrcShape(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("power", C.c_float),
+        ("max_iters", C.c_uint32),
+        ("bailout", C.c_float),
+        ("center", C.c_float * 3),
+        ("radius", C.c_float),
+    ]
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def _load() -> C.CDLL:
+    if not LIB_PATH.exists():
+        build()
+    return C.CDLL(str(LIB_PATH))
+
+
+_lib = _load()
+_P = C.c_void_p
+_S = C.c_size_t
+_U = C.c_uint32
+_F = C.c_float
+_I = C.c_int
+_SH = C.POINTER(OrcShape)
+for _name, _res, _args in [
+    ("orc_shape_size", _I, []),
+    ("orc_de_points_f32", None, [_SH, _P, _P, _S, _P, _P, _P, _P, _I]),
+    ("orc_de_points_f64", None, [_SH, _P, _P, _S, _P, _P, _I]),
+    ("orc_de_grad_f64", None, [_SH, _P, _P, _S, _P, _P, _P, _I]),
+    ("orc_de_grid_f32", None, [_SH, _P, _P, _U, _F, _P, _P, _I]),
+    ("orc_sample_spans", None, [_SH, _P, _S, _U, _P, _P, _P, _I]),
+    ("orc_surface_nets", None, [_SH, _P, _U, _P, _P, _S, _P, _S, C.POINTER(_S), C.POINTER(_S)]),
+    ("orc_create_spans", None, [_P, _P]),
+    ("orc_refine", _S, [_SH, _P, _U, _U, _P, _P, _S, _P, _I]),
+    ("orc_ray_dirs", None, [_P, _U, _U, _U, _U, _P]),
+    ("orc_sphere_trace", None, [_SH, _P, _P, _S, _I, _U, _F, _F, _P, _P, _I]),
+    ("orc_fd_diff", None, [_SH, _P, _P, _S, _F, _P, _I]),
+    ("orc_powi_f32", _F, [_F, _I]),
+]:
+    _fn = getattr(_lib, _name)
+    _fn.restype = _res
+    _fn.argtypes = _args
+
+
+def default_threads() -> int:
+    return int(os.environ.get("ORACLE_THREADS", os.cpu_count() or 1))
+
+
+def _shape(kind, power=8.0, max_iters=15, bailout=2.0, center=(0.0, 0.0, 0.0), radius=1.0) -> OrcShape:
+    s = OrcShape()
+    s.kind = kind
+    s.power = power
+    s.max_iters = max_iters
+    s.bailout = bailout
+    s.center[:] = list(center)
+    s.radius = radius
+    return s
+
+
+def mandelbulb(max_iters=15, bailout=2.0, power=8) -> OrcShape:
+    """`Mandelbulb<P>::new(max_iters, bailout)` (src/shape/mandelbulb.rs:19-26)."""
+    return _shape(K_MANDELBULB, float(power), max_iters, bailout)
+
+
+def mandelbulb_trig(max_iters=15, bailout=2.0, power=8.0) -> OrcShape:
+    """The trig map (src/shape/mandelbulb.rs:128-146) with a real power."""
+    return _shape(K_MANDELBULB_TRIG, float(power), max_iters, bailout)
+
+
+def sphere(center=(0.0, 0.0, 0.0), radius=1.0) -> OrcShape:
+    """`Sphere::new` (src/shape/sphere.rs:13-19)."""
+    return _shape(K_SPHERE, 0.0, 1, 0.0, center, radius)
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def de_points(shape: OrcShape, pts, powers=None, threads=None):
+    """`min_distance_from` over points (src/shape/mandelbulb.rs:59-79).
+    Returns (d, iters, r, dr)."""
+    pts = _f32(pts).reshape(-1, 3)
+    n = len(pts)
+    pw = None if powers is None else _f32(powers)
+    d = np.empty(n, np.float32)
+    it = np.empty(n, np.uint32)
+    r = np.empty(n, np.float32)
+    dr = np.empty(n, np.float32)
+    _lib.orc_de_points_f32(C.byref(shape), _ptr(pts), _ptr(pw), n, _ptr(d), _ptr(it), _ptr(r), _ptr(dr),
+                           threads or default_threads())
+    return d, it, r, dr
+
+
+def de_points_f64(shape: OrcShape, pts, powers=None, threads=None):
+    """The same map in FP64 (accuracy oracle).  Returns (d, iters)."""
+    pts = np.ascontiguousarray(np.asarray(pts, dtype=np.float64)).reshape(-1, 3)
+    n = len(pts)
+    pw = None if powers is None else _f32(powers)
+    d = np.empty(n, np.float64)
+    it = np.empty(n, np.uint32)
+    _lib.orc_de_points_f64(C.byref(shape), _ptr(pts), _ptr(pw), n, _ptr(d), _ptr(it), threads or default_threads())
+    return d, it
+
+
+def de_grad_f64(shape: OrcShape, pts, powers=None, threads=None):
+    """FP64 forward-mode dual numbers: (d, grad[n,4] = dd/dp xyz, dd/dpower, iters)."""
+    pts = np.ascontiguousarray(np.asarray(pts, dtype=np.float64)).reshape(-1, 3)
+    n = len(pts)
+    pw = None if powers is None else _f32(powers)
+    d = np.empty(n, np.float64)
+    g = np.empty((n, 4), np.float64)
+    it = np.empty(n, np.uint32)
+    _lib.orc_de_grad_f64(C.byref(shape), _ptr(pts), _ptr(pw), n, _ptr(d), _ptr(g), _ptr(it),
+                         threads or default_threads())
+    return d, g, it
+
+
+def de_grid(shape: OrcShape, start, end, n: int, divisor: float, threads=None):
+    """Corner-formula grid (src/mesh/buffer.rs:78-83) in GridTable order."""
+    s = _f32(start)
+    e = _f32(end)
+    d = np.empty(n ** 3, np.float32)
+    it = np.empty(n ** 3, np.uint32)
+    _lib.orc_de_grid_f32(C.byref(shape), _ptr(s), _ptr(e), n, float(divisor), _ptr(d), _ptr(it),
+                         threads or default_threads())
+    return d, it
+
+
+def sample_spans(shape: OrcShape, spans, res: int, threads=None):
+    """Naive surface nets step 1 + sign classification per span
+    (src/mesh/buffer.rs:63-147).  Returns (dists[m,(res+1)^3], cross[m,words], iters)."""
+    sp = _f32(spans).reshape(-1, 6)
+    m = len(sp)
+    per = (res + 1) ** 3
+    words = (res ** 3 + 63) // 64
+    d = np.empty((m, per), np.float32)
+    cb = np.empty((m, words), np.uint64)
+    it = np.empty((m, per), np.uint32)
+    _lib.orc_sample_spans(C.byref(shape), _ptr(sp), m, res, _ptr(d), _ptr(cb), _ptr(it), threads or default_threads())
+    return d, cb, it
+
+
+def surface_nets(shape: OrcShape, span, res: int):
+    """The whole `naive_surface_nets` (src/mesh/buffer.rs:58-391).
+    Returns (dists, vertices[nv,7], indices)."""
+    sp = _f32(span).reshape(6)
+    nv = C.c_size_t()
+    ni = C.c_size_t()
+    d = np.empty((res + 1) ** 3, np.float32)
+    _lib.orc_surface_nets(C.byref(shape), _ptr(sp), res, _ptr(d), None, 0, None, 0, C.byref(nv), C.byref(ni))
+    v = np.empty((nv.value, 7), np.float32)
+    ix = np.empty(ni.value, np.uint32)
+    _lib.orc_surface_nets(C.byref(shape), _ptr(sp), res, _ptr(d), _ptr(v), nv.value, _ptr(ix), ni.value,
+                          C.byref(nv), C.byref(ni))
+    return d, v, ix
+
+
+def create_spans(span) -> np.ndarray:
+    """`create_spans` (src/octree/mod.rs:315-329): 8 child spans."""
+    sp = _f32(span).reshape(6)
+    out = np.empty((8, 6), np.float32)
+    _lib.orc_create_spans(_ptr(sp), _ptr(out))
+    return out
+
+
+def refine(shape: OrcShape, root, min_depth: int, max_depth: int, half_diag, threads=None, cap=None):
+    """Level-synchronous refinement; returns (leaves[m,6], level_counts)."""
+    r = _f32(root).reshape(6)
+    hd = _f32(half_diag)
+    lc = np.zeros(max_depth + 1, np.uint64)
+    total = _lib.orc_refine(C.byref(shape), _ptr(r), min_depth, max_depth, _ptr(hd), None, 0, _ptr(lc),
+                            threads or default_threads())
+    leaves = np.empty((total, 6), np.float32)
+    _lib.orc_refine(C.byref(shape), _ptr(r), min_depth, max_depth, _ptr(hd), _ptr(leaves), total, _ptr(lc),
+                    threads or default_threads())
+    return leaves, lc
+
+
+def ray_dirs(pose, W: int, H: int, row0: int = 0, rows: int | None = None) -> np.ndarray:
+    """Pinhole primary-ray directions for rows [row0, row0+rows)."""
+    p = _f32(pose).reshape(14)
+    rows = H - row0 if rows is None else rows
+    out = np.empty((rows * W, 3), np.float32)
+    _lib.orc_ray_dirs(_ptr(p), W, H, row0, rows, _ptr(out))
+    return out
+
+
+def sphere_trace(shape: OrcShape, origins, dirs, max_steps=256, eps_rel=1e-4, t_max=10.0, threads=None):
+    """Sphere tracing (src/mesh/mod.rs:229-241 with a relative hit test)."""
+    o = _f32(origins).reshape(-1, 3)
+    dd = _f32(dirs).reshape(-1, 3)
+    n = len(dd)
+    stride = 0 if len(o) == 1 else 1
+    depth = np.empty(n, np.float32)
+    steps = np.empty(n, np.uint32)
+    _lib.orc_sphere_trace(C.byref(shape), _ptr(o), _ptr(dd), n, stride, max_steps, float(eps_rel), float(t_max),
+                          _ptr(depth), _ptr(steps), threads or default_threads())
+    return depth, steps
+
+
+def fd_diff(shape: OrcShape, pts, h: float, powers=None, threads=None) -> np.ndarray:
+    """DE(p + h e_a) - DE(p - h e_a) per axis (src/mesh/buffer.rs:256-266 stencil)."""
+    p = _f32(pts).reshape(-1, 3)
+    pw = None if powers is None else _f32(powers)
+    out = np.empty((len(p), 3), np.float32)
+    _lib.orc_fd_diff(C.byref(shape), _ptr(p), _ptr(pw), len(p), float(h), _ptr(out), threads or default_threads())
+    return out
+
+
+def powi(x: float, n: int) -> float:
+    return _lib.orc_powi_f32(x, n)

@@ -1,0 +1,95 @@
+"""C-ABI checks that need no GPU: the library loads, exports every symbol the
+public header declares, and its host-only entry points behave."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared_symbols():
+    names = set()
+    for h in (ROOT / "include").glob("*.h"):
+        text = h.read_text()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(cth_[a-z0-9_]+)\s*\(", text):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_header_declares_entry_points():
+    names = declared_symbols()
+    assert "cth_min_distance_batch" in names and "cth_sample_spans_dev" in names
+    assert len(names) >= 15
+
+
+def test_library_exports_every_declared_symbol():
+    from cantucci_amd import _lib
+    missing = [n for n in declared_symbols() if not hasattr(_lib.lib, n)]
+    assert not missing, f"not exported: {missing}"
+
+
+def test_header_has_no_torch_or_cpp_types():
+    text = (ROOT / "include" / "cantucci_hip.h").read_text()
+    for bad in ("torch", "at::", "std::", "Tensor"):
+        assert bad not in text
+
+
+def test_struct_layout_matches_oracle():
+    import oracle
+    from cantucci_amd import _lib
+    assert C.sizeof(_lib.CthShape) == oracle.binding._lib.orc_shape_size() == 32
+    for f in ("kind", "power", "max_iters", "bailout", "center", "radius"):
+        assert getattr(_lib.CthShape, f).offset == getattr(oracle.OrcShape, f).offset
+
+
+def test_bounding_box_host_only():
+    from cantucci_amd import _lib
+    span = _lib.CthSpan()
+    s = _lib.make_shape(_lib.CTH_SHAPE_MANDELBULB, 8.0, 15, 2.0)
+    assert _lib.lib.cth_bounding_box(C.byref(s), C.byref(span)) == 0
+    assert list(span.start) == pytest.approx([-1.2] * 3) and list(span.end) == pytest.approx([1.2] * 3)
+    sp = _lib.make_shape(_lib.CTH_SHAPE_SPHERE, 0, 1, 0, (1.0, 2.0, 3.0), 0.5)
+    assert _lib.lib.cth_bounding_box(C.byref(sp), C.byref(span)) == 0
+    assert list(span.start) == [0.5, 1.5, 2.5] and list(span.end) == [1.5, 2.5, 3.5]
+
+
+def test_invalid_shapes_rejected_without_gpu():
+    from cantucci_amd import _lib
+    span = _lib.CthSpan()
+    bad_iters = _lib.make_shape(_lib.CTH_SHAPE_MANDELBULB, 8.0, 0, 2.0)  # assert!(max_iters >= 1)
+    assert _lib.lib.cth_bounding_box(C.byref(bad_iters), C.byref(span)) == _lib.CTH_E_INVALID
+    assert b"max_iters" in _lib.lib.cth_last_error()
+    bad_power = _lib.make_shape(_lib.CTH_SHAPE_MANDELBULB, 8.5, 15, 2.0)  # Mandelbulb<P: u8>
+    assert _lib.lib.cth_bounding_box(C.byref(bad_power), C.byref(span)) == _lib.CTH_E_INVALID
+    assert _lib.lib.cth_min_distance_batch(None, C.byref(bad_power), None, 0, None) == _lib.CTH_E_INVALID
+
+
+def test_context_creation_fails_loudly_without_gpu():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    from cantucci_amd import _lib
+    h = C.c_void_p()
+    rc = _lib.lib.cth_ctx_create(0, C.byref(h))
+    assert rc == _lib.CTH_E_DEVICE and not h.value
+
+
+def test_fastdiv_magic_numbers():
+    """Host-side magic numbers of the device's divide-by-constant: check the
+    formula the kernels use against exact division on sampled numerators."""
+    import numpy as np
+    for d in list(range(1, 300)) + [729, 4096, 16641, 2146689, 16777216]:
+        s = 0
+        while (1 << s) < d:
+            s += 1
+        m = ((((1 << s) - d) << 32) // d + 1) & 0xFFFFFFFF
+        if d == 1:
+            m = 0
+        n = np.unique(np.concatenate([np.arange(0, 5000, dtype=np.uint64),
+                                      np.random.default_rng(d).integers(0, 2 ** 31, 5000, dtype=np.uint64),
+                                      np.array([2 ** 31 - 1], np.uint64)]))
+        q = (((n * np.uint64(m)) >> np.uint64(32)) + n) >> np.uint64(s)
+        assert (q == n // np.uint64(d)).all(), d

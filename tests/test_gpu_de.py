@@ -1,0 +1,256 @@
+"""GPU parity of the distance-estimation kernels against the oracle.
+
+Bar: the P8 loop state (final r, dr and the iteration count) is bit-identical
+to the reference's f32 sequence; the distance differs only through the final
+`ln` (ocml vs glibc), at most a few ulp.  NaNs carry the x86 sign bit.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+F32 = np.float32
+C1 = dict(start=(-1.5, -1.5, -1.5), end=(1.5, 1.5, 1.5))
+
+
+def ulp_dist(a, b):
+    a = np.asarray(a, F32).view(np.int32).astype(np.int64)
+    b = np.asarray(b, F32).view(np.int32).astype(np.int64)
+    a = np.where(a < 0, np.int64(-2 ** 31) - a, a)
+    b = np.where(b < 0, np.int64(-2 ** 31) - b, b)
+    return np.abs(a - b)
+
+
+def same_bits(a, b):
+    return np.asarray(a, F32).view(np.uint32) == np.asarray(b, F32).view(np.uint32)
+
+
+def assert_de_parity(d_gpu, d_ref, max_ulp=4):
+    nan_g, nan_r = np.isnan(d_gpu), np.isnan(d_ref)
+    assert (nan_g == nan_r).all(), "NaN pattern differs"
+    # NaNs carry the x86 default-NaN sign bit
+    assert same_bits(d_gpu[nan_g], d_ref[nan_r]).all()
+    fin = ~nan_g
+    u = ulp_dist(d_gpu[fin], d_ref[fin])
+    assert u.max() <= max_ulp, f"max ulp {u.max()}"
+    rel = np.abs(d_gpu[fin].astype(np.float64) - d_ref[fin]) / np.maximum(np.abs(d_ref[fin].astype(np.float64)), 1e-30)
+    assert rel.max() < 1e-5
+
+
+def grid_points(n, divisor, start=C1["start"], end=C1["end"]):
+    """The reference's corner formula p = s + (e - s) * (i / divisor) in f32."""
+    t = np.arange(n, dtype=F32) / F32(divisor)
+    ax = [F32(start[a]) + (F32(end[a]) - F32(start[a])) * t for a in range(3)]
+    X, Y, Z = np.meshgrid(ax[0], ax[1], ax[2], indexing="ij")
+    return np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1).astype(F32)
+
+
+def gpu(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+@pytest.mark.parametrize("divisor", [63.0, 64.0])
+def test_c1_grid_p8_parity(ctx, torch_cuda, divisor):
+    """64^3 grid, P8, 15 iterations, bailout 2 (divisor 64 puts the origin and
+    64 z-axis points on the grid)."""
+    import cantucci_amd as ca
+    from cantucci_amd import device as dev
+    torch = torch_cuda
+    shape = ca.Mandelbulb.classic(15, 2.0, ctx=ctx)
+    d, it = dev.de_grid(shape, C1["start"], C1["end"], 64, divisor, with_iters=True)
+    torch.cuda.synchronize()
+    dref, itref = O.de_grid(O.mandelbulb(15, 2.0), C1["start"], C1["end"], 64, divisor)
+    assert (it.cpu().numpy().astype(np.uint32) == itref).all()
+    assert_de_parity(d.cpu().numpy(), dref)
+    # the loop state from explicit points with the same coordinates is bit-identical
+    pts = grid_points(64, divisor)
+    r, dr, it2 = dev.de_state(shape, gpu(torch, pts))
+    _, it_o, r_o, dr_o = O.de_points(O.mandelbulb(15, 2.0), pts)
+    assert (it2.cpu().numpy().astype(np.uint32) == it_o).all()
+    assert same_bits(r.cpu().numpy(), r_o).all()
+    assert same_bits(dr.cpu().numpy(), dr_o).all()
+
+
+def edge_points():
+    e = [[0, 0, 0], [-0.0, 0, 0], [0, -0.0, 0.5], [-0.0, -0.0, -0.7], [0, 0, 1e-30], [0, 0, 2.0], [2.0, 0, 0],
+         [0, 0, -1.1], [1e-20, 0, 0.3], [0, 1e-20, -0.3], [1e-38, 1e-38, 0.5], [1e-45, 0, 0.1], [3.0, 4.0, 0.0],
+         [1.0, 1.0, 1.0], [-1.2, 1.2, -1.2], [0.5, 0.5, 0.0], [1e-3, -1e-3, 1e-3], [0.9, 0.0, -0.0],
+         [1.4142135, 1.4142135, 0.0], [np.inf, 0, 0], [1e30, 1e30, 1e30]]
+    return np.array(e, F32)
+
+
+@pytest.mark.parametrize("params", [(15, 2.0), (8, 5.0), (6, 2.5), (1, 2.0), (100, 2.0)])
+def test_points_p8_state_bitexact(ctx, torch_cuda, params):
+    import cantucci_amd as ca
+    from cantucci_amd import device as dev
+    from golden_data import BENCH_POINTS
+    torch = torch_cuda
+    mi, bo = params
+    rng = np.random.default_rng(100 + mi)
+    pts = np.concatenate([rng.uniform(-1.5, 1.5, (200000, 3)).astype(F32), edge_points(), BENCH_POINTS])
+    shape = ca.Mandelbulb.classic(mi, bo, ctx=ctx)
+    g = gpu(torch, pts)
+    d, it = dev.de_points(shape, g, with_iters=True)
+    r, dr, it2 = dev.de_state(shape, g)
+    torch.cuda.synchronize()
+    d_o, it_o, r_o, dr_o = O.de_points(O.mandelbulb(mi, bo), pts)
+    assert (it.cpu().numpy().astype(np.uint32) == it_o).all()
+    assert (it2.cpu().numpy().astype(np.uint32) == it_o).all()
+    rg, drg = r.cpu().numpy(), dr.cpu().numpy()
+    nan = np.isnan(r_o)
+    assert (np.isnan(rg) == nan).all() and same_bits(rg[~nan], r_o[~nan]).all()
+    nan = np.isnan(dr_o)
+    assert (np.isnan(drg) == nan).all() and same_bits(drg[~nan], dr_o[~nan]).all()
+    assert_de_parity(d.cpu().numpy(), d_o)
+
+
+@pytest.mark.parametrize("P", [2, 3, 5, 9])
+def test_trig_integer_power(ctx, torch_cuda, P):
+    """Mandelbulb<P != 8>: the trig map with ocml transcendentals.  Iteration
+    counts must agree except at a bailout boundary; distances of points whose
+    orbit is well conditioned (the FP64 value is within 1e-6 of the f32
+    reference) agree to 1e-5."""
+    import cantucci_amd as ca
+    from cantucci_amd import device as dev
+    torch = torch_cuda
+    rng = np.random.default_rng(200 + P)
+    pts = rng.uniform(-1.5, 1.5, (100000, 3)).astype(F32)
+    shape = ca.Mandelbulb(15, 2.0, P, ctx=ctx)
+    d, it = dev.de_points(shape, gpu(torch, pts), with_iters=True)
+    torch.cuda.synchronize()
+    d, it = d.cpu().numpy(), it.cpu().numpy().astype(np.uint32)
+    d_o, it_o, _, _ = O.de_points(O.mandelbulb(15, 2.0, P), pts)
+    d64, _ = O.de_points_f64(O.mandelbulb(15, 2.0, P), pts)
+    rel = np.abs(d.astype(np.float64) - d_o) / np.maximum(np.abs(d_o.astype(np.float64)), 1e-30)
+    cond = np.abs(d_o - d64) <= 1e-6 * np.abs(d64)
+    print(f"P={P}: iters equal {np.mean(it == it_o):.5f}, rel>1e-5 all {np.mean(rel > 1e-5):.5f}, "
+          f"well-conditioned {cond.mean():.3f} of which rel>1e-5 {np.mean(rel[cond] > 1e-5):.6f}")
+    assert np.mean(it == it_o) > 0.995
+    assert np.mean(rel[cond] > 1e-5) < 1e-3
+
+
+def test_trig_real_power_per_point(ctx, torch_cuda):
+    import cantucci_amd as ca
+    from cantucci_amd import device as dev
+    from cantucci_amd.workloads import shell_points
+    torch = torch_cuda
+    pts, powers = shell_points(100000, seed=1)
+    shape = ca.MandelbulbTrig(15, 2.0, 8.0, ctx=ctx)
+    d, it = dev.de_points(shape, gpu(torch, pts), gpu(torch, powers), with_iters=True)
+    torch.cuda.synchronize()
+    d, it = d.cpu().numpy(), it.cpu().numpy().astype(np.uint32)
+    s = O.mandelbulb_trig(15, 2.0)
+    d_o, it_o, _, _ = O.de_points(s, pts, powers)
+    d64, _ = O.de_points_f64(s, pts, powers)
+    rel = np.abs(d.astype(np.float64) - d_o) / np.maximum(np.abs(d_o.astype(np.float64)), 1e-30)
+    cond = np.abs(d_o - d64) <= 1e-6 * np.abs(d64)
+    print(f"real power: iters equal {np.mean(it == it_o):.5f}, rel>1e-5 all {np.mean(rel > 1e-5):.5f}, "
+          f"well-conditioned {cond.mean():.3f} of which rel>1e-5 {np.mean(rel[cond] > 1e-5):.6f}")
+    assert np.mean(it == it_o) > 0.99
+    assert np.mean(rel[cond] > 1e-5) < 1e-3
+
+
+def test_sphere_exact(ctx, torch_cuda):
+    import cantucci_amd as ca
+    from cantucci_amd import device as dev
+    torch = torch_cuda
+    pts = np.random.default_rng(3).uniform(-2, 2, (10000, 3)).astype(F32)
+    shape = ca.Sphere((0.25, -0.5, 0.125), 0.75, ctx=ctx)
+    d = dev.de_points(shape, gpu(torch, pts)).cpu().numpy()
+    d_o = O.de_points(O.sphere((0.25, -0.5, 0.125), 0.75), pts)[0]
+    assert same_bits(d, d_o).all()
+
+
+def test_shape_trait_host_queries(ctx):
+    import cantucci_amd as ca
+    from golden_data import BENCH_POINTS
+    m = ca.Mandelbulb.classic(8, 5.0, ctx=ctx)
+    d = m.batch_min_distance_from(BENCH_POINTS)
+    d_o = O.de_points(O.mandelbulb(8, 5.0), BENCH_POINTS)[0]
+    assert_de_parity(d, d_o)
+    assert m.min_distance_from(BENCH_POINTS[3]) == d[3]
+    assert (m.batch_contains(BENCH_POINTS) == (d < 0)).all()
+    with pytest.raises(ca.CthError) as e:
+        m.batch_max_distance_from(BENCH_POINTS)
+    assert e.value.code == 2
+    with pytest.raises(ca.CthError):
+        m.batch_bounded_distance_from(BENCH_POINTS)
+    assert m.max_distance_from(BENCH_POINTS[0]) is None
+    s = ca.Sphere((0, 0, 0), 1.0, ctx=ctx)
+    pts = np.array([[0.5, 0, 0], [1.0, 0, 0], [2, 0, 0], [0, 0, 0]], F32)
+    assert list(s.batch_contains(pts)) == [True, True, False, True]
+    b = s.batch_bounded_distance_from(pts)
+    assert (b[:, 0] == b[:, 1]).all() and (s.batch_max_distance_from(pts) == b[:, 0]).all()
+    assert m.bounding_box() == ((-1.2, -1.2, -1.2), (1.2, 1.2, 1.2)) or np.allclose(m.bounding_box(), [[-1.2] * 3, [1.2] * 3])
+    assert len(m.batch_min_distance_from(np.zeros((0, 3), F32))) == 0
+
+
+def test_empty_and_invalid_device_calls(ctx, torch_cuda):
+    import cantucci_amd as ca
+    from cantucci_amd import device as dev
+    torch = torch_cuda
+    shape = ca.Mandelbulb.classic(15, 2.0, ctx=ctx)
+    assert dev.de_points(shape, torch.zeros((0, 3), device="cuda")).numel() == 0
+    with pytest.raises(ca.CthError):
+        dev.sample_spans(shape, torch.zeros((1, 6), device="cuda"), 6)  # res not a power of two
+
+
+def c2_leaf_sample(count=400, seed=0):
+    from cantucci_amd.workloads import half_diagonals
+    leaves, _ = O.refine(O.mandelbulb(15, 2.0), [-1.5, -1.5, -1.5, 1.5, 1.5, 1.5], 2, 6, half_diagonals(3.0, 6))
+    idx = np.random.default_rng(seed).choice(len(leaves), min(count, len(leaves)), replace=False)
+    return leaves[np.sort(idx)]
+
+
+@pytest.mark.parametrize("res", [8, 4, 16])
+def test_span_sampling_and_sign_bits(ctx, torch_cuda, res):
+    """Octree-leaf spans (depth-6 surface leaves of the C2 refinement): corner
+    distances at parity and crossing bits identical to the oracle."""
+    import cantucci_amd as ca
+    from cantucci_amd import device as dev
+    torch = torch_cuda
+    spans = c2_leaf_sample(300 if res <= 8 else 60)
+    shape = ca.Mandelbulb.classic(15, 2.0, ctx=ctx)
+    d, cb = dev.sample_spans(shape, gpu(torch, spans), res)
+    torch.cuda.synchronize()
+    d_o, cb_o, _ = O.sample_spans(O.mandelbulb(15, 2.0), spans, res)
+    assert_de_parity(d.cpu().numpy().ravel(), d_o.ravel())
+    assert (cb.cpu().numpy().view(np.uint64) == cb_o).all()
+
+
+def test_span_sampling_sphere_and_host_entry(ctx):
+    import cantucci_amd as ca
+    from cantucci_amd import _lib
+    import ctypes as C
+    spans = np.array([[-1, -1, -1, 1, 1, 1], [0, 0, 0, 0.5, 0.5, 0.5]], F32)
+    res = 8
+    per = (res + 1) ** 3
+    d = np.empty((2, per), F32)
+    cb = np.empty((2, 8), np.uint64)
+    shape = _lib.make_shape(_lib.CTH_SHAPE_SPHERE, 0, 1, 0, (0.0, 0.0, 0.0), 0.6)
+    _lib.check(_lib.lib.cth_sample_spans(ctx.handle, C.byref(shape), spans.ctypes.data, 2, res, d.ctypes.data,
+                                         cb.ctypes.data))
+    d_o, cb_o, _ = O.sample_spans(O.sphere((0, 0, 0), 0.6), spans, res)
+    assert same_bits(d, d_o).all() and (cb == cb_o).all()
+
+
+def test_fast_mode_error_is_reported(torch_cuda):
+    """Perf mode (FMA, hardware rcp/sqrt) is not bit-exact; its error on the
+    64^3 grid is measured here, never used for parity."""
+    import cantucci_amd as ca
+    from cantucci_amd import device as dev
+    torch = torch_cuda
+    fctx = ca.Context(0, "fast")
+    shape = ca.Mandelbulb.classic(15, 2.0, ctx=fctx)
+    d, it = dev.de_grid(shape, C1["start"], C1["end"], 64, 63.0, with_iters=True)
+    torch.cuda.synchronize()
+    dref, itref = O.de_grid(O.mandelbulb(15, 2.0), C1["start"], C1["end"], 64, 63.0)
+    d = d.cpu().numpy()
+    rel = np.abs(d.astype(np.float64) - dref) / np.maximum(np.abs(dref.astype(np.float64)), 1e-30)
+    frac = np.mean(rel > 1e-5)
+    print(f"fast mode: iters equal {np.mean(it.cpu().numpy() == itref):.4f}, rel>1e-5 {frac:.4f}, "
+          f"median rel {np.median(rel):.2e}")
+    assert np.median(rel) < 1e-5 and frac < 0.1
+    fctx.close()
