@@ -62,6 +62,14 @@ def _load() -> C.CDLL:
         raise ImportError(
             f"{LIB_PATH} is missing: build it with `python -m cantucci_amd.build` "
             "(there is no CPU fallback)")
+    # torch ships its own HIP runtime under the same SONAME (libamdhip64.so.7).
+    # Loading torch first makes the dynamic linker bind this library to that
+    # already-loaded runtime, so the process has one HIP runtime and torch's
+    # device buffers and streams are valid for our kernels.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     return C.CDLL(str(LIB_PATH))
 
 
