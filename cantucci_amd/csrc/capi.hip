@@ -57,7 +57,10 @@ static int check_shape(const cth_shape* s) {
     return resolve_mandel(*s, u, fam);
 }
 
-static hipStream_t pick_stream(Ctx* c, void* stream) { return stream ? (hipStream_t)stream : c->stream; }
+// `_dev` entry points run on the caller's stream; NULL is HIP's default
+// (null) stream, as everywhere in HIP.  The context's own stream serves the
+// host-pointer entry points.
+static hipStream_t pick_stream(Ctx*, void* stream) { return (hipStream_t)stream; }
 
 // RAII device buffer for the host-pointer entry points.
 struct DevBuf {

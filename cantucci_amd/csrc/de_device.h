@@ -38,6 +38,14 @@ __device__ __forceinline__ bool on_z_axis(float x, float y) {
     return ((__float_as_uint(x) | __float_as_uint(y)) & 0x7fffffffu) == 0u;
 }
 
+// IEEE-correct f32 square root and division.  The library is compiled with
+// -fhip-fp32-correctly-rounded-divide-sqrt, under which `/` and
+// __builtin_sqrtf lower to correctly rounded sequences.  (HIP's `__fsqrt_rn`
+// is the native approximate square root unless OCML_BASIC_ROUNDED_OPERATIONS
+// is defined, so it is not used.)
+__device__ __forceinline__ float sqrt_rn(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ float div_rn(float a, float b) { return a / b; }
+
 // Every NaN the reference produces is the x86 default NaN 0xFFC00000 or a
 // propagation of it, so its sign bit is set ("inside" for is_sign_positive).
 __device__ __forceinline__ float canon_nan(float d) { return (d != d) ? __uint_as_float(0xffc00000u) : d; }
@@ -93,7 +101,7 @@ struct P8Exact {
         y2 = y * y;
         w2 = x2 + y2;
         z2 = z * z;
-        return __fsqrt_rn(w2 + z2);
+        return sqrt_rn(w2 + z2);
     }
     __device__ __forceinline__ void body(float r, const MandelUniform&) {
         // dr = r.powi(7) * 8.0 * dr + 1.0      (:73)
@@ -104,7 +112,7 @@ struct P8Exact {
         float nx, ny, nz;
         if (on_z_axis(x, y)) {
             // rotate_on_z_axis::<8> (:114-126)
-            float th = acosf(__fdiv_rn(z, r));
+            float th = acosf(div_rn(z, r));
             float r8 = r4 * r4;
             nx = 0.0f;
             ny = 0.0f;
@@ -116,17 +124,17 @@ struct P8Exact {
             float z4 = z2 * z2, z6 = z4 * z2, z8 = z4 * z4;
             float w4 = w2 * w2, w6 = w2 * w4, w8 = w4 * w4;
             float num = ((z8 - (28.0f * z6) * w2) + (70.0f * z4) * w4) - (28.0f * z2) * w6;
-            float a = 1.0f + __fdiv_rn(num, w8);
+            float a = 1.0f + div_rn(num, w8);
             nx = a * ((((x8 - (28.0f * x6) * y2) + (70.0f * x4) * y4) - (28.0f * x2) * y6) - y8);
             ny = (((8.0f * a) * x) * y) * (((x6 - (7.0f * x4) * y2) + (7.0f * x2) * y4) - y6);
-            nz = (((8.0f * z) * __fsqrt_rn(w2)) * (z2 - w2)) * ((z4 - (6.0f * z2) * w2) + w4);
+            nz = (((8.0f * z) * sqrt_rn(w2)) * (z2 - w2)) * ((z4 - (6.0f * z2) * w2) + w4);
         }
         x = nx + px;
         y = ny + py;
         z = nz + pz;
     }
     // 0.5 * (r.ln() * r) / dr   (:77-78)
-    __device__ __forceinline__ float finish(float r) const { return canon_nan(__fdiv_rn(0.5f * (logf(r) * r), dr)); }
+    __device__ __forceinline__ float finish(float r) const { return canon_nan(div_rn(0.5f * (logf(r) * r), dr)); }
 };
 
 // ---------------------------------------------------------------------------
@@ -151,12 +159,12 @@ struct TrigExact {
         else
             Pi = (floorf(power) == power && fabsf(power) < 64.0f) ? (int)power : -1;
     }
-    __device__ __forceinline__ float mag() const { return __fsqrt_rn((x * x + y * y) + z * z); }
+    __device__ __forceinline__ float mag() const { return sqrt_rn((x * x + y * y) + z * z); }
     __device__ __forceinline__ void body(float r, const MandelUniform&) {
         float rp = (Pi >= 0) ? powi_ladder(r, Pi - 1) : powf(r, P - 1.0f);
         dr = ((rp * P) * dr) + 1.0f;
         float R = (Pi >= 0) ? powi_ladder(r, Pi) : powf(r, P);
-        float th = acosf(__fdiv_rn(z, r));
+        float th = acosf(div_rn(z, r));
         float nx, ny, nz;
         if (on_z_axis(x, y)) {
             nx = 0.0f;
@@ -175,7 +183,7 @@ struct TrigExact {
         y = ny + py;
         z = nz + pz;
     }
-    __device__ __forceinline__ float finish(float r) const { return canon_nan(__fdiv_rn(0.5f * (logf(r) * r), dr)); }
+    __device__ __forceinline__ float finish(float r) const { return canon_nan(div_rn(0.5f * (logf(r) * r), dr)); }
 };
 
 // ---------------------------------------------------------------------------
@@ -238,7 +246,7 @@ struct P8Fast {
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float sphere_de(float px, float py, float pz, const float* c, float radius) {
     float dx = c[0] - px, dy = c[1] - py, dz = c[2] - pz;
-    return __fsqrt_rn((dx * dx + dy * dy) + dz * dz) - radius;
+    return sqrt_rn((dx * dx + dy * dy) + dz * dz) - radius;
 }
 
 }  // namespace cth

@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "de_device.h"
+
 namespace cth {
 
 // Division of n < 2^31 by a runtime constant d via a 32-bit multiply-high:
@@ -59,7 +61,7 @@ struct SrcGrid {
     FastDiv n, n2;
     float power;
     __device__ __forceinline__ float coord(int ax, uint32_t i) const {
-        float v = (inv != 0.0f) ? (float)i * inv : __fdiv_rn((float)i, divisor);
+        float v = (inv != 0.0f) ? (float)i * inv : div_rn((float)i, divisor);
         return s[ax] + a[ax] * v;
     }
     __device__ __forceinline__ void get(uint32_t idx, float& x, float& y, float& z, float& p) const {

@@ -14,9 +14,10 @@
  *   ShapeMesh::get_focii (sphere tracing)               src/mesh/mod.rs:205-243
  *
  * Conventions
- *   - Plain C types only.  Host-pointer entry points are synchronous; `_dev`
- *     entry points take device pointers and are asynchronous on `stream`
- *     (a hipStream_t passed as void*; NULL selects the context's stream).
+ *   - Plain C types only.  Host-pointer entry points are synchronous (they use
+ *     the context's own stream); `_dev` entry points take device pointers and
+ *     are asynchronous on `stream` (a hipStream_t passed as void*; NULL is
+ *     HIP's default stream).
  *   - Every call returns a cth_status; on failure cth_last_error() returns a
  *     thread-local message.  Nothing aborts the process.  Where the reference
  *     panics (e.g. `batch_max_distance_from` on a Mandelbulb,
