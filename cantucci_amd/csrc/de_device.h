@@ -46,9 +46,17 @@ __device__ __forceinline__ bool on_z_axis(float x, float y) {
 __device__ __forceinline__ float sqrt_rn(float x) { return __builtin_sqrtf(x); }
 __device__ __forceinline__ float div_rn(float a, float b) { return a / b; }
 
-// Every NaN the reference produces is the x86 default NaN 0xFFC00000 or a
-// propagation of it, so its sign bit is set ("inside" for is_sign_positive).
-__device__ __forceinline__ float canon_nan(float d) { return (d != d) ? __uint_as_float(0xffc00000u) : d; }
+// NaN sign of the reference.  x86 SSE arithmetic creates the default NaN
+// 0xFFC00000 (sign set) and propagates NaN operands unchanged; glibc's acosf
+// returns +NaN (0x7FC00000) for arguments outside [-1, 1].  In the distance
+// estimator acosf(z / r) sees +-inf only when r underflows to 0 with z != 0,
+// which is then the orbit's first NaN.  Every other NaN source is arithmetic,
+// so a lane flag recording that one case fixes the sign of a NaN result
+// (`is_sign_positive` then classifies it exactly as the reference does).
+__device__ __forceinline__ float canon_nan(float d, bool pos) {
+    return (d != d) ? __uint_as_float(pos ? 0x7fc00000u : 0xffc00000u) : d;
+}
+__device__ __forceinline__ float canon_nan(float d) { return canon_nan(d, false); }
 
 // `f32::powi` with the LLVM ExpandPowI ladder: multiply the running square
 // into the result at each set bit, lowest first.  n >= 0.
@@ -86,12 +94,14 @@ struct MandelUniform {
 struct P8Exact {
     float px, py, pz, x, y, z, dr;
     float x2, y2, w2, z2;  // squares of the last magnitude, reused by rotate
+    bool pos_nan;          // the orbit's first NaN came from acosf(+-inf)
 
     __device__ __forceinline__ void start(float ax, float ay, float az, float, const MandelUniform&) {
         px = x = ax;
         py = y = ay;
         pz = z = az;
         dr = 1.0f;
+        pos_nan = false;
     }
     // r = sqrt((x*x + y*y) + z*z) (:412-418).  x*x, y*y, x*x + y*y are the same
     // IEEE operations as the rotate's x2, y2, rxy2 (:153-168), so they are
@@ -112,7 +122,9 @@ struct P8Exact {
         float nx, ny, nz;
         if (on_z_axis(x, y)) {
             // rotate_on_z_axis::<8> (:114-126)
-            float th = acosf(div_rn(z, r));
+            float c = div_rn(z, r);
+            if (__builtin_isinf(c)) pos_nan = true;
+            float th = acosf(c);
             float r8 = r4 * r4;
             nx = 0.0f;
             ny = 0.0f;
@@ -134,7 +146,9 @@ struct P8Exact {
         z = nz + pz;
     }
     // 0.5 * (r.ln() * r) / dr   (:77-78)
-    __device__ __forceinline__ float finish(float r) const { return canon_nan(div_rn(0.5f * (logf(r) * r), dr)); }
+    __device__ __forceinline__ float finish(float r) const {
+        return canon_nan(div_rn(0.5f * (logf(r) * r), dr), pos_nan);
+    }
 };
 
 // ---------------------------------------------------------------------------
@@ -144,6 +158,7 @@ struct P8Exact {
 struct TrigExact {
     float px, py, pz, x, y, z, dr, P;
     int Pi;  // integer power for the powi ladder, or -1 for powf
+    bool pos_nan;
 
     // Integral powers use the powi ladder (the reference's const-generic u8
     // power); the real-power kind uses powf unless the point's power happens
@@ -154,6 +169,7 @@ struct TrigExact {
         pz = z = az;
         dr = 1.0f;
         P = power;
+        pos_nan = false;
         if (u.integral)
             Pi = u.ipower;
         else
@@ -164,7 +180,9 @@ struct TrigExact {
         float rp = (Pi >= 0) ? powi_ladder(r, Pi - 1) : powf(r, P - 1.0f);
         dr = ((rp * P) * dr) + 1.0f;
         float R = (Pi >= 0) ? powi_ladder(r, Pi) : powf(r, P);
-        float th = acosf(div_rn(z, r));
+        float c = div_rn(z, r);
+        if (__builtin_isinf(c)) pos_nan = true;
+        float th = acosf(c);
         float nx, ny, nz;
         if (on_z_axis(x, y)) {
             nx = 0.0f;
@@ -183,7 +201,9 @@ struct TrigExact {
         y = ny + py;
         z = nz + pz;
     }
-    __device__ __forceinline__ float finish(float r) const { return canon_nan(div_rn(0.5f * (logf(r) * r), dr)); }
+    __device__ __forceinline__ float finish(float r) const {
+        return canon_nan(div_rn(0.5f * (logf(r) * r), dr), pos_nan);
+    }
 };
 
 // ---------------------------------------------------------------------------

@@ -39,6 +39,29 @@ def assert_de_parity(d_gpu, d_ref, max_ulp=4):
     assert rel.max() < 1e-5
 
 
+def assert_same_or_both_nan(a, b):
+    """Bit-identical, except that the sign of an intermediate NaN is not part
+    of the reference's observable behaviour (only the final distance's is)."""
+    na, nb = np.isnan(a), np.isnan(b)
+    assert (na == nb).all()
+    assert same_bits(a[~na], b[~nb]).all()
+
+
+def ulp_stable(shape, pts, d_ref, powers=None, tol=1e-6):
+    """Points whose f32 reference distance is determined to `tol` at the ulp
+    level: moving any coordinate by one ulp either way changes it by at most
+    tol relative.  Elsewhere the f32 orbit is chaotic and no implementation
+    with different (even correctly rounded) transcendentals can agree."""
+    ok = np.ones(len(pts), bool)
+    for ax in range(3):
+        for direction in (np.inf, -np.inf):
+            q = pts.copy()
+            q[:, ax] = np.nextafter(q[:, ax], F32(direction))
+            dq = O.de_points(shape, q, powers)[0]
+            ok &= np.abs(dq.astype(np.float64) - d_ref) <= tol * np.abs(d_ref.astype(np.float64))
+    return ok
+
+
 def grid_points(n, divisor, start=C1["start"], end=C1["end"]):
     """The reference's corner formula p = s + (e - s) * (i / divisor) in f32."""
     t = np.arange(n, dtype=F32) / F32(divisor)
@@ -69,8 +92,8 @@ def test_c1_grid_p8_parity(ctx, torch_cuda, divisor):
     r, dr, it2 = dev.de_state(shape, gpu(torch, pts))
     _, it_o, r_o, dr_o = O.de_points(O.mandelbulb(15, 2.0), pts)
     assert (it2.cpu().numpy().astype(np.uint32) == it_o).all()
-    assert same_bits(r.cpu().numpy(), r_o).all()
-    assert same_bits(dr.cpu().numpy(), dr_o).all()
+    assert_same_or_both_nan(r.cpu().numpy(), r_o)
+    assert_same_or_both_nan(dr.cpu().numpy(), dr_o)
 
 
 def edge_points():
@@ -124,9 +147,11 @@ def test_trig_integer_power(ctx, torch_cuda, P):
     d_o, it_o, _, _ = O.de_points(O.mandelbulb(15, 2.0, P), pts)
     d64, _ = O.de_points_f64(O.mandelbulb(15, 2.0, P), pts)
     rel = np.abs(d.astype(np.float64) - d_o) / np.maximum(np.abs(d_o.astype(np.float64)), 1e-30)
-    cond = np.abs(d_o - d64) <= 1e-6 * np.abs(d64)
+    cond = ulp_stable(O.mandelbulb(15, 2.0, P), pts, d_o)
     print(f"P={P}: iters equal {np.mean(it == it_o):.5f}, rel>1e-5 all {np.mean(rel > 1e-5):.5f}, "
-          f"well-conditioned {cond.mean():.3f} of which rel>1e-5 {np.mean(rel[cond] > 1e-5):.6f}")
+          f"ulp-stable {cond.mean():.3f} of which rel>1e-5 {np.mean(rel[cond] > 1e-5):.6f}, "
+          f"vs fp64 rel>1e-5 gpu {np.mean(np.abs(d - d64) > 1e-5 * np.abs(d64)):.5f} "
+          f"oracle {np.mean(np.abs(d_o - d64) > 1e-5 * np.abs(d64)):.5f}")
     assert np.mean(it == it_o) > 0.995
     assert np.mean(rel[cond] > 1e-5) < 1e-3
 
@@ -145,9 +170,11 @@ def test_trig_real_power_per_point(ctx, torch_cuda):
     d_o, it_o, _, _ = O.de_points(s, pts, powers)
     d64, _ = O.de_points_f64(s, pts, powers)
     rel = np.abs(d.astype(np.float64) - d_o) / np.maximum(np.abs(d_o.astype(np.float64)), 1e-30)
-    cond = np.abs(d_o - d64) <= 1e-6 * np.abs(d64)
+    cond = ulp_stable(s, pts, d_o, powers)
     print(f"real power: iters equal {np.mean(it == it_o):.5f}, rel>1e-5 all {np.mean(rel > 1e-5):.5f}, "
-          f"well-conditioned {cond.mean():.3f} of which rel>1e-5 {np.mean(rel[cond] > 1e-5):.6f}")
+          f"ulp-stable {cond.mean():.3f} of which rel>1e-5 {np.mean(rel[cond] > 1e-5):.6f}, "
+          f"vs fp64 rel>1e-5 gpu {np.mean(np.abs(d - d64) > 1e-5 * np.abs(d64)):.5f} "
+          f"oracle {np.mean(np.abs(d_o - d64) > 1e-5 * np.abs(d64)):.5f}")
     assert np.mean(it == it_o) > 0.99
     assert np.mean(rel[cond] > 1e-5) < 1e-3
 
