@@ -49,14 +49,20 @@ __device__ __forceinline__ float div_rn(float a, float b) { return a / b; }
 // NaN sign of the reference.  x86 SSE arithmetic creates the default NaN
 // 0xFFC00000 (sign set) and propagates NaN operands unchanged; glibc's acosf
 // returns +NaN (0x7FC00000) for arguments outside [-1, 1].  In the distance
-// estimator acosf(z / r) sees +-inf only when r underflows to 0 with z != 0,
-// which is then the orbit's first NaN.  Every other NaN source is arithmetic,
-// so a lane flag recording that one case fixes the sign of a NaN result
-// (`is_sign_positive` then classifies it exactly as the reference does).
+// estimator acosf(z / r) sees +-inf only when r underflows to 0 with z != 0;
+// that NaN is then the orbit's first and every later NaN of the orbit is a
+// propagation of it.  A NaN distance carries the orbit's sign when r or dr is
+// NaN at the end; otherwise it was created by the final formula itself
+// (ln 0 * 0) and is the default NaN.  `is_sign_positive` then classifies
+// every NaN exactly as the reference does.
 __device__ __forceinline__ float canon_nan(float d, bool pos) {
     return (d != d) ? __uint_as_float(pos ? 0x7fc00000u : 0xffc00000u) : d;
 }
 __device__ __forceinline__ float canon_nan(float d) { return canon_nan(d, false); }
+__device__ __forceinline__ float finish_exact(float r, float dr, bool orbit_pos_nan) {
+    float d = div_rn(0.5f * (logf(r) * r), dr);
+    return canon_nan(d, orbit_pos_nan && (r != r || dr != dr));
+}
 
 // `f32::powi` with the LLVM ExpandPowI ladder: multiply the running square
 // into the result at each set bit, lowest first.  n >= 0.
@@ -147,7 +153,7 @@ struct P8Exact {
     }
     // 0.5 * (r.ln() * r) / dr   (:77-78)
     __device__ __forceinline__ float finish(float r) const {
-        return canon_nan(div_rn(0.5f * (logf(r) * r), dr), pos_nan);
+        return finish_exact(r, dr, pos_nan);
     }
 };
 
@@ -202,7 +208,7 @@ struct TrigExact {
         z = nz + pz;
     }
     __device__ __forceinline__ float finish(float r) const {
-        return canon_nan(div_rn(0.5f * (logf(r) * r), dr), pos_nan);
+        return finish_exact(r, dr, pos_nan);
     }
 };
 

@@ -35,6 +35,9 @@ def assert_de_parity(d_gpu, d_ref, max_ulp=4):
     fin = ~nan_g
     u = ulp_dist(d_gpu[fin], d_ref[fin])
     assert u.max() <= max_ulp, f"max ulp {u.max()}"
+    inf = np.isinf(d_ref)
+    assert same_bits(d_gpu[inf], d_ref[inf]).all()
+    fin &= ~inf
     rel = np.abs(d_gpu[fin].astype(np.float64) - d_ref[fin]) / np.maximum(np.abs(d_ref[fin].astype(np.float64)), 1e-30)
     assert rel.max() < 1e-5
 
