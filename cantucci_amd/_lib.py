@@ -51,6 +51,17 @@ class CthCamera(C.Structure):
     ]
 
 
+class CthLaunch(C.Structure):
+    _fields_ = [("block", C.c_int32), ("blocks_per_cu", C.c_int32), ("chunk", C.c_uint32),
+                ("refill_min", C.c_uint32), ("dynamic", C.c_uint32)]
+
+
+CTH_KERNEL_DE = 0
+CTH_KERNEL_TRACE = 1
+CTH_KERNEL_ADJOINT = 2
+CTH_KERNEL_SPANS = 3
+
+
 class CthError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"cantucci_hip error {code}: {msg}")
@@ -91,6 +102,8 @@ _SIGS = {
     "cth_ctx_stream": (_P, [_P]),
     "cth_synchronize": (_I, [_P]),
     "cth_ctx_num_cus": (_I, [_P]),
+    "cth_ctx_set_launch": (_I, [_P, _I, C.POINTER(CthLaunch)]),
+    "cth_ctx_get_launch": (_I, [_P, _I, C.POINTER(CthLaunch)]),
     "cth_bounding_box": (_I, [_SH, C.POINTER(CthSpan)]),
     "cth_min_distance_batch": (_I, [_P, _SH, _P, _S, _P]),
     "cth_max_distance_batch": (_I, [_P, _SH, _P, _S, _P]),

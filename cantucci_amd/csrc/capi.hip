@@ -130,6 +130,34 @@ int cth_ctx_set_mode(cth_ctx* c, int mode) {
 
 int cth_ctx_get_mode(const cth_ctx* c) { return c ? c->mode : -1; }
 
+int cth_ctx_set_launch(cth_ctx* c, int kernel, const cth_launch* l) {
+    if (!c || !l) return fail(CTH_E_INVALID, "null argument");
+    LaunchShape* ls = c->launch_shape(kernel);
+    if (!ls) return fail(CTH_E_INVALID, "unknown kernel family");
+    if (!(l->block == 64 || l->block == 128 || l->block == 256)) return fail(CTH_E_INVALID, "block must be 64, 128 or 256");
+    if (l->blocks_per_cu < 1 || l->blocks_per_cu > 32) return fail(CTH_E_INVALID, "blocks_per_cu must be in [1, 32]");
+    if (l->chunk < 64 || l->chunk > 65536) return fail(CTH_E_INVALID, "chunk must be in [64, 65536]");
+    if (l->refill_min < 1 || l->refill_min > 64) return fail(CTH_E_INVALID, "refill_min must be in [1, 64]");
+    ls->block = l->block;
+    ls->blocks_per_cu = l->blocks_per_cu;
+    ls->chunk = l->chunk;
+    ls->refill_min = l->refill_min;
+    ls->dynamic = l->dynamic ? 1u : 0u;
+    return CTH_OK;
+}
+
+int cth_ctx_get_launch(const cth_ctx* c, int kernel, cth_launch* l) {
+    if (!c || !l) return fail(CTH_E_INVALID, "null argument");
+    LaunchShape* ls = const_cast<cth_ctx*>(c)->launch_shape(kernel);
+    if (!ls) return fail(CTH_E_INVALID, "unknown kernel family");
+    l->block = ls->block;
+    l->blocks_per_cu = ls->blocks_per_cu;
+    l->chunk = ls->chunk;
+    l->refill_min = ls->refill_min;
+    l->dynamic = ls->dynamic;
+    return CTH_OK;
+}
+
 void* cth_ctx_stream(cth_ctx* c) { return c ? (void*)c->stream : nullptr; }
 
 int cth_ctx_num_cus(const cth_ctx* c) { return c ? c->num_cus : 0; }

@@ -119,37 +119,44 @@ struct P8Exact {
         z2 = z * z;
         return sqrt_rn(w2 + z2);
     }
-    __device__ __forceinline__ void body(float r, const MandelUniform&) {
+    // One loop body for every lane of the wave, committed only where `live`
+    // (lanes without a live point compute on stale state and discard it).
+    // The z-axis branch is rare, so it runs behind a wave-uniform test.
+    __device__ __forceinline__ void body(float r, const MandelUniform&, bool live) {
         // dr = r.powi(7) * 8.0 * dr + 1.0      (:73)
         float r2 = r * r;
         float r4 = r2 * r2;
         float r7 = (r * r2) * r4;
-        dr = ((r7 * 8.0f) * dr) + 1.0f;
-        float nx, ny, nz;
-        if (on_z_axis(x, y)) {
-            // rotate_on_z_axis::<8> (:114-126)
-            float c = div_rn(z, r);
-            if (__builtin_isinf(c)) pos_nan = true;
-            float th = acosf(c);
-            float r8 = r4 * r4;
-            nx = 0.0f;
-            ny = 0.0f;
-            nz = r8 * cosf(th * 8.0f);
-        } else {
-            // rotate_inner_p8_scalar (:148-200)
-            float x4 = x2 * x2, x6 = x4 * x2, x8 = x4 * x4;
-            float y4 = y2 * y2, y6 = y4 * y2, y8 = y4 * y4;
-            float z4 = z2 * z2, z6 = z4 * z2, z8 = z4 * z4;
-            float w4 = w2 * w2, w6 = w2 * w4, w8 = w4 * w4;
-            float num = ((z8 - (28.0f * z6) * w2) + (70.0f * z4) * w4) - (28.0f * z2) * w6;
-            float a = 1.0f + div_rn(num, w8);
-            nx = a * ((((x8 - (28.0f * x6) * y2) + (70.0f * x4) * y4) - (28.0f * x2) * y6) - y8);
-            ny = (((8.0f * a) * x) * y) * (((x6 - (7.0f * x4) * y2) + (7.0f * x2) * y4) - y6);
-            nz = (((8.0f * z) * sqrt_rn(w2)) * (z2 - w2)) * ((z4 - (6.0f * z2) * w2) + w4);
+        float ndr = ((r7 * 8.0f) * dr) + 1.0f;
+        // rotate_inner_p8_scalar (:148-200)
+        float x4 = x2 * x2, x6 = x4 * x2, x8 = x4 * x4;
+        float y4 = y2 * y2, y6 = y4 * y2, y8 = y4 * y4;
+        float z4 = z2 * z2, z6 = z4 * z2, z8 = z4 * z4;
+        float w4 = w2 * w2, w6 = w2 * w4, w8 = w4 * w4;
+        float num = ((z8 - (28.0f * z6) * w2) + (70.0f * z4) * w4) - (28.0f * z2) * w6;
+        float a = 1.0f + div_rn(num, w8);
+        float nx = a * ((((x8 - (28.0f * x6) * y2) + (70.0f * x4) * y4) - (28.0f * x2) * y6) - y8);
+        float ny = (((8.0f * a) * x) * y) * (((x6 - (7.0f * x4) * y2) + (7.0f * x2) * y4) - y6);
+        float nz = (((8.0f * z) * sqrt_rn(w2)) * (z2 - w2)) * ((z4 - (6.0f * z2) * w2) + w4);
+        bool axis = live && on_z_axis(x, y);
+        if (__ballot(axis)) {
+            if (axis) {
+                // rotate_on_z_axis::<8> (:114-126)
+                float c = div_rn(z, r);
+                if (__builtin_isinf(c)) pos_nan = true;
+                float th = acosf(c);
+                float r8 = r4 * r4;
+                nx = 0.0f;
+                ny = 0.0f;
+                nz = r8 * cosf(th * 8.0f);
+            }
         }
-        x = nx + px;
-        y = ny + py;
-        z = nz + pz;
+        if (live) {
+            dr = ndr;
+            x = nx + px;
+            y = ny + py;
+            z = nz + pz;
+        }
     }
     // 0.5 * (r.ln() * r) / dr   (:77-78)
     __device__ __forceinline__ float finish(float r) const {
@@ -182,30 +189,33 @@ struct TrigExact {
             Pi = (floorf(power) == power && fabsf(power) < 64.0f) ? (int)power : -1;
     }
     __device__ __forceinline__ float mag() const { return sqrt_rn((x * x + y * y) + z * z); }
-    __device__ __forceinline__ void body(float r, const MandelUniform&) {
+    __device__ __forceinline__ void body(float r, const MandelUniform&, bool live) {
+        // dr = r.powi(P - 1) * (P as f32) * dr + 1.0   (:73)
         float rp = (Pi >= 0) ? powi_ladder(r, Pi - 1) : powf(r, P - 1.0f);
-        dr = ((rp * P) * dr) + 1.0f;
+        float ndr = ((rp * P) * dr) + 1.0f;
+        // rotate_inner_px_generic (:128-146); on the z-axis rotate_on_z_axis
+        // (:114-126) keeps only (0, 0, R cos(P theta)), the same z product.
         float R = (Pi >= 0) ? powi_ladder(r, Pi) : powf(r, P);
         float c = div_rn(z, r);
-        if (__builtin_isinf(c)) pos_nan = true;
         float th = acosf(c);
-        float nx, ny, nz;
+        float ph = atan2f(y, x);
+        th = th * P;
+        ph = ph * P;
+        float st = sinf(th), ct = cosf(th), sp = sinf(ph), cp = cosf(ph);
+        float nx = (st * cp) * R;
+        float ny = (sp * st) * R;
+        float nz = ct * R;
         if (on_z_axis(x, y)) {
             nx = 0.0f;
             ny = 0.0f;
-            nz = R * cosf(th * P);
-        } else {
-            float ph = atan2f(y, x);
-            th = th * P;
-            ph = ph * P;
-            float st = sinf(th), ct = cosf(th), sp = sinf(ph), cp = cosf(ph);
-            nx = (st * cp) * R;
-            ny = (sp * st) * R;
-            nz = ct * R;
         }
-        x = nx + px;
-        y = ny + py;
-        z = nz + pz;
+        if (live) {
+            if (__builtin_isinf(c)) pos_nan = true;
+            dr = ndr;
+            x = nx + px;
+            y = ny + py;
+            z = nz + pz;
+        }
     }
     __device__ __forceinline__ float finish(float r) const {
         return finish_exact(r, dr, pos_nan);
@@ -232,35 +242,35 @@ struct P8Fast {
         z2 = z * z;
         return __builtin_amdgcn_sqrtf(w2 + z2);
     }
-    __device__ __forceinline__ void body(float r, const MandelUniform&) {
+    __device__ __forceinline__ void body(float r, const MandelUniform&, bool live) {
         float r2 = r * r;
         float r4 = r2 * r2;
         float r7 = (r * r2) * r4;
-        dr = __builtin_fmaf(r7 * 8.0f, dr, 1.0f);
-        float nx, ny, nz;
+        float ndr = __builtin_fmaf(r7 * 8.0f, dr, 1.0f);
+        float x4 = x2 * x2, x6 = x4 * x2, x8 = x4 * x4;
+        float y4 = y2 * y2, y6 = y4 * y2, y8 = y4 * y4;
+        float z4 = z2 * z2, z6 = z4 * z2, z8 = z4 * z4;
+        float w4 = w2 * w2, w6 = w2 * w4, w8 = w4 * w4;
+        float num = __builtin_fmaf(-28.0f * z2, w6, __builtin_fmaf(70.0f * z4, w4, __builtin_fmaf(-28.0f * z6, w2, z8)));
+        float a = __builtin_fmaf(num, __builtin_amdgcn_rcpf(w8), 1.0f);
+        float px8 = __builtin_fmaf(-28.0f * x2, y6, __builtin_fmaf(70.0f * x4, y4, __builtin_fmaf(-28.0f * x6, y2, x8))) - y8;
+        float py6 = __builtin_fmaf(7.0f * x2, y4, __builtin_fmaf(-7.0f * x4, y2, x6)) - y6;
+        float pz4 = __builtin_fmaf(-6.0f * z2, w2, z4) + w4;
+        float nx = a * px8;
+        float ny = ((8.0f * a) * (x * y)) * py6;
+        float nz = ((8.0f * z) * __builtin_amdgcn_sqrtf(w2)) * ((z2 - w2) * pz4);
         if (on_z_axis(x, y)) {
-            float r8 = r4 * r4;
+            // z on the axis: cos(8 acos(+-1)) == 1 and z / r == +-1 unless r == 0.
             nx = 0.0f;
             ny = 0.0f;
-            // z on the axis: cos(8 acos(+-1)) == 1 and z / r == +-1 unless r == 0.
-            nz = (z == 0.0f) ? __uint_as_float(0xffc00000u) : r8;
-        } else {
-            float x4 = x2 * x2, x6 = x4 * x2, x8 = x4 * x4;
-            float y4 = y2 * y2, y6 = y4 * y2, y8 = y4 * y4;
-            float z4 = z2 * z2, z6 = z4 * z2, z8 = z4 * z4;
-            float w4 = w2 * w2, w6 = w2 * w4, w8 = w4 * w4;
-            float num = __builtin_fmaf(-28.0f * z2, w6, __builtin_fmaf(70.0f * z4, w4, __builtin_fmaf(-28.0f * z6, w2, z8)));
-            float a = __builtin_fmaf(num, __builtin_amdgcn_rcpf(w8), 1.0f);
-            float px8 = __builtin_fmaf(-28.0f * x2, y6, __builtin_fmaf(70.0f * x4, y4, __builtin_fmaf(-28.0f * x6, y2, x8))) - y8;
-            float py6 = __builtin_fmaf(7.0f * x2, y4, __builtin_fmaf(-7.0f * x4, y2, x6)) - y6;
-            float pz4 = __builtin_fmaf(-6.0f * z2, w2, z4) + w4;
-            nx = a * px8;
-            ny = ((8.0f * a) * (x * y)) * py6;
-            nz = ((8.0f * z) * __builtin_amdgcn_sqrtf(w2)) * ((z2 - w2) * pz4);
+            nz = (z == 0.0f) ? __uint_as_float(0xffc00000u) : r4 * r4;
         }
-        x = nx + px;
-        y = ny + py;
-        z = nz + pz;
+        if (live) {
+            dr = ndr;
+            x = nx + px;
+            y = ny + py;
+            z = nz + pz;
+        }
     }
     __device__ __forceinline__ float finish(float r) const {
         return canon_nan((0.5f * (__logf(r) * r)) * __builtin_amdgcn_rcpf(dr));

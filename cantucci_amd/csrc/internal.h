@@ -18,9 +18,11 @@ namespace cth {
 constexpr uint32_t COUNTER_RING = 4096;
 
 struct LaunchShape {
-    int block = 256;        // threads per workgroup (4 waves)
-    int blocks_per_cu = 8;  // persistent workgroups per CU
-    uint32_t chunk = 256;   // indices claimed per wave atomic
+    int block = 256;           // threads per workgroup (64, 128 or 256)
+    int blocks_per_cu = 16;    // persistent workgroups per CU
+    uint32_t chunk = 256;      // indices per chunk
+    uint32_t refill_min = 32;  // idle lanes that trigger a refill (1..64)
+    uint32_t dynamic = 0;      // 0: static interleaved chunks, 1: atomic claims
 };
 
 struct Ctx {
@@ -31,6 +33,7 @@ struct Ctx {
     uint32_t* counters = nullptr;  // 2 * COUNTER_RING
     uint32_t ring_pos = 0;
     LaunchShape shape_de;
+    LaunchShape shape_spans{256, 8, 1024, 32, 1};
     // Next counter pair of the ring.
     uint32_t* take_counters() {
         uint32_t i = __atomic_fetch_add(&ring_pos, 1u, __ATOMIC_RELAXED) % COUNTER_RING;
@@ -38,6 +41,15 @@ struct Ctx {
     }
     LaunchShape shape_trace;
     LaunchShape shape_adj;
+    LaunchShape* launch_shape(int kernel) {
+        switch (kernel) {
+            case CTH_KERNEL_DE: return &shape_de;
+            case CTH_KERNEL_TRACE: return &shape_trace;
+            case CTH_KERNEL_ADJOINT: return &shape_adj;
+            case CTH_KERNEL_SPANS: return &shape_spans;
+            default: return nullptr;
+        }
+    }
 };
 
 // Status helpers (capi.hip)

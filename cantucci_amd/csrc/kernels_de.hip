@@ -101,15 +101,17 @@ static WorkQueue make_queue(Ctx& c, uint32_t total, const LaunchShape& ls, uint3
     q.waves = blocks * waves_per_block;
     q.total = total;
     q.chunk = ls.chunk;
+    q.refill_min = ls.refill_min < 1 ? 1 : (ls.refill_min > 64 ? 64 : ls.refill_min);
+    q.dynamic = ls.dynamic ? 1u : 0u;
     return q;
 }
 
 template <class Src, class Sink>
 static int run_refill(Ctx& c, int family, const MandelUniform& u, const Src& src, const Sink& sink,
-                      uint32_t total, hipStream_t st) {
+                      uint32_t total, hipStream_t st, const LaunchShape& ls) {
     uint32_t blocks = 0;
-    WorkQueue q = make_queue(c, total, c.shape_de, blocks);
-    dim3 grid(blocks), block(c.shape_de.block);
+    WorkQueue q = make_queue(c, total, ls, blocks);
+    dim3 grid(blocks), block(ls.block);
     bool fast = c.mode == CTH_MODE_FAST;
     if (family == FAM_P8) {
         if (fast)
@@ -140,10 +142,10 @@ int launch_de_points(Ctx& c, const cth_shape& s, const float* pts, const float* 
     if (rc != CTH_OK) return rc;
     if (r_out) {
         SinkState sink{r_out, dr_out, iters};
-        return run_refill(c, family, u, src, sink, n, st);
+        return run_refill(c, family, u, src, sink, n, st, c.shape_de);
     }
     SinkDist sink{out, iters};
-    return run_refill(c, family, u, src, sink, n, st);
+    return run_refill(c, family, u, src, sink, n, st, c.shape_de);
 }
 
 static bool is_pow2_float(float v, float& inv) {
@@ -182,7 +184,7 @@ int launch_de_grid(Ctx& c, const cth_shape& s, const float* start, const float* 
     int rc = resolve_mandel(s, u, family);
     if (rc != CTH_OK) return rc;
     SinkDist sink{out, iters};
-    return run_refill(c, family, u, src, sink, (uint32_t)total, st);
+    return run_refill(c, family, u, src, sink, (uint32_t)total, st, c.shape_de);
 }
 
 int launch_sample_spans(Ctx& c, const cth_shape& s, const float* spans, uint32_t nspans, uint32_t res, float* dists,
@@ -212,7 +214,7 @@ int launch_sample_spans(Ctx& c, const cth_shape& s, const float* spans, uint32_t
         rc = resolve_mandel(s, u, family);
         if (rc != CTH_OK) return rc;
         SinkDist sink{dists, nullptr};
-        rc = run_refill(c, family, u, src, sink, (uint32_t)total, st);
+        rc = run_refill(c, family, u, src, sink, (uint32_t)total, st, c.shape_spans);
     }
     if (rc != CTH_OK || cross == nullptr) return rc;
     uint32_t res_log2 = 0;

@@ -24,6 +24,8 @@ struct WorkQueue {
     uint32_t waves;    // waves in the launch
     uint32_t total;    // indices in [0, total)
     uint32_t chunk;    // indices claimed per atomic
+    uint32_t refill_min;  // idle lanes that trigger a write-back + refill (1..64)
+    uint32_t dynamic;     // 1: claim chunks with an atomic; 0: static interleave
 };
 
 __device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
@@ -37,6 +39,20 @@ __device__ __forceinline__ uint32_t wave_claim(uint32_t* ctr, uint32_t amount) {
     uint32_t v = 0;
     if ((int)__lane_id() == leader) v = atomicAdd(ctr, amount);
     return (uint32_t)__builtin_amdgcn_readlane((int)v, leader);
+}
+
+// Wave index within the launch.
+__device__ __forceinline__ uint32_t wave_id() { return blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); }
+
+// Next chunk of the wave: either the next unclaimed chunk of the global
+// counter (dynamic) or, statically, chunks w, w + W, w + 2W, ... of the
+// index space (W waves), which needs no atomics and spreads every wave over
+// the whole domain so that slow and fast regions average out.
+__device__ __forceinline__ uint32_t next_chunk(const WorkQueue& q, uint32_t& round) {
+    if (q.dynamic) return wave_claim(q.next, q.chunk);
+    uint32_t b = (round * q.waves + wave_id()) * q.chunk;
+    ++round;
+    return b;
 }
 
 // Called once per wave after its last claim; the final wave zeroes both
@@ -57,70 +73,82 @@ __device__ __forceinline__ void wave_retire(const WorkQueue& q) {
 // and writing results through `sink`.
 //   src.get(i, x, y, z, power)
 //   sink.put(i, d, iters, r, dr)
+//
+// Each tick runs one loop body on every lane (results committed where the
+// lane's point is live).  A lane whose point finishes holds (r, dr) until at
+// least `q.refill_min` lanes of the wave are idle; then the held points are
+// written and the idle lanes take new indices.  Batching the refill keeps the
+// per-tick control flow to a few scalar instructions.
 template <class Ev, class Src, class Sink>
 __device__ __forceinline__ void refill_loop(const Src& src, const MandelUniform& u, const Sink& sink,
                                             const WorkQueue& q) {
     Ev ev;
-    bool has = false;  // the lane holds a point that needs another loop body
+    bool live = false;  // the lane's point needs another loop body
+    bool hold = false;  // the lane holds a finished point not yet written
     uint32_t idx = 0, k = 0;
     float r = 0.0f;
     uint32_t c_next = 0, c_end = 0;  // wave-uniform window of claimed indices
+    uint32_t round = 0;
     bool exhausted = false;
 
     for (;;) {
-        // Hand new points to empty lanes.  A point that bails before its first
-        // body (|p| > bailout) is finished on the spot and the lane refilled.
-        while (!exhausted) {
-            uint64_t need = __ballot(!has);
-            if (need == 0) break;
-            if (c_next >= c_end) {
-                uint32_t b = wave_claim(q.next, q.chunk);
-                if (b >= q.total) {
-                    exhausted = true;
-                    break;
-                }
-                c_next = b;
-                c_end = (q.total - b < q.chunk) ? q.total : b + q.chunk;
+        uint64_t idle = __ballot(!live);
+        if ((uint32_t)__popcll(idle) >= q.refill_min) {
+            if (hold) {
+                sink.put(idx, ev.finish(r), k, r, ev.dr);
+                hold = false;
             }
-            uint32_t avail = c_end - c_next;
-            uint32_t nneed = (uint32_t)__popcll(need);
-            if (!has) {
-                uint32_t rank = lane_rank(need);
-                if (rank < avail) {
-                    idx = c_next + rank;
-                    float px, py, pz, pw;
-                    src.get(idx, px, py, pz, pw);
-                    ev.start(px, py, pz, pw, u);
-                    k = 0;
-                    r = ev.mag();
-                    if (r > u.bailout) {
-                        sink.put(idx, ev.finish(r), 0u, r, ev.dr);
-                    } else {
-                        has = true;
+            // Hand new points to idle lanes.  A point that bails before its
+            // first body (|p| > bailout) is written on the spot and the lane
+            // refilled again.
+            while (!exhausted) {
+                uint64_t need = __ballot(!live);
+                if (need == 0) break;
+                if (c_next >= c_end) {
+                    uint32_t b = next_chunk(q, round);
+                    if (b >= q.total) {
+                        exhausted = true;
+                        break;
+                    }
+                    c_next = b;
+                    c_end = (q.total - b < q.chunk) ? q.total : b + q.chunk;
+                }
+                uint32_t avail = c_end - c_next;
+                uint32_t nneed = (uint32_t)__popcll(need);
+                if (!live) {
+                    uint32_t rank = lane_rank(need);
+                    if (rank < avail) {
+                        idx = c_next + rank;
+                        float px, py, pz, pw;
+                        src.get(idx, px, py, pz, pw);
+                        ev.start(px, py, pz, pw, u);
+                        k = 0;
+                        r = ev.mag();
+                        if (r > u.bailout) {
+                            sink.put(idx, ev.finish(r), 0u, r, ev.dr);
+                        } else {
+                            live = true;
+                        }
                     }
                 }
+                c_next += (nneed < avail) ? nneed : avail;
             }
-            c_next += (nneed < avail) ? nneed : avail;
+            if (__ballot(live) == 0) break;
         }
-        if (__ballot(has) == 0) break;
-        if (has) {
-            ev.body(r, u);
-            ++k;
-            bool done = (k == u.max_iters);
-            if (!done) {
-                float rn = ev.mag();
-                if (rn > u.bailout) done = true;
-                r = rn;
-            }
-            // At the iteration cap, r is the magnitude computed before the last
-            // body (the reference's off-by-one, src/shape/mandelbulb.rs:65-78).
-            if (done) {
-                sink.put(idx, ev.finish(r), k, r, ev.dr);
-                has = false;
-            }
+        ev.body(r, u, live);
+        k += live ? 1u : 0u;
+        bool cap = live && (k == u.max_iters);
+        float rn = ev.mag();
+        bool bail = live && !cap && (rn > u.bailout);
+        // At the iteration cap, r stays the magnitude computed before the last
+        // body (the reference's off-by-one, src/shape/mandelbulb.rs:65-78).
+        if (live && !cap) r = rn;
+        if (cap || bail) {
+            live = false;
+            hold = true;
         }
     }
-    wave_retire(q);
+    if (q.dynamic) wave_retire(q);
 }
 
 }  // namespace cth

@@ -31,6 +31,17 @@ class Context:
         check(lib.cth_ctx_set_mode(self.handle, m))
         self.mode = mode
 
+    def set_launch(self, kernel: int, block: int, blocks_per_cu: int, chunk: int, refill_min: int,
+                   dynamic: int = 0) -> None:
+        l = _lib.CthLaunch(block, blocks_per_cu, chunk, refill_min, dynamic)
+        check(lib.cth_ctx_set_launch(self.handle, kernel, C.byref(l)))
+
+    def get_launch(self, kernel: int) -> dict:
+        l = _lib.CthLaunch()
+        check(lib.cth_ctx_get_launch(self.handle, kernel, C.byref(l)))
+        return dict(block=l.block, blocks_per_cu=l.blocks_per_cu, chunk=l.chunk, refill_min=l.refill_min,
+                    dynamic=l.dynamic)
+
     @property
     def stream(self) -> int:
         return lib.cth_ctx_stream(self.handle) or 0

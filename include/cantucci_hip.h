@@ -104,6 +104,27 @@ int cth_synchronize(cth_ctx* ctx);
 /* Number of compute units of the context's device. */
 int cth_ctx_num_cus(const cth_ctx* ctx);
 
+/* Launch shape of a kernel family (the autotuner's knobs).  block is 64, 128
+ * or 256 threads; blocks_per_cu persistent workgroups per CU (1..32); chunk
+ * indices claimed per wave from the work counter (64..65536); refill_min the
+ * number of idle lanes (1..64) at which a wave writes finished points and
+ * takes new ones; dynamic = 1 claims chunks from an atomic counter, 0 walks
+ * chunks w, w+W, w+2W, ... statically (W waves).  No counterpart in the
+ * reference (its work is scheduled by a CPU thread pool,
+ * src/mesh/mod.rs:60-63,141-148). */
+typedef struct cth_launch {
+    int32_t block;
+    int32_t blocks_per_cu;
+    uint32_t chunk;
+    uint32_t refill_min;
+    uint32_t dynamic;
+} cth_launch;
+
+enum { CTH_KERNEL_DE = 0, CTH_KERNEL_TRACE = 1, CTH_KERNEL_ADJOINT = 2, CTH_KERNEL_SPANS = 3 };
+
+int cth_ctx_set_launch(cth_ctx* ctx, int kernel, const cth_launch* launch);
+int cth_ctx_get_launch(const cth_ctx* ctx, int kernel, cth_launch* launch);
+
 /* ---- Shape trait mirror (host buffers, synchronous) ------------------------ */
 
 /* Shape::bounding_box (src/shape/mod.rs:39): Mandelbulb [-1.2,1.2]^3
