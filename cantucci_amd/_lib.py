@@ -112,7 +112,8 @@ _SIGS = {
     "cth_min_distance_dev": (_I, [_P, _SH, _P, _P, _S, _P, _P, _P]),
     "cth_de_state_dev": (_I, [_P, _SH, _P, _P, _S, _P, _P, _P, _P]),
     "cth_de_grid_dev": (_I, [_P, _SH, _P, _P, _U, _F, _P, _P, _P]),
-    "cth_sample_spans_dev": (_I, [_P, _SH, _P, _S, _U, _P, _P, _P]),
+    "cth_sample_spans_dev": (_I, [_P, _SH, _P, _S, _U, _P, _P, _P, _P]),
+    "cth_refine_dev": (_I, [_P, _SH, _P, _U, _U, _P, _P, _S, C.POINTER(_S), _P, _P]),
     "cth_sample_spans": (_I, [_P, _SH, _P, _S, _U, _P, _P]),
 }
 

@@ -221,7 +221,7 @@ int cth_de_grid_dev(cth_ctx* c, const cth_shape* s, const float start[3], const 
 }
 
 int cth_sample_spans_dev(cth_ctx* c, const cth_shape* s, const float* spans, size_t nspans, uint32_t res,
-                         float* dists, uint64_t* cross_bits, void* stream) {
+                         float* dists, uint64_t* cross_bits, uint32_t* iters, void* stream) {
     if (!c) return fail(CTH_E_INVALID, "null context");
     if (int rc = check_shape(s)) return rc;
     if (nspans && (!spans || !dists)) return fail(CTH_E_INVALID, "null buffer");
@@ -235,10 +235,24 @@ int cth_sample_spans_dev(cth_ctx* c, const cth_shape* s, const float* spans, siz
     for (size_t off = 0; off < nspans; off += batch) {
         size_t m = (nspans - off) < batch ? (nspans - off) : batch;
         int rc = launch_sample_spans(*c, *s, spans + 6 * off, (uint32_t)m, res, dists + off * per,
-                                     cross_bits ? cross_bits + off * words : nullptr, st);
+                                     cross_bits ? cross_bits + off * words : nullptr,
+                                     iters ? iters + off * per : nullptr, st);
         if (rc) return rc;
     }
     return CTH_OK;
+}
+
+int cth_refine_dev(cth_ctx* c, const cth_shape* s, const float root[6], uint32_t min_depth, uint32_t max_depth,
+                   const float* half_diag, float* leaves, size_t cap, size_t* nleaves, uint64_t* level_counts,
+                   void* stream) {
+    if (!c) return fail(CTH_E_INVALID, "null context");
+    if (int rc = check_shape(s)) return rc;
+    if (!root || !half_diag || !nleaves) return fail(CTH_E_INVALID, "null argument");
+    if (max_depth > 20) return fail(CTH_E_INVALID, "max_depth above 20");
+    for (int a = 0; a < 3; ++a)
+        if (!(root[a] < root[3 + a])) return fail(CTH_E_INVALID, "root span must have start < end");
+    return refine_levels(*c, *s, root, min_depth, max_depth, half_diag, leaves, cap, nleaves, level_counts,
+                         pick_stream(c, stream));
 }
 
 int cth_sample_spans(cth_ctx* c, const cth_shape* s, const cth_span* spans, size_t nspans, uint32_t res,
@@ -259,7 +273,7 @@ int cth_sample_spans(cth_ctx* c, const cth_shape* s, const cth_span* spans, size
     rc = check_hip(hipMemcpyAsync(dsp.p, spans, nspans * sizeof(cth_span), hipMemcpyHostToDevice, c->stream), "H2D");
     if (!rc)
         rc = cth_sample_spans_dev(c, s, (const float*)dsp.p, nspans, res, (float*)dd.p,
-                                  cross_bits ? (uint64_t*)dc.p : nullptr, c->stream);
+                                  cross_bits ? (uint64_t*)dc.p : nullptr, nullptr, c->stream);
     if (!rc)
         rc = check_hip(hipMemcpyAsync(dists, dd.p, nspans * per * sizeof(float), hipMemcpyDeviceToHost, c->stream),
                        "D2H");

@@ -68,7 +68,10 @@ int launch_de_points(Ctx& c, const cth_shape& s, const float* pts, const float* 
 int launch_de_grid(Ctx& c, const cth_shape& s, const float* start, const float* end, uint32_t n, float divisor,
                    float* out, uint32_t* iters, hipStream_t st);
 int launch_sample_spans(Ctx& c, const cth_shape& s, const float* spans, uint32_t nspans, uint32_t res,
-                        float* dists, uint64_t* cross, hipStream_t st);
+                        float* dists, uint64_t* cross, uint32_t* iters, hipStream_t st);
+int refine_levels(Ctx& c, const cth_shape& s, const float* root, uint32_t min_depth, uint32_t max_depth,
+                  const float* half_diag, float* leaves, size_t cap, size_t* nleaves, uint64_t* level_counts,
+                  hipStream_t st);
 int launch_contains_sphere(Ctx& c, const cth_shape& s, const float* pts, uint32_t n, uint8_t* out, hipStream_t st);
 int launch_less_than_zero(Ctx& c, const float* d, uint32_t n, uint8_t* out, hipStream_t st);
 int launch_dup_pairs(Ctx& c, const float* d, uint32_t n, float* pairs, hipStream_t st);

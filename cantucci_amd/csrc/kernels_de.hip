@@ -6,15 +6,11 @@
 #include <algorithm>
 
 #include "internal.h"
+#include "launch.h"
 #include "point_sources.h"
 #include "work_loop.h"
 
 namespace cth {
-
-template <class Ev, class Src, class Sink>
-__global__ __launch_bounds__(256) void k_refill(Src src, MandelUniform u, Sink sink, WorkQueue q) {
-    refill_loop<Ev>(src, u, sink, q);
-}
 
 // Sphere: no iteration, one point per thread.
 template <class Src>
@@ -89,41 +85,6 @@ __global__ void k_dup_pairs(const float* d, uint32_t n, float* pairs) {
 // Launch helpers
 // ---------------------------------------------------------------------------
 
-static WorkQueue make_queue(Ctx& c, uint32_t total, const LaunchShape& ls, uint32_t& blocks) {
-    uint32_t waves_per_block = (uint32_t)ls.block / 64u;
-    uint64_t want = (uint64_t)c.num_cus * (uint64_t)ls.blocks_per_cu;
-    uint64_t need = ((uint64_t)total + (uint64_t)ls.chunk * waves_per_block - 1) / ((uint64_t)ls.chunk * waves_per_block);
-    blocks = (uint32_t)std::max<uint64_t>(1, std::min(want, need));
-    WorkQueue q;
-    uint32_t* pair = c.take_counters();
-    q.next = pair;
-    q.done = pair + 1;
-    q.waves = blocks * waves_per_block;
-    q.total = total;
-    q.chunk = ls.chunk;
-    q.refill_min = ls.refill_min < 1 ? 1 : (ls.refill_min > 64 ? 64 : ls.refill_min);
-    q.dynamic = ls.dynamic ? 1u : 0u;
-    return q;
-}
-
-template <class Src, class Sink>
-static int run_refill(Ctx& c, int family, const MandelUniform& u, const Src& src, const Sink& sink,
-                      uint32_t total, hipStream_t st, const LaunchShape& ls) {
-    uint32_t blocks = 0;
-    WorkQueue q = make_queue(c, total, ls, blocks);
-    dim3 grid(blocks), block(ls.block);
-    bool fast = c.mode == CTH_MODE_FAST;
-    if (family == FAM_P8) {
-        if (fast)
-            hipLaunchKernelGGL((k_refill<P8Fast, Src, Sink>), grid, block, 0, st, src, u, sink, q);
-        else
-            hipLaunchKernelGGL((k_refill<P8Exact, Src, Sink>), grid, block, 0, st, src, u, sink, q);
-    } else {
-        hipLaunchKernelGGL((k_refill<TrigExact, Src, Sink>), grid, block, 0, st, src, u, sink, q);
-    }
-    return check_hip(hipGetLastError(), "refill kernel launch");
-}
-
 static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 int launch_de_points(Ctx& c, const cth_shape& s, const float* pts, const float* powers, uint32_t n, float* out,
@@ -188,7 +149,7 @@ int launch_de_grid(Ctx& c, const cth_shape& s, const float* start, const float* 
 }
 
 int launch_sample_spans(Ctx& c, const cth_shape& s, const float* spans, uint32_t nspans, uint32_t res, float* dists,
-                        uint64_t* cross, hipStream_t st) {
+                        uint64_t* cross, uint32_t* iters, hipStream_t st) {
     if (nspans == 0) return CTH_OK;
     if (res == 0 || (res & (res - 1)) != 0) return fail(CTH_E_INVALID, "resolution must be a power of two");
     if (res > 512) return fail(CTH_E_INVALID, "resolution above 512");
@@ -206,14 +167,14 @@ int launch_sample_spans(Ctx& c, const cth_shape& s, const float* spans, uint32_t
     int rc = CTH_OK;
     if (s.kind == CTH_SHAPE_SPHERE) {
         hipLaunchKernelGGL((k_sphere<SrcSpans>), dim3(cdiv(total, 256)), dim3(256), 0, st, src, (uint32_t)total,
-                           s.center[0], s.center[1], s.center[2], s.radius, dists, (uint32_t*)nullptr);
+                           s.center[0], s.center[1], s.center[2], s.radius, dists, iters);
         rc = check_hip(hipGetLastError(), "sphere span launch");
     } else {
         MandelUniform u;
         int family = 0;
         rc = resolve_mandel(s, u, family);
         if (rc != CTH_OK) return rc;
-        SinkDist sink{dists, nullptr};
+        SinkDist sink{dists, iters};
         rc = run_refill(c, family, u, src, sink, (uint32_t)total, st, c.shape_spans);
     }
     if (rc != CTH_OK || cross == nullptr) return rc;

@@ -1,0 +1,185 @@
+// Level-synchronous octree refinement on the GPU.
+//
+// The reference refines a CPU octree one leaf at a time (ShapeMesh::update,
+// src/mesh/mod.rs:82-178, child spans from create_spans,
+// src/octree/mod.rs:315-329).  Here a whole level is one batch: the centre
+// distance of every node at depth d is evaluated by the lane-refill work loop,
+// the split flags are prefix-summed, and every split node writes its 8
+// children (create_spans order) into the next level's array.  Parents stay in
+// level order, so the level-d array equals the oracle's breadth-first order.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <vector>
+
+#include "internal.h"
+#include "launch.h"
+#include "point_sources.h"
+#include "work_loop.h"
+
+namespace cth {
+
+// Node centre: start + (end - start) / 2 (SpanExt::center, src/octree/mod.rs:21-23);
+// halving is exact, so `* 0.5f` equals the reference's `/ 2.0`.
+struct SrcCenters {
+    const float* spans;
+    float power;
+    __device__ __forceinline__ void get(uint32_t i, float& x, float& y, float& z, float& p) const {
+        const float* sp = spans + 6 * (size_t)i;
+        x = sp[0] + (sp[3] - sp[0]) * 0.5f;
+        y = sp[1] + (sp[4] - sp[1]) * 0.5f;
+        z = sp[2] + (sp[5] - sp[2]) * 0.5f;
+        p = power;
+    }
+};
+
+// Split rule of the refinement policy: |DE(centre)| < half diagonal, or NaN.
+struct SinkSplit {
+    uint32_t* flags;
+    float half_diag;
+    __device__ __forceinline__ void put(uint32_t i, float d, uint32_t, float, float) const {
+        flags[i] = ((d != d) || (fabsf(d) < half_diag)) ? 1u : 0u;
+    }
+};
+
+__global__ void k_sphere_split(const float* nodes, uint32_t n, float cx, float cy, float cz, float radius,
+                               float half_diag, uint32_t* flags) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    SrcCenters src{nodes, 0.0f};
+    float x, y, z, p;
+    src.get(i, x, y, z, p);
+    const float c[3] = {cx, cy, cz};
+    float d = sphere_de(x, y, z, c, radius);
+    flags[i] = ((d != d) || (fabsf(d) < half_diag)) ? 1u : 0u;
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// One thread per (node, child): child c of a split node i lands at
+// 8 * offs[i] + c.  create_spans (src/octree/mod.rs:315-329): centre
+// = start + (end - start) / 2; child bit 2 = +x half, bit 1 = +y, bit 0 = +z.
+__global__ void k_emit_children(const float* __restrict__ nodes, const uint32_t* __restrict__ flags,
+                                const uint32_t* __restrict__ offs, uint32_t n, float* __restrict__ children) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t i = t >> 3, c = t & 7u;
+    if (i >= n || !flags[i]) return;
+    const float* sp = nodes + 6 * (size_t)i;
+    float* out = children + 6 * ((size_t)offs[i] * 8 + c);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float s = sp[a], e = sp[3 + a];
+        float ctr = s + (e - s) * 0.5f;
+        bool hi = (c >> (2 - a)) & 1u;
+        out[a] = hi ? ctr : s;
+        out[3 + a] = hi ? e : ctr;
+    }
+}
+
+__global__ void k_emit_leaves(const float* __restrict__ nodes, const uint32_t* __restrict__ flags,
+                              const uint32_t* __restrict__ offs, uint32_t n, float* __restrict__ leaves,
+                              uint64_t cap) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t i = t / 6, a = t - i * 6;
+    if (i >= n || !flags[i] || offs[i] >= cap) return;
+    leaves[6 * (size_t)offs[i] + a] = nodes[6 * (size_t)i + a];
+}
+
+static inline uint32_t cdiv32(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+int refine_levels(Ctx& c, const cth_shape& s, const float* root, uint32_t min_depth, uint32_t max_depth,
+                  const float* half_diag, float* leaves, size_t cap, size_t* nleaves, uint64_t* level_counts,
+                  hipStream_t st) {
+    MandelUniform u{};
+    int family = FAM_SPHERE;
+    if (s.kind != CTH_SHAPE_SPHERE) {
+        int rc = resolve_mandel(s, u, family);
+        if (rc) return rc;
+    }
+    const uint64_t kMaxNodes = 1ull << 28;
+    // Scratch: two node arrays, flags, offsets, scan temp.  Sized per level.
+    std::vector<void*> owned;
+    auto alloc = [&](size_t bytes, void** p) -> int {
+        int rc = check_hip(hipMallocAsync(p, bytes ? bytes : 4, st), "hipMallocAsync");
+        if (!rc) owned.push_back(*p);
+        return rc;
+    };
+    auto release = [&]() {
+        for (void* p : owned) (void)hipFreeAsync(p, st);
+        owned.clear();
+    };
+    float* cur = nullptr;
+    int rc = alloc(6 * sizeof(float), (void**)&cur);
+    if (!rc) rc = check_hip(hipMemcpyAsync(cur, root, 6 * sizeof(float), hipMemcpyHostToDevice, st), "H2D root");
+    uint64_t n = 1;
+    *nleaves = 0;
+    for (uint32_t depth = 0; depth <= max_depth && rc == CTH_OK; ++depth) {
+        if (level_counts) level_counts[depth] = n;
+        if (n == 0) {
+            for (uint32_t d2 = depth + 1; d2 <= max_depth && level_counts; ++d2) level_counts[d2] = 0;
+            break;
+        }
+        uint32_t* flags = nullptr;
+        uint32_t* offs = nullptr;
+        rc = alloc(n * sizeof(uint32_t), (void**)&flags);
+        if (!rc) rc = alloc(n * sizeof(uint32_t), (void**)&offs);
+        if (rc) break;
+        if (depth < min_depth) {
+            hipLaunchKernelGGL(k_fill_u32, dim3(cdiv32(n, 256)), dim3(256), 0, st, flags, (uint32_t)n, 1u);
+            rc = check_hip(hipGetLastError(), "fill launch");
+        } else if (family == FAM_SPHERE) {
+            hipLaunchKernelGGL(k_sphere_split, dim3(cdiv32(n, 256)), dim3(256), 0, st, cur, (uint32_t)n, s.center[0],
+                               s.center[1], s.center[2], s.radius, half_diag[depth], flags);
+            rc = check_hip(hipGetLastError(), "sphere split launch");
+        } else {
+            SrcCenters src{cur, s.power};
+            SinkSplit sink{flags, half_diag[depth]};
+            rc = run_refill(c, family, u, src, sink, (uint32_t)n, st, c.shape_spans);
+        }
+        if (rc) break;
+        size_t temp_bytes = 0;
+        rc = check_hip(hipcub::DeviceScan::ExclusiveSum(nullptr, temp_bytes, flags, offs, (int)n, st), "scan size");
+        void* temp = nullptr;
+        if (!rc) rc = alloc(temp_bytes, &temp);
+        if (!rc) rc = check_hip(hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, flags, offs, (int)n, st), "scan");
+        uint32_t last[2] = {0, 0};
+        if (!rc) rc = check_hip(hipMemcpyAsync(&last[0], offs + (n - 1), 4, hipMemcpyDeviceToHost, st), "D2H");
+        if (!rc) rc = check_hip(hipMemcpyAsync(&last[1], flags + (n - 1), 4, hipMemcpyDeviceToHost, st), "D2H");
+        if (!rc) rc = check_hip(hipStreamSynchronize(st), "sync");
+        if (rc) break;
+        uint64_t split = (uint64_t)last[0] + last[1];
+        if (depth == max_depth) {
+            *nleaves = split;
+            if (leaves && split) {
+                hipLaunchKernelGGL(k_emit_leaves, dim3(cdiv32(n * 6, 256)), dim3(256), 0, st, cur, flags, offs,
+                                   (uint32_t)n, leaves, (uint64_t)cap);
+                rc = check_hip(hipGetLastError(), "leaves launch");
+            }
+            break;
+        }
+        uint64_t next_n = split * 8;
+        if (next_n > kMaxNodes) {
+            rc = fail(CTH_E_INVALID, "refinement exceeds 2^28 nodes in one level");
+            break;
+        }
+        float* next = nullptr;
+        rc = alloc(next_n * 6 * sizeof(float) + 4, (void**)&next);
+        if (rc) break;
+        if (next_n) {
+            hipLaunchKernelGGL(k_emit_children, dim3(cdiv32(n * 8, 256)), dim3(256), 0, st, cur, flags, offs,
+                               (uint32_t)n, next);
+            rc = check_hip(hipGetLastError(), "children launch");
+        }
+        cur = next;
+        n = next_n;
+    }
+    int rs = check_hip(hipStreamSynchronize(st), "sync");
+    release();
+    int rf = check_hip(hipStreamSynchronize(st), "free sync");
+    return rc ? rc : (rs ? rs : rf);
+}
+
+}  // namespace cth

@@ -1,0 +1,54 @@
+// Launch plumbing shared by the kernel units: the lane-refill kernel
+// template and the persistent-grid sizing.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "internal.h"
+#include "work_loop.h"
+
+namespace cth {
+
+template <class Ev, class Src, class Sink>
+__global__ __launch_bounds__(256) void k_refill(Src src, MandelUniform u, Sink sink, WorkQueue q) {
+    refill_loop<Ev>(src, u, sink, q);
+}
+
+inline WorkQueue make_queue(Ctx& c, uint32_t total, const LaunchShape& ls, uint32_t& blocks) {
+    uint32_t waves_per_block = (uint32_t)ls.block / 64u;
+    uint64_t want = (uint64_t)c.num_cus * (uint64_t)ls.blocks_per_cu;
+    uint64_t need = ((uint64_t)total + (uint64_t)ls.chunk * waves_per_block - 1) / ((uint64_t)ls.chunk * waves_per_block);
+    blocks = (uint32_t)std::max<uint64_t>(1, std::min(want, need));
+    WorkQueue q;
+    uint32_t* pair = c.take_counters();
+    q.next = pair;
+    q.done = pair + 1;
+    q.waves = blocks * waves_per_block;
+    q.total = total;
+    q.chunk = ls.chunk;
+    q.refill_min = ls.refill_min < 1 ? 1 : (ls.refill_min > 64 ? 64 : ls.refill_min);
+    q.dynamic = ls.dynamic ? 1u : 0u;
+    return q;
+}
+
+template <class Src, class Sink>
+inline int run_refill(Ctx& c, int family, const MandelUniform& u, const Src& src, const Sink& sink,
+                      uint32_t total, hipStream_t st, const LaunchShape& ls) {
+    uint32_t blocks = 0;
+    WorkQueue q = make_queue(c, total, ls, blocks);
+    dim3 grid(blocks), block(ls.block);
+    bool fast = c.mode == CTH_MODE_FAST;
+    if (family == FAM_P8) {
+        if (fast)
+            hipLaunchKernelGGL((k_refill<P8Fast, Src, Sink>), grid, block, 0, st, src, u, sink, q);
+        else
+            hipLaunchKernelGGL((k_refill<P8Exact, Src, Sink>), grid, block, 0, st, src, u, sink, q);
+    } else {
+        hipLaunchKernelGGL((k_refill<TrigExact, Src, Sink>), grid, block, 0, st, src, u, sink, q);
+    }
+    return check_hip(hipGetLastError(), "refill kernel launch");
+}
+
+}  // namespace cth

@@ -69,15 +69,41 @@ def de_grid(shape: Shape, start, end, n: int, divisor: float, with_iters: bool =
     return (out, iters) if with_iters else out
 
 
-def sample_spans(shape: Shape, spans: torch.Tensor, res: int, with_cross: bool = True, ctx: Context | None = None):
-    """Corner distances ((n, (res+1)^3) f32) and crossing words ((n, ceil(res^3/64)) int64)."""
+def sample_spans(shape: Shape, spans: torch.Tensor, res: int, with_cross: bool = True, with_iters: bool = False,
+                 ctx: Context | None = None, out: torch.Tensor | None = None, cross: torch.Tensor | None = None):
+    """Corner distances ((n, (res+1)^3) f32), crossing words ((n, ceil(res^3/64))
+    int64) and optionally per-corner iteration counts."""
     ctx = ctx or shape.ctx
     spans = _f32(spans).contiguous()
     m = spans.shape[0]
     per = (res + 1) ** 3
     words = (res ** 3 + 63) // 64
-    dists = torch.empty((m, per), dtype=torch.float32, device=spans.device)
-    cross = torch.empty((m, words), dtype=torch.int64, device=spans.device) if with_cross else None
+    dists = out if out is not None else torch.empty((m, per), dtype=torch.float32, device=spans.device)
+    if with_cross and cross is None:
+        cross = torch.empty((m, words), dtype=torch.int64, device=spans.device)
+    if not with_cross:
+        cross = None
+    iters = torch.empty((m, per), dtype=torch.int32, device=spans.device) if with_iters else None
     check(lib.cth_sample_spans_dev(ctx.handle, C.byref(shape.desc), _ptr(spans), m, res, _ptr(dists), _ptr(cross),
-                                   _stream()))
+                                   _ptr(iters), _stream()))
+    if with_iters:
+        return dists, cross, iters
     return dists, cross
+
+
+def refine(shape: Shape, root, min_depth: int, max_depth: int, half_diag, cap: int | None = None,
+           leaves: torch.Tensor | None = None, ctx: Context | None = None, device="cuda"):
+    """Level-synchronous refinement; returns (leaves (m, 6) f32 on device, level_counts)."""
+    import numpy as np
+    ctx = ctx or shape.ctx
+    r = (C.c_float * 6)(*root)
+    hd = np.ascontiguousarray(np.asarray(half_diag, np.float32))
+    lc = np.zeros(max_depth + 1, np.uint64)
+    n = C.c_size_t()
+    if leaves is None:
+        check(lib.cth_refine_dev(ctx.handle, C.byref(shape.desc), r, min_depth, max_depth, hd.ctypes.data, None, 0,
+                                 C.byref(n), lc.ctypes.data, _stream()))
+        leaves = torch.empty((max(n.value, 1), 6), dtype=torch.float32, device=device)
+    check(lib.cth_refine_dev(ctx.handle, C.byref(shape.desc), r, min_depth, max_depth, hd.ctypes.data,
+                             _ptr(leaves), leaves.shape[0], C.byref(n), lc.ctypes.data, _stream()))
+    return leaves[:n.value], lc

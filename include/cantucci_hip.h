@@ -175,9 +175,25 @@ int cth_de_grid_dev(cth_ctx* ctx, const cth_shape* shape, const float start[3], 
  * cross_bits is non-null, bit c of span word block w = ceil(res^3/64) is set
  * when cell c = (x*res + y)*res + z has corners of both IEEE signs.  `res`
  * must be a power of two (src/mesh/buffer.rs:38-39).  `spans` holds 6 floats
- * per span (start xyz, end xyz). */
+ * per span (start xyz, end xyz).  `iters` (nullable) receives each corner's
+ * loop-body count, laid out like `dists`. */
 int cth_sample_spans_dev(cth_ctx* ctx, const cth_shape* shape, const float* spans, size_t nspans, uint32_t res,
-                         float* dists, uint64_t* cross_bits, void* stream);
+                         float* dists, uint64_t* cross_bits, uint32_t* iters, void* stream);
+
+/* Level-synchronous octree refinement (the octree-adaptive workload's policy;
+ * the reference splits leaves near camera focus points, src/mesh/mod.rs:91-104,
+ * after pre-splitting two levels, :52-56).  Starting from `root` (6 floats),
+ * every node of depth d < min_depth splits; deeper nodes split when
+ * |DE(centre)| < half_diag[d] or the distance is NaN; centre and children follow
+ * SpanExt::center and create_spans (src/octree/mod.rs:21-23, 315-329).  Nodes of
+ * max_depth that pass the test are the surface leaves: their count goes to
+ * *nleaves and, if `leaves` (device, 6 floats each) is non-null, the first
+ * `cap` of them in breadth-first create_spans order.  half_diag and
+ * level_counts (nullable; nodes per depth) are host arrays of max_depth + 1.
+ * Returns after the leaf count is known (kernels run on `stream`). */
+int cth_refine_dev(cth_ctx* ctx, const cth_shape* shape, const float root[6], uint32_t min_depth, uint32_t max_depth,
+                   const float* half_diag, float* leaves, size_t cap, size_t* nleaves, uint64_t* level_counts,
+                   void* stream);
 
 /* Host-buffer variant of cth_sample_spans_dev (synchronous). */
 int cth_sample_spans(cth_ctx* ctx, const cth_shape* shape, const cth_span* spans, size_t nspans, uint32_t res,

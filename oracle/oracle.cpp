@@ -795,11 +795,15 @@ size_t orc_refine(const orc_shape* shape, const float* root, uint32_t min_depth,
         if (level_counts) level_counts[depth] = nn;
         std::vector<uint8_t> split(nn, 0);
         parallel_for(nn, threads, 64, [&](size_t i) {
+            if (depth < min_depth) {
+                split[i] = 1;  // forced split: no distance evaluated
+                return;
+            }
             const float* sp = &cur[6 * i];
             float c[3];
             for (int a = 0; a < 3; ++a) c[a] = sp[a] + (sp[3 + a] - sp[a]) / 2.0f;
             float dd = shape_de(s, c[0], c[1], c[2]);
-            split[i] = (depth < min_depth) || std::isnan(dd) || (std::fabs(dd) < half_diag[depth]);
+            split[i] = std::isnan(dd) || (std::fabs(dd) < half_diag[depth]);
         });
         if (depth == max_depth) {
             for (size_t i = 0; i < nn; ++i)

@@ -284,3 +284,33 @@ def test_fast_mode_error_is_reported(torch_cuda):
           f"median rel {np.median(rel):.2e}")
     assert np.median(rel) < 1e-5 and frac < 0.1
     fctx.close()
+
+
+@pytest.mark.parametrize("max_depth", [5, 7])
+def test_refinement_matches_oracle(ctx, torch_cuda, max_depth):
+    """GPU level-synchronous refinement: same per-level node counts and the same
+    surface leaves in the same breadth-first create_spans order."""
+    import cantucci_amd as ca
+    from cantucci_amd import device as dev
+    from cantucci_amd.workloads import half_diagonals
+    hd = half_diagonals(3.0, max_depth)
+    root = [-1.5, -1.5, -1.5, 1.5, 1.5, 1.5]
+    shape = ca.Mandelbulb.classic(15, 2.0, ctx=ctx)
+    leaves, lc = dev.refine(shape, root, 2, max_depth, hd)
+    torch_cuda.cuda.synchronize()
+    leaves_o, lc_o = O.refine(O.mandelbulb(15, 2.0), root, 2, max_depth, hd)
+    assert list(lc) == list(lc_o)
+    assert same_bits(leaves.cpu().numpy(), leaves_o).all()
+
+
+def test_refinement_sphere_and_errors(ctx, torch_cuda):
+    import cantucci_amd as ca
+    from cantucci_amd import device as dev
+    from cantucci_amd.workloads import half_diagonals
+    hd = half_diagonals(2.0, 5)
+    root = [-1, -1, -1, 1, 1, 1]
+    leaves, lc = dev.refine(ca.Sphere((0.1, 0.0, -0.05), 0.6, ctx=ctx), root, 1, 5, hd)
+    leaves_o, lc_o = O.refine(O.sphere((0.1, 0.0, -0.05), 0.6), root, 1, 5, hd)
+    assert list(lc) == list(lc_o) and same_bits(leaves.cpu().numpy(), leaves_o).all()
+    with pytest.raises(ca.CthError):
+        dev.refine(ca.Mandelbulb.classic(15, 2.0, ctx=ctx), [1, -1, -1, -1, 1, 1], 0, 3, hd)
