@@ -255,6 +255,58 @@ int cth_refine_dev(cth_ctx* c, const cth_shape* s, const float root[6], uint32_t
                          pick_stream(c, stream));
 }
 
+int cth_sphere_trace_dev(cth_ctx* c, const cth_shape* s, const float* origins, int per_ray_origin,
+                         const float* dirs, size_t n, uint32_t max_steps, float eps_abs, float eps_rel, float t_max,
+                         float* depth, uint32_t* steps, float* hit_pos, void* stream) {
+    if (!c) return fail(CTH_E_INVALID, "null context");
+    if (int rc = check_shape(s)) return rc;
+    if (n && (!origins || !dirs || !depth || !steps)) return fail(CTH_E_INVALID, "null buffer");
+    if (max_steps == 0) return fail(CTH_E_INVALID, "max_steps must be >= 1");
+    if (n >= (size_t(1) << 31)) return fail(CTH_E_INVALID, "more than 2^31 rays");
+    return launch_trace_rays(*c, *s, origins, per_ray_origin, dirs, (uint32_t)n, max_steps, eps_abs, eps_rel, t_max,
+                             depth, steps, hit_pos, pick_stream(c, stream));
+}
+
+int cth_sphere_trace(cth_ctx* c, const cth_shape* s, const float* origins, int per_ray_origin, const float* dirs,
+                     size_t n, uint32_t max_steps, float eps_abs, float eps_rel, float t_max, float* depth,
+                     uint32_t* steps, float* hit_pos) {
+    if (!c) return fail(CTH_E_INVALID, "null context");
+    if (int rc = check_shape(s)) return rc;
+    if (n == 0) return CTH_OK;
+    if (!origins || !dirs || !depth || !steps) return fail(CTH_E_INVALID, "null buffer");
+    size_t no = per_ray_origin ? n : 1;
+    DevBuf dor, ddi, dde, dst, dhp;
+    int rc = dor.alloc(no * 12);
+    if (!rc) rc = ddi.alloc(n * 12);
+    if (!rc) rc = dde.alloc(n * 4);
+    if (!rc) rc = dst.alloc(n * 4);
+    if (!rc && hit_pos) rc = dhp.alloc(n * 12);
+    if (rc) return rc;
+    rc = check_hip(hipMemcpyAsync(dor.p, origins, no * 12, hipMemcpyHostToDevice, c->stream), "H2D");
+    if (!rc) rc = check_hip(hipMemcpyAsync(ddi.p, dirs, n * 12, hipMemcpyHostToDevice, c->stream), "H2D");
+    if (!rc)
+        rc = cth_sphere_trace_dev(c, s, (const float*)dor.p, per_ray_origin, (const float*)ddi.p, n, max_steps,
+                                  eps_abs, eps_rel, t_max, (float*)dde.p, (uint32_t*)dst.p,
+                                  hit_pos ? (float*)dhp.p : nullptr, c->stream);
+    if (!rc) rc = check_hip(hipMemcpyAsync(depth, dde.p, n * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+    if (!rc) rc = check_hip(hipMemcpyAsync(steps, dst.p, n * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+    if (!rc && hit_pos) rc = check_hip(hipMemcpyAsync(hit_pos, dhp.p, n * 12, hipMemcpyDeviceToHost, c->stream), "D2H");
+    int rs = check_hip(hipStreamSynchronize(c->stream), "sync");
+    return rc ? rc : rs;
+}
+
+int cth_render_depth_dev(cth_ctx* c, const cth_shape* s, const cth_camera* cams, uint32_t ncams, uint32_t width,
+                         uint32_t height, uint32_t max_steps, float eps_abs, float eps_rel, float t_max, float* depth,
+                         uint32_t* steps, void* stream) {
+    if (!c) return fail(CTH_E_INVALID, "null context");
+    if (int rc = check_shape(s)) return rc;
+    if (ncams && width && height && (!cams || !depth || !steps)) return fail(CTH_E_INVALID, "null buffer");
+    if (max_steps == 0) return fail(CTH_E_INVALID, "max_steps must be >= 1");
+    if ((uint64_t)width * height == 0 || ncams == 0) return CTH_OK;
+    return launch_trace_camera(*c, *s, cams, ncams, width, height, max_steps, eps_abs, eps_rel, t_max, depth, steps,
+                               pick_stream(c, stream));
+}
+
 int cth_sample_spans(cth_ctx* c, const cth_shape* s, const cth_span* spans, size_t nspans, uint32_t res,
                      float* dists, uint64_t* cross_bits) {
     if (!c) return fail(CTH_E_INVALID, "null context");

@@ -92,6 +92,8 @@ struct MandelUniform {
     int ipower;     // integer power when `integral`
     int integral;   // 1: every point uses ipower (reference Mandelbulb<P>);
                     // 0: real power per point (powf unless integral and < 64)
+    float center[3];  // sphere evaluator only
+    float radius;
 };
 
 // ---------------------------------------------------------------------------
@@ -284,5 +286,26 @@ __device__ __forceinline__ float sphere_de(float px, float py, float pz, const f
     float dx = c[0] - px, dy = c[1] - py, dz = c[2] - pz;
     return sqrt_rn((dx * dx + dy * dy) + dz * dz) - radius;
 }
+
+// The sphere as an evaluator of the lane-refill loops: it has no loop body,
+// so mag() reports +inf (an immediate "bailout") and finish() returns the
+// exact distance.  The centre and radius travel in the uniforms' spare fields.
+struct SphereEv {
+    float px, py, pz, dr;
+    float c[3], radius;
+    __device__ __forceinline__ void start(float ax, float ay, float az, float, const MandelUniform& u) {
+        px = ax;
+        py = ay;
+        pz = az;
+        dr = 1.0f;
+        c[0] = u.center[0];
+        c[1] = u.center[1];
+        c[2] = u.center[2];
+        radius = u.radius;
+    }
+    __device__ __forceinline__ float mag() const { return __builtin_inff(); }
+    __device__ __forceinline__ void body(float, const MandelUniform&, bool) {}
+    __device__ __forceinline__ float finish(float) const { return sphere_de(px, py, pz, c, radius); }
+};
 
 }  // namespace cth

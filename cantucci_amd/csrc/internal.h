@@ -39,7 +39,7 @@ struct Ctx {
         uint32_t i = __atomic_fetch_add(&ring_pos, 1u, __ATOMIC_RELAXED) % COUNTER_RING;
         return counters + 2 * i;
     }
-    LaunchShape shape_trace;
+    LaunchShape shape_trace{256, 16, 256, 32, 0};
     LaunchShape shape_adj;
     LaunchShape* launch_shape(int kernel) {
         switch (kernel) {
@@ -72,6 +72,12 @@ int launch_sample_spans(Ctx& c, const cth_shape& s, const float* spans, uint32_t
 int refine_levels(Ctx& c, const cth_shape& s, const float* root, uint32_t min_depth, uint32_t max_depth,
                   const float* half_diag, float* leaves, size_t cap, size_t* nleaves, uint64_t* level_counts,
                   hipStream_t st);
+int launch_trace_camera(Ctx& c, const cth_shape& s, const cth_camera* cams, uint32_t ncams, uint32_t W, uint32_t H,
+                        uint32_t max_steps, float eps_abs, float eps_rel, float t_max, float* depth,
+                        uint32_t* steps, hipStream_t st);
+int launch_trace_rays(Ctx& c, const cth_shape& s, const float* origins, int per_ray_origin, const float* dirs,
+                      uint32_t n, uint32_t max_steps, float eps_abs, float eps_rel, float t_max, float* depth,
+                      uint32_t* steps, float* hit_pos, hipStream_t st);
 int launch_contains_sphere(Ctx& c, const cth_shape& s, const float* pts, uint32_t n, uint8_t* out, hipStream_t st);
 int launch_less_than_zero(Ctx& c, const float* d, uint32_t n, uint8_t* out, hipStream_t st);
 int launch_dup_pairs(Ctx& c, const float* d, uint32_t n, float* pairs, hipStream_t st);

@@ -1,0 +1,226 @@
+// Sphere tracing with two-level lane refill.
+//
+// The reference marches 25 focus rays per frame on the CPU
+// (ShapeMesh::get_focii, src/mesh/mod.rs:205-243): d = DE(pos); pos += dir * d;
+// stop when d < EPSILON, at most 100 steps, no escape test.  Here every lane
+// owns one ray; each ray step is one distance evaluation whose loop bodies run
+// in the same branch-free ticks as the plain distance kernel.  When a lane's
+// evaluation finishes, the lane applies the step (pos += dir * d, t += d) and
+// either starts the ray's next evaluation or writes the ray and takes a new
+// one, so lanes stay busy although both the evaluation lengths and the ray
+// step counts differ wildly.  The hit test is d < eps_abs + eps_rel * t
+// (eps_rel = 0 gives the reference's absolute test; eps_abs = 0 the relative
+// one), a ray escapes when t > t_max (+inf: never, like the reference).
+#include <hip/hip_runtime.h>
+
+#include "internal.h"
+#include "launch.h"
+#include "point_sources.h"
+#include "work_loop.h"
+
+namespace cth {
+
+struct TraceParams {
+    uint32_t max_steps;
+    float eps_abs, eps_rel, t_max;
+};
+
+// Pinhole camera rays generated from the pixel index:
+//   u = ((2 (x + 0.5)) / W - 1) * tan_half * aspect
+//   v = (1 - (2 (y + 0.5)) / H) * tan_half
+//   d = (fwd + right * u) + up * v,  dir = d * (1 / |d|)
+// (the focus rays of get_focii are this construction on the near plane,
+// src/mesh/mod.rs:209-231, src/camera.rs:69-103,186-189).
+struct RaysCamera {
+    const cth_camera* cams;
+    uint32_t W, H;
+    FastDiv wh, w;
+    __device__ __forceinline__ void get(uint32_t idx, float* o, float* d) const {
+        uint32_t cam = fdiv(idx, wh);
+        uint32_t pix = idx - cam * wh.d;
+        uint32_t y = fdiv(pix, w);
+        uint32_t x = pix - y * w.d;
+        const cth_camera& c = cams[cam];
+        float th = c.tan_half_fov, asp = c.aspect;
+        float u = ((div_rn(2.0f * ((float)x + 0.5f), (float)W)) - 1.0f) * th * asp;
+        float v = (1.0f - div_rn(2.0f * ((float)y + 0.5f), (float)H)) * th;
+        float dd[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) dd[a] = (c.fwd[a] + c.right[a] * u) + c.up[a] * v;
+        float m = sqrt_rn((dd[0] * dd[0] + dd[1] * dd[1]) + dd[2] * dd[2]);
+        float inv = div_rn(1.0f, m);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            d[a] = dd[a] * inv;
+            o[a] = c.eye[a];
+        }
+    }
+};
+
+// Explicit rays: origins (one shared or one per ray) and unit directions.
+struct RaysExplicit {
+    const float* origins;
+    const float* dirs;
+    int per_ray_origin;
+    __device__ __forceinline__ void get(uint32_t idx, float* o, float* d) const {
+        const float* op = origins + (per_ray_origin ? 3 * (size_t)idx : 0);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            o[a] = op[a];
+            d[a] = dirs[3 * (size_t)idx + a];
+        }
+    }
+};
+
+struct TraceOut {
+    float* depth;
+    uint32_t* steps;
+    float* hit_pos;  // nullable: position after the last step
+};
+
+template <class Ev, class Rays>
+__global__ __launch_bounds__(256) void k_trace(Rays rays, MandelUniform u, TraceParams tp, TraceOut out,
+                                               WorkQueue q) {
+    Ev ev;
+    bool live = false;     // an evaluation of this lane's ray is in progress
+    bool pending = false;  // an evaluation finished and its step is not applied yet
+    bool has_ray = false;
+    uint32_t idx = 0, k = 0, nsteps = 0;
+    float r = 0.0f, t = 0.0f;
+    float pos[3] = {0.0f, 0.0f, 0.0f}, dir[3] = {0.0f, 0.0f, 0.0f};
+    uint32_t c_next = 0, c_end = 0, round = 0;
+    bool exhausted = false;
+
+    for (;;) {
+        if ((uint32_t)__popcll(__ballot(!live)) >= q.refill_min) {
+            for (;;) {
+                // Apply finished evaluations: one sphere-tracing step.
+                if (pending) {
+                    pending = false;
+                    float d = ev.finish(r);
+                    pos[0] = pos[0] + dir[0] * d;
+                    pos[1] = pos[1] + dir[1] * d;
+                    pos[2] = pos[2] + dir[2] * d;
+                    t = t + d;
+                    ++nsteps;
+                    bool hit = d < (tp.eps_abs + tp.eps_rel * t);
+                    if (hit || t > tp.t_max || nsteps == tp.max_steps) {
+                        out.depth[idx] = hit ? t : __builtin_inff();
+                        out.steps[idx] = nsteps;
+                        if (out.hit_pos) {
+                            out.hit_pos[3 * (size_t)idx] = pos[0];
+                            out.hit_pos[3 * (size_t)idx + 1] = pos[1];
+                            out.hit_pos[3 * (size_t)idx + 2] = pos[2];
+                        }
+                        has_ray = false;
+                    }
+                }
+                // New rays for lanes without one.
+                if (!exhausted) {
+                    uint64_t need = __ballot(!has_ray);
+                    if (need) {
+                        if (c_next >= c_end) {
+                            uint32_t b = next_chunk(q, round);
+                            if (b >= q.total) {
+                                exhausted = true;
+                            } else {
+                                c_next = b;
+                                c_end = (q.total - b < q.chunk) ? q.total : b + q.chunk;
+                            }
+                        }
+                        if (!exhausted) {
+                            uint32_t avail = c_end - c_next;
+                            uint32_t nneed = (uint32_t)__popcll(need);
+                            if (!has_ray) {
+                                uint32_t rank = lane_rank(need);
+                                if (rank < avail) {
+                                    idx = c_next + rank;
+                                    rays.get(idx, pos, dir);
+                                    t = 0.0f;
+                                    nsteps = 0;
+                                    has_ray = true;
+                                }
+                            }
+                            c_next += (nneed < avail) ? nneed : avail;
+                        }
+                    }
+                }
+                // Start the next evaluation of every ray that has none running.
+                if (has_ray && !live) {
+                    ev.start(pos[0], pos[1], pos[2], u.power, u);
+                    k = 0;
+                    r = ev.mag();
+                    if (r > u.bailout)
+                        pending = true;  // zero loop bodies: finished already
+                    else
+                        live = true;
+                }
+                bool more = __ballot(pending) != 0 || (!exhausted && __ballot(!has_ray) != 0);
+                if (!more) break;
+            }
+            if (__ballot(has_ray) == 0) break;
+        }
+        ev.body(r, u, live);
+        k += live ? 1u : 0u;
+        bool cap = live && (k == u.max_iters);
+        float rn = ev.mag();
+        bool bail = live && !cap && (rn > u.bailout);
+        if (live && !cap) r = rn;
+        if (cap || bail) {
+            live = false;
+            pending = true;
+        }
+    }
+    if (q.dynamic) wave_retire(q);
+}
+
+template <class Rays>
+static int run_trace(Ctx& c, const cth_shape& s, const Rays& rays, uint32_t total, const TraceParams& tp,
+                     const TraceOut& out, hipStream_t st) {
+    if (total == 0) return CTH_OK;
+    MandelUniform u{};
+    int family = FAM_SPHERE;
+    if (s.kind == CTH_SHAPE_SPHERE) {
+        u.max_iters = 1;
+        u.bailout = 0.0f;
+        for (int a = 0; a < 3; ++a) u.center[a] = s.center[a];
+        u.radius = s.radius;
+    } else {
+        int rc = resolve_mandel(s, u, family);
+        if (rc) return rc;
+    }
+    uint32_t blocks = 0;
+    WorkQueue q = make_queue(c, total, c.shape_trace, blocks);
+    dim3 grid(blocks), block(c.shape_trace.block);
+    if (family == FAM_SPHERE) {
+        hipLaunchKernelGGL((k_trace<SphereEv, Rays>), grid, block, 0, st, rays, u, tp, out, q);
+    } else if (family == FAM_P8) {
+        if (c.mode == CTH_MODE_FAST)
+            hipLaunchKernelGGL((k_trace<P8Fast, Rays>), grid, block, 0, st, rays, u, tp, out, q);
+        else
+            hipLaunchKernelGGL((k_trace<P8Exact, Rays>), grid, block, 0, st, rays, u, tp, out, q);
+    } else {
+        hipLaunchKernelGGL((k_trace<TrigExact, Rays>), grid, block, 0, st, rays, u, tp, out, q);
+    }
+    return check_hip(hipGetLastError(), "trace launch");
+}
+
+int launch_trace_camera(Ctx& c, const cth_shape& s, const cth_camera* cams, uint32_t ncams, uint32_t W, uint32_t H,
+                        uint32_t max_steps, float eps_abs, float eps_rel, float t_max, float* depth,
+                        uint32_t* steps, hipStream_t st) {
+    uint64_t total = (uint64_t)ncams * W * H;
+    if (total >= (1ull << 31)) return fail(CTH_E_INVALID, "more than 2^31 rays in one launch");
+    RaysCamera rays{cams, W, H, make_fastdiv(W * H), make_fastdiv(W)};
+    TraceParams tp{max_steps, eps_abs, eps_rel, t_max};
+    return run_trace(c, s, rays, (uint32_t)total, tp, TraceOut{depth, steps, nullptr}, st);
+}
+
+int launch_trace_rays(Ctx& c, const cth_shape& s, const float* origins, int per_ray_origin, const float* dirs,
+                      uint32_t n, uint32_t max_steps, float eps_abs, float eps_rel, float t_max, float* depth,
+                      uint32_t* steps, float* hit_pos, hipStream_t st) {
+    RaysExplicit rays{origins, dirs, per_ray_origin};
+    TraceParams tp{max_steps, eps_abs, eps_rel, t_max};
+    return run_trace(c, s, rays, n, tp, TraceOut{depth, steps, hit_pos}, st);
+}
+
+}  // namespace cth

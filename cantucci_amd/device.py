@@ -107,3 +107,36 @@ def refine(shape: Shape, root, min_depth: int, max_depth: int, half_diag, cap: i
     check(lib.cth_refine_dev(ctx.handle, C.byref(shape.desc), r, min_depth, max_depth, hd.ctypes.data,
                              _ptr(leaves), leaves.shape[0], C.byref(n), lc.ctypes.data, _stream()))
     return leaves[:n.value], lc
+
+
+def sphere_trace(shape: Shape, origins: torch.Tensor, dirs: torch.Tensor, max_steps: int = 100, eps_abs: float = 1e-6,
+                 eps_rel: float = 0.0, t_max: float = float("inf"), with_pos: bool = False,
+                 ctx: Context | None = None):
+    """Sphere-trace explicit rays; returns (depth, steps[, hit_pos])."""
+    ctx = ctx or shape.ctx
+    origins = _f32(origins).contiguous().reshape(-1, 3)
+    dirs = _f32(dirs).contiguous()
+    n = dirs.shape[0]
+    per_ray = 1 if origins.shape[0] == n and n > 1 else 0
+    depth = torch.empty(n, dtype=torch.float32, device=dirs.device)
+    steps = torch.empty(n, dtype=torch.int32, device=dirs.device)
+    pos = torch.empty((n, 3), dtype=torch.float32, device=dirs.device) if with_pos else None
+    check(lib.cth_sphere_trace_dev(ctx.handle, C.byref(shape.desc), _ptr(origins), per_ray, _ptr(dirs), n, max_steps,
+                                   eps_abs, eps_rel, t_max, _ptr(depth), _ptr(steps), _ptr(pos), _stream()))
+    return (depth, steps, pos) if with_pos else (depth, steps)
+
+
+def render_depth(shape: Shape, cams: torch.Tensor, width: int, height: int, max_steps: int = 256,
+                 eps_abs: float = 0.0, eps_rel: float = 1e-4, t_max: float = 10.0, ctx: Context | None = None,
+                 depth: torch.Tensor | None = None, steps: torch.Tensor | None = None):
+    """Depth images of (ncams, 14) f32 camera records; returns (depth, steps) of shape (ncams, H, W)."""
+    ctx = ctx or shape.ctx
+    cams = _f32(cams).contiguous()
+    m = cams.shape[0]
+    if depth is None:
+        depth = torch.empty((m, height, width), dtype=torch.float32, device=cams.device)
+    if steps is None:
+        steps = torch.empty((m, height, width), dtype=torch.int32, device=cams.device)
+    check(lib.cth_render_depth_dev(ctx.handle, C.byref(shape.desc), _ptr(cams), m, width, height, max_steps, eps_abs,
+                                   eps_rel, t_max, _ptr(depth), _ptr(steps), _stream()))
+    return depth, steps

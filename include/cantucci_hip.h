@@ -91,6 +91,18 @@ typedef struct cth_span {
     float end[3];
 } cth_span;
 
+/* Pinhole camera: eye position, orthonormal right/up/forward basis (the
+ * look_at_rh frame of src/camera.rs:69-75), tan(vertical fov / 2) and
+ * width / height (src/camera.rs:114-144,186-189). */
+typedef struct cth_camera {
+    float eye[3];
+    float right[3];
+    float up[3];
+    float fwd[3];
+    float tan_half_fov;
+    float aspect;
+} cth_camera;
+
 typedef struct cth_ctx cth_ctx;
 
 /* ---- context ------------------------------------------------------------ */
@@ -194,6 +206,32 @@ int cth_sample_spans_dev(cth_ctx* ctx, const cth_shape* shape, const float* span
 int cth_refine_dev(cth_ctx* ctx, const cth_shape* shape, const float root[6], uint32_t min_depth, uint32_t max_depth,
                    const float* half_diag, float* leaves, size_t cap, size_t* nleaves, uint64_t* level_counts,
                    void* stream);
+
+/* Sphere tracing of explicit rays: the marcher of ShapeMesh::get_focii
+ * (src/mesh/mod.rs:229-241):  d = DE(pos); pos += dir * d; t += d; a ray hits
+ * when d < eps_abs + eps_rel * t (the reference: eps_abs = 1e-6, eps_rel = 0),
+ * escapes when t > t_max (the reference has none: +inf) and stops after
+ * max_steps (the reference: 100).  depth = t on a hit, +inf otherwise; steps =
+ * distance evaluations; hit_pos (nullable, 3 floats per ray) = the position
+ * after the last step (get_focii returns it).  origins holds one point, or one
+ * per ray when per_ray_origin != 0. */
+int cth_sphere_trace_dev(cth_ctx* ctx, const cth_shape* shape, const float* origins, int per_ray_origin,
+                         const float* dirs, size_t n, uint32_t max_steps, float eps_abs, float eps_rel, float t_max,
+                         float* depth, uint32_t* steps, float* hit_pos, void* stream);
+
+/* Host-buffer variant of cth_sphere_trace_dev (synchronous). */
+int cth_sphere_trace(cth_ctx* ctx, const cth_shape* shape, const float* origins, int per_ray_origin,
+                     const float* dirs, size_t n, uint32_t max_steps, float eps_abs, float eps_rel, float t_max,
+                     float* depth, uint32_t* steps, float* hit_pos);
+
+/* Primary-ray depth images: ncams cameras (device array) x height x width
+ * pixel-centre rays, generated on the device in f32 as
+ *   u = ((2 (x + 0.5)) / W - 1) * tan_half_fov * aspect,  v = (1 - (2 (y + 0.5)) / H) * tan_half_fov,
+ *   dir = normalize((fwd + right * u) + up * v),  origin = eye,
+ * and traced like cth_sphere_trace_dev.  Output index = (cam * H + y) * W + x. */
+int cth_render_depth_dev(cth_ctx* ctx, const cth_shape* shape, const cth_camera* cams, uint32_t ncams,
+                         uint32_t width, uint32_t height, uint32_t max_steps, float eps_abs, float eps_rel,
+                         float t_max, float* depth, uint32_t* steps, void* stream);
 
 /* Host-buffer variant of cth_sample_spans_dev (synchronous). */
 int cth_sample_spans(cth_ctx* ctx, const cth_shape* shape, const cth_span* spans, size_t nspans, uint32_t res,

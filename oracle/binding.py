@@ -66,7 +66,7 @@ for _name, _res, _args in [
     ("orc_create_spans", None, [_P, _P]),
     ("orc_refine", _S, [_SH, _P, _U, _U, _P, _P, _S, _P, _I]),
     ("orc_ray_dirs", None, [_P, _U, _U, _U, _U, _P]),
-    ("orc_sphere_trace", None, [_SH, _P, _P, _S, _I, _U, _F, _F, _P, _P, _I]),
+    ("orc_sphere_trace", None, [_SH, _P, _P, _S, _I, _U, _F, _F, _F, _P, _P, _P, _I]),
     ("orc_fd_diff", None, [_SH, _P, _P, _S, _F, _P, _I]),
     ("orc_powi_f32", _F, [_F, _I]),
 ]:
@@ -222,17 +222,19 @@ def ray_dirs(pose, W: int, H: int, row0: int = 0, rows: int | None = None) -> np
     return out
 
 
-def sphere_trace(shape: OrcShape, origins, dirs, max_steps=256, eps_rel=1e-4, t_max=10.0, threads=None):
-    """Sphere tracing (src/mesh/mod.rs:229-241 with a relative hit test)."""
+def sphere_trace(shape: OrcShape, origins, dirs, max_steps=256, eps_rel=1e-4, t_max=10.0, eps_abs=0.0,
+                 with_pos=False, threads=None):
+    """Sphere tracing (src/mesh/mod.rs:229-241; hit when d < eps_abs + eps_rel * t)."""
     o = _f32(origins).reshape(-1, 3)
     dd = _f32(dirs).reshape(-1, 3)
     n = len(dd)
     stride = 0 if len(o) == 1 else 1
     depth = np.empty(n, np.float32)
     steps = np.empty(n, np.uint32)
-    _lib.orc_sphere_trace(C.byref(shape), _ptr(o), _ptr(dd), n, stride, max_steps, float(eps_rel), float(t_max),
-                          _ptr(depth), _ptr(steps), threads or default_threads())
-    return depth, steps
+    pos = np.empty((n, 3), np.float32) if with_pos else None
+    _lib.orc_sphere_trace(C.byref(shape), _ptr(o), _ptr(dd), n, stride, max_steps, float(eps_abs), float(eps_rel),
+                          float(t_max), _ptr(depth), _ptr(steps), _ptr(pos), threads or default_threads())
+    return (depth, steps, pos) if with_pos else (depth, steps)
 
 
 def fd_diff(shape: OrcShape, pts, h: float, powers=None, threads=None) -> np.ndarray:

@@ -850,13 +850,15 @@ void orc_ray_dirs(const float* pose, uint32_t W, uint32_t H, uint32_t row0, uint
         }
 }
 
-// Sphere tracing (the `get_focii` marcher, src/mesh/mod.rs:229-241, with the
-// relative hit test):  d = DE(pos); pos += dir * d; t += d;
-//   hit  if d < eps_rel * t   -> depth = t
-//   miss if t > t_max          -> depth = +inf
+// Sphere tracing (the `get_focii` marcher, src/mesh/mod.rs:229-241):
+//   d = DE(pos); pos += dir * d; t += d;
+//   hit  if d < eps_abs + eps_rel * t  -> depth = t   (reference: eps_abs 1e-6, eps_rel 0)
+//   miss if t > t_max                  -> depth = +inf (reference: no escape)
 // after max_steps without either -> depth = +inf, steps = max_steps.
+// hit_pos (nullable) receives the position after the last step.
 void orc_sphere_trace(const orc_shape* shape, const float* origins, const float* dirs, size_t n, int origin_stride,
-                      uint32_t max_steps, float eps_rel, float t_max, float* depth, uint32_t* steps, int threads) {
+                      uint32_t max_steps, float eps_abs, float eps_rel, float t_max, float* depth, uint32_t* steps,
+                      float* hit_pos, int threads) {
     const orc_shape s = *shape;
     parallel_for(n, threads, 64, [&](size_t i) {
         const float* o = origins + (origin_stride ? 3 * i : 0);
@@ -872,7 +874,7 @@ void orc_sphere_trace(const orc_shape* shape, const float* origins, const float*
             pz = pz + dz * d;
             t = t + d;
             ++k;
-            if (d < eps_rel * t) {
+            if (d < eps_abs + eps_rel * t) {
                 out = t;
                 break;
             }
@@ -880,6 +882,11 @@ void orc_sphere_trace(const orc_shape* shape, const float* origins, const float*
         }
         depth[i] = out;
         steps[i] = k;
+        if (hit_pos) {
+            hit_pos[3 * i] = px;
+            hit_pos[3 * i + 1] = py;
+            hit_pos[3 * i + 2] = pz;
+        }
     });
 }
 

@@ -1,0 +1,114 @@
+"""GPU sphere tracing against the oracle."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+F32 = np.float32
+
+
+def gpu(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def oracle_render(shape, cams, W, H, max_steps=256, eps_rel=1e-4, t_max=10.0):
+    depth, steps = [], []
+    for cam in cams:
+        dirs = O.ray_dirs(cam, W, H)
+        d, s = O.sphere_trace(shape, cam[:3][None], dirs, max_steps, eps_rel, t_max)
+        depth.append(d)
+        steps.append(s)
+    return np.stack(depth).reshape(len(cams), H, W), np.stack(steps).reshape(len(cams), H, W)
+
+
+def test_sphere_rays_bitexact(ctx, torch_cuda):
+    """A sphere's distance is exact arithmetic, so the GPU trace equals the
+    oracle's bit for bit (depth, steps and final position)."""
+    import cantucci_amd as ca
+    from cantucci_amd import device as dev
+    torch = torch_cuda
+    rng = np.random.default_rng(1)
+    dirs = rng.normal(size=(20000, 3))
+    dirs[:, 1] = np.abs(dirs[:, 1]) * 3
+    dirs = (dirs / np.linalg.norm(dirs, axis=1, keepdims=True)).astype(F32)
+    origin = np.array([[0.1, -3.0, 0.2]], F32)
+    shape = ca.Sphere((0.0, 0.0, 0.0), 0.8, ctx=ctx)
+    d, s, p = dev.sphere_trace(shape, gpu(torch, origin), gpu(torch, dirs), 256, 0.0, 1e-4, 10.0, with_pos=True)
+    torch.cuda.synchronize()
+    d_o, s_o, p_o = O.sphere_trace(O.sphere((0, 0, 0), 0.8), origin, dirs, 256, 1e-4, 10.0, 0.0, with_pos=True)
+    assert (s.cpu().numpy().astype(np.uint32) == s_o).all()
+    assert (d.cpu().numpy().view(np.uint32) == d_o.view(np.uint32)).all()
+    assert (p.cpu().numpy().view(np.uint32) == p_o.view(np.uint32)).all()
+    assert np.isfinite(d_o).any() and np.isinf(d_o).any()
+
+
+def test_camera_render_matches_oracle(ctx, torch_cuda):
+    """Primary rays of the 32 seeded orbit poses at reduced resolution, P8
+    Mandelbulb, 256 steps, hit when d < 1e-4 t, escape at t > 10."""
+    import cantucci_amd as ca
+    from cantucci_amd import device as dev
+    from cantucci_amd.workloads import orbit_poses
+    torch = torch_cuda
+    W, H = 48, 27
+    cams = orbit_poses(32, seed=0, W=W, H=H)
+    shape = ca.Mandelbulb.classic(15, 2.0, ctx=ctx)
+    depth, steps = dev.render_depth(shape, gpu(torch, cams), W, H)
+    torch.cuda.synchronize()
+    depth, steps = depth.cpu().numpy(), steps.cpu().numpy().astype(np.uint32)
+    d_o, s_o = oracle_render(O.mandelbulb(15, 2.0), cams, W, H)
+    hit_g, hit_o = np.isfinite(depth), np.isfinite(d_o)
+    same_steps = steps == s_o
+    both = hit_g & hit_o
+    rel = np.abs(depth[both].astype(np.float64) - d_o[both]) / d_o[both]
+    print(f"rays {depth.size}: hits {hit_o.mean():.3f}, hit agreement {np.mean(hit_g == hit_o):.5f}, "
+          f"steps equal {same_steps.mean():.5f}, depth bit-equal {np.mean(depth.view(np.uint32) == d_o.view(np.uint32)):.5f}, "
+          f"max rel depth {rel.max() if rel.size else 0:.2e}")
+    assert hit_o.mean() > 0.05
+    assert np.mean(hit_g == hit_o) > 0.999
+    assert same_steps.mean() > 0.99
+    assert np.percentile(rel, 99.9) < 1e-5
+
+
+def test_get_focii_semantics(ctx, torch_cuda):
+    """The reference's focus-point marcher: absolute epsilon 1e-6, 100 steps,
+    no escape, returning the position after the hitting step
+    (src/mesh/mod.rs:205-243)."""
+    import cantucci_amd as ca
+    from cantucci_amd import device as dev
+    from cantucci_amd.workloads import look_at_basis
+    torch = torch_cuda
+    eye = np.array([-3.0, 0.0, 0.0])
+    s, u, f = look_at_basis(eye, (0.0, 0.0, 0.0))
+    # 5 x 5 near-plane cell centres (FOCUS_POINTS = 5), vertical fov 1 rad (src/app.rs:98)
+    th = math.tan(0.5)
+    dirs = []
+    for x in range(5):
+        for y in range(5):
+            a, b = (-1 + (2 * x + 1) / 5) * th * 1.5, (-1 + (2 * y + 1) / 5) * th
+            d = f + s * a + u * b
+            dirs.append(d / np.linalg.norm(d))
+    dirs = np.array(dirs, F32)
+    origin = eye.astype(F32)[None]
+    shape = ca.Mandelbulb.classic(6, 2.5, ctx=ctx)  # the app's shape (src/app.rs:105)
+    d, st, p = dev.sphere_trace(shape, gpu(torch, origin), gpu(torch, dirs), 100, 1e-6, 0.0, float("inf"),
+                                with_pos=True)
+    torch.cuda.synchronize()
+    d_o, s_o, p_o = O.sphere_trace(O.mandelbulb(6, 2.5), origin, dirs, 100, 0.0, float("inf"), 1e-6, with_pos=True)
+    assert (st.cpu().numpy().astype(np.uint32) == s_o).mean() >= 0.96
+    hit = np.isfinite(d_o)
+    assert hit.any()
+    assert np.allclose(p.cpu().numpy()[hit], p_o[hit], rtol=1e-5, atol=1e-6)
+
+
+def test_trace_rejects_bad_arguments(ctx, torch_cuda):
+    import cantucci_amd as ca
+    from cantucci_amd import device as dev
+    torch = torch_cuda
+    shape = ca.Mandelbulb.classic(15, 2.0, ctx=ctx)
+    with pytest.raises(ca.CthError):
+        dev.sphere_trace(shape, torch.zeros((1, 3), device="cuda"), torch.ones((4, 3), device="cuda"), 0)
+    d, s = dev.sphere_trace(shape, torch.zeros((1, 3), device="cuda"), torch.ones((0, 3), device="cuda"), 10)
+    assert d.numel() == 0
