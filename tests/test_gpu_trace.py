@@ -61,14 +61,18 @@ def test_camera_render_matches_oracle(ctx, torch_cuda):
     d_o, s_o = oracle_render(O.mandelbulb(15, 2.0), cams, W, H)
     hit_g, hit_o = np.isfinite(depth), np.isfinite(d_o)
     same_steps = steps == s_o
-    both = hit_g & hit_o
+    both = hit_g & hit_o & same_steps
     rel = np.abs(depth[both].astype(np.float64) - d_o[both]) / d_o[both]
+    bit = depth.view(np.uint32) == d_o.view(np.uint32)
     print(f"rays {depth.size}: hits {hit_o.mean():.3f}, hit agreement {np.mean(hit_g == hit_o):.5f}, "
-          f"steps equal {same_steps.mean():.5f}, depth bit-equal {np.mean(depth.view(np.uint32) == d_o.view(np.uint32)):.5f}, "
-          f"max rel depth {rel.max() if rel.size else 0:.2e}")
+          f"steps equal {same_steps.mean():.5f}, depth bit-equal {bit.mean():.5f}, "
+          f"max rel depth (equal steps) {rel.max() if rel.size else 0:.2e}")
     assert hit_o.mean() > 0.05
     assert np.mean(hit_g == hit_o) > 0.999
-    assert same_steps.mean() > 0.99
+    # A step-count difference needs a hit test d < 1e-4 t decided differently
+    # by an ulp of the final ln; it is rare and is the only source of larger
+    # depth differences.
+    assert same_steps.mean() > 0.995
     assert np.percentile(rel, 99.9) < 1e-5
 
 
