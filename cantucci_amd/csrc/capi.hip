@@ -307,6 +307,41 @@ int cth_render_depth_dev(cth_ctx* c, const cth_shape* s, const cth_camera* cams,
                                pick_stream(c, stream));
 }
 
+int cth_de_vjp_dev(cth_ctx* c, const cth_shape* s, const float* pts, const float* powers, size_t n,
+                   const float* grad_d, float* d, float* grad_p, float* grad_power, void* stream) {
+    if (!c) return fail(CTH_E_INVALID, "null context");
+    if (int rc = check_shape(s)) return rc;
+    if (n && !pts) return fail(CTH_E_INVALID, "null buffer");
+    if (n >= (size_t(1) << 31)) return fail(CTH_E_INVALID, "batch larger than 2^31");
+    hipStream_t st = pick_stream(c, stream);
+    float* gp = grad_power;
+    if (grad_power && s->kind != CTH_SHAPE_MANDELBULB_TRIG) {
+        int rc = check_hip(hipMemsetAsync(grad_power, 0, n * sizeof(float), st), "memset grad_power");
+        if (rc) return rc;
+        gp = nullptr;
+    }
+    return launch_adjoint(*c, *s, pts, powers, (uint32_t)n, grad_d, d, grad_p, gp, nullptr, st);
+}
+
+int cth_normals_dev(cth_ctx* c, const cth_shape* s, const float* pts, const float* powers, size_t n, float* d,
+                    float* normals, void* stream) {
+    if (!c) return fail(CTH_E_INVALID, "null context");
+    if (int rc = check_shape(s)) return rc;
+    if (n && (!pts || !normals)) return fail(CTH_E_INVALID, "null buffer");
+    if (n >= (size_t(1) << 31)) return fail(CTH_E_INVALID, "batch larger than 2^31");
+    return launch_adjoint(*c, *s, pts, powers, (uint32_t)n, nullptr, d, nullptr, nullptr, normals,
+                          pick_stream(c, stream));
+}
+
+int cth_normals_fd_dev(cth_ctx* c, const cth_shape* s, const float* pts, const float* powers, size_t n, float h,
+                       float* d, float* normals, float* grad_p, void* stream) {
+    if (!c) return fail(CTH_E_INVALID, "null context");
+    if (int rc = check_shape(s)) return rc;
+    if (n && !pts) return fail(CTH_E_INVALID, "null buffer");
+    if (!(h > 0.0f)) return fail(CTH_E_INVALID, "h must be positive");
+    return launch_fd_normals(*c, *s, pts, powers, (uint32_t)n, h, d, grad_p, normals, pick_stream(c, stream));
+}
+
 int cth_sample_spans(cth_ctx* c, const cth_shape* s, const cth_span* spans, size_t nspans, uint32_t res,
                      float* dists, uint64_t* cross_bits) {
     if (!c) return fail(CTH_E_INVALID, "null context");

@@ -40,7 +40,7 @@ struct Ctx {
         return counters + 2 * i;
     }
     LaunchShape shape_trace{256, 16, 256, 32, 0};
-    LaunchShape shape_adj;
+    LaunchShape shape_adj{256, 16, 256, 32, 0};
     LaunchShape* launch_shape(int kernel) {
         switch (kernel) {
             case CTH_KERNEL_DE: return &shape_de;
@@ -78,6 +78,10 @@ int launch_trace_camera(Ctx& c, const cth_shape& s, const cth_camera* cams, uint
 int launch_trace_rays(Ctx& c, const cth_shape& s, const float* origins, int per_ray_origin, const float* dirs,
                       uint32_t n, uint32_t max_steps, float eps_abs, float eps_rel, float t_max, float* depth,
                       uint32_t* steps, float* hit_pos, hipStream_t st);
+int launch_adjoint(Ctx& c, const cth_shape& s, const float* pts, const float* powers, uint32_t n,
+                   const float* grad_d, float* d, float* grad_p, float* grad_power, float* normals, hipStream_t st);
+int launch_fd_normals(Ctx& c, const cth_shape& s, const float* pts, const float* powers, uint32_t n, float h,
+                      float* d, float* grad_p, float* normals, hipStream_t st);
 int launch_contains_sphere(Ctx& c, const cth_shape& s, const float* pts, uint32_t n, uint8_t* out, hipStream_t st);
 int launch_less_than_zero(Ctx& c, const float* d, uint32_t n, uint8_t* out, hipStream_t st);
 int launch_dup_pairs(Ctx& c, const float* d, uint32_t n, float* pairs, hipStream_t st);

@@ -12,19 +12,6 @@
 
 namespace cth {
 
-// Sphere: no iteration, one point per thread.
-template <class Src>
-__global__ __launch_bounds__(256) void k_sphere(Src src, uint32_t n, float cx, float cy, float cz, float radius,
-                                                float* out, uint32_t* iters) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float x, y, z, p;
-    src.get(i, x, y, z, p);
-    const float c[3] = {cx, cy, cz};
-    out[i] = sphere_de(x, y, z, c, radius);
-    if (iters) iters[i] = 0;
-}
-
 // One wave per 64-cell word: lane l classifies cell w*64 + l of its span.
 // A cell crosses the surface iff its 8 corners do not all share the IEEE sign
 // bit (`is_sign_positive`, src/mesh/buffer.rs:130-141).

@@ -16,6 +16,19 @@ __global__ __launch_bounds__(256) void k_refill(Src src, MandelUniform u, Sink s
     refill_loop<Ev>(src, u, sink, q);
 }
 
+// Sphere: no iteration, one point per thread.
+template <class Src>
+__global__ __launch_bounds__(256) void k_sphere(Src src, uint32_t n, float cx, float cy, float cz, float radius,
+                                                float* out, uint32_t* iters) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float x, y, z, p;
+    src.get(i, x, y, z, p);
+    const float c[3] = {cx, cy, cz};
+    out[i] = sphere_de(x, y, z, c, radius);
+    if (iters) iters[i] = 0;
+}
+
 inline WorkQueue make_queue(Ctx& c, uint32_t total, const LaunchShape& ls, uint32_t& blocks) {
     uint32_t waves_per_block = (uint32_t)ls.block / 64u;
     uint64_t want = (uint64_t)c.num_cus * (uint64_t)ls.blocks_per_cu;

@@ -140,3 +140,72 @@ def render_depth(shape: Shape, cams: torch.Tensor, width: int, height: int, max_
     check(lib.cth_render_depth_dev(ctx.handle, C.byref(shape.desc), _ptr(cams), m, width, height, max_steps, eps_abs,
                                    eps_rel, t_max, _ptr(depth), _ptr(steps), _stream()))
     return depth, steps
+
+
+def de_vjp(shape: Shape, pts: torch.Tensor, powers: torch.Tensor | None = None, grad_d: torch.Tensor | None = None,
+           with_power: bool = False, ctx: Context | None = None):
+    """(d, grad_p[, grad_power]) from the fused forward + reverse sweep."""
+    ctx = ctx or shape.ctx
+    pts = _f32(pts).contiguous()
+    n = pts.shape[0]
+    d = torch.empty(n, dtype=torch.float32, device=pts.device)
+    gp = torch.empty((n, 3), dtype=torch.float32, device=pts.device)
+    gpow = torch.empty(n, dtype=torch.float32, device=pts.device) if with_power else None
+    check(lib.cth_de_vjp_dev(ctx.handle, C.byref(shape.desc), _ptr(pts), _ptr(powers), n, _ptr(grad_d), _ptr(d),
+                             _ptr(gp), _ptr(gpow), _stream()))
+    return (d, gp, gpow) if with_power else (d, gp)
+
+
+def normals(shape: Shape, pts: torch.Tensor, powers: torch.Tensor | None = None, ctx: Context | None = None,
+            out: torch.Tensor | None = None, d: torch.Tensor | None = None):
+    """Unit normals (n, 3) from the fused forward + adjoint; returns (d, normals)."""
+    ctx = ctx or shape.ctx
+    pts = _f32(pts).contiguous()
+    n = pts.shape[0]
+    if out is None:
+        out = torch.empty((n, 3), dtype=torch.float32, device=pts.device)
+    if d is None:
+        d = torch.empty(n, dtype=torch.float32, device=pts.device)
+    check(lib.cth_normals_dev(ctx.handle, C.byref(shape.desc), _ptr(pts), _ptr(powers), n, _ptr(d), _ptr(out),
+                              _stream()))
+    return d, out
+
+
+def normals_fd(shape: Shape, pts: torch.Tensor, h: float, powers: torch.Tensor | None = None,
+               ctx: Context | None = None, out: torch.Tensor | None = None, d: torch.Tensor | None = None,
+               grad: torch.Tensor | None = None):
+    """6-tap central-difference normals; returns (d, normals)."""
+    ctx = ctx or shape.ctx
+    pts = _f32(pts).contiguous()
+    n = pts.shape[0]
+    if out is None:
+        out = torch.empty((n, 3), dtype=torch.float32, device=pts.device)
+    if d is None:
+        d = torch.empty(n, dtype=torch.float32, device=pts.device)
+    check(lib.cth_normals_fd_dev(ctx.handle, C.byref(shape.desc), _ptr(pts), _ptr(powers), n, float(h), _ptr(d),
+                                 _ptr(out), _ptr(grad), _stream()))
+    return d, out
+
+
+class MandelbulbDistance(torch.autograd.Function):
+    """torch.autograd.Function over the C ABI: forward = cth_min_distance_dev,
+    backward = cth_de_vjp_dev (forward recomputed per lane, reverse sweep)."""
+
+    @staticmethod
+    def forward(ctx_, pts, powers, shape):
+        ctx_.shape = shape
+        ctx_.save_for_backward(pts, powers)
+        return de_points(shape, pts.detach(), None if powers is None else powers.detach())
+
+    @staticmethod
+    def backward(ctx_, grad_d):
+        pts, powers = ctx_.saved_tensors
+        shape = ctx_.shape
+        _, gp, gpow = de_vjp(shape, pts.detach(), None if powers is None else powers.detach(),
+                             grad_d.contiguous(), with_power=True)
+        return gp, (gpow if (powers is not None and ctx_.needs_input_grad[1]) else None), None
+
+
+def distance(shape: Shape, pts: torch.Tensor, powers: torch.Tensor | None = None) -> torch.Tensor:
+    """Differentiable distance estimate (autograd through the adjoint kernel)."""
+    return MandelbulbDistance.apply(pts, powers, shape)

@@ -233,6 +233,34 @@ int cth_render_depth_dev(cth_ctx* ctx, const cth_shape* shape, const cth_camera*
                          uint32_t width, uint32_t height, uint32_t max_steps, float eps_abs, float eps_rel,
                          float t_max, float* depth, uint32_t* steps, void* stream);
 
+/* Differentiable distance: the vector-Jacobian product of min_distance_from
+ * (the backward of an autograd function whose forward is
+ * cth_min_distance_dev).  The forward iteration is recomputed per lane in
+ * parity arithmetic (its distance goes to `d`, nullable), then a reverse sweep
+ * through the spherical power map (trig family) or the P8 polynomial
+ * (src/shape/mandelbulb.rs:59-79,128-200) gives
+ *   grad_p[i] = grad_d[i] * dd/dp   (n x 3, nullable)
+ *   grad_power[i] = grad_d[i] * dd/dpower  (nullable; CTH_SHAPE_MANDELBULB_TRIG
+ *                                           only, 0 for the other kinds)
+ * grad_d NULL means all ones.  The derivative ignores the measure-zero
+ * switches of the loop (bailout, iteration cap).  Uses a device workspace of
+ * at most ~1 GiB (stream-ordered allocation). */
+int cth_de_vjp_dev(cth_ctx* ctx, const cth_shape* shape, const float* pts, const float* powers, size_t n,
+                   const float* grad_d, float* d, float* grad_p, float* grad_power, void* stream);
+
+/* Surface normals from the fused forward + adjoint: normals[i] = grad / |grad|
+ * (n x 3); the replacement for the 6-tap difference normal of
+ * src/mesh/buffer.rs:256-266.  d (nullable) receives the distances. */
+int cth_normals_dev(cth_ctx* ctx, const cth_shape* shape, const float* pts, const float* powers, size_t n, float* d,
+                    float* normals, void* stream);
+
+/* The reference's 6-tap central-difference normal (src/mesh/buffer.rs:256-266)
+ * with step h on every axis: 7 distance evaluations per point (p and p +- h e_a,
+ * formed as p + unit_a * (+-h)); normals = normalize(differences) (nullable),
+ * grad_p = differences / (2h) (nullable), d = DE(p) (nullable). */
+int cth_normals_fd_dev(cth_ctx* ctx, const cth_shape* shape, const float* pts, const float* powers, size_t n, float h,
+                       float* d, float* normals, float* grad_p, void* stream);
+
 /* Host-buffer variant of cth_sample_spans_dev (synchronous). */
 int cth_sample_spans(cth_ctx* ctx, const cth_shape* shape, const cth_span* spans, size_t nspans, uint32_t res,
                      float* dists, uint64_t* cross_bits);

@@ -175,6 +175,7 @@ inline void rotate_trig(T x, T y, T z, T P, int Pi, bool integral, T& ox, T& oy,
 }
 
 struct MandelParams {
+    bool real_power;     // kind 1: the power is a real parameter (d/dpower defined)
     int family;          // 0: P8 polynomial, 1: trig
     float P;             // power as f32
     int Pi;              // integer power (when integral)
@@ -190,6 +191,7 @@ inline MandelParams mandel_params(const orc_shape& s, float power) {
     m.max_iters = s.max_iters;
     m.bailout = s.bailout;
     m.P = power;
+    m.real_power = (s.kind == K_MANDELBULB_TRIG);
     if (s.kind == K_MANDELBULB) {
         m.Pi = (int)power;
         m.integral = true;
@@ -363,18 +365,21 @@ inline Dual mandel_de_dual(double px, double py, double pz, const MandelParams& 
     Py.g[1] = 1.0;
     Pz.g[2] = 1.0;
     Dual P = dconst(m.P);
-    if (!m.integral) P.g[3] = 1.0;
+    if (m.real_power) P.g[3] = 1.0;
+    // The real-power kind is differentiated as powf throughout (an integral
+    // power only changes the f32 forward's lowering, not the function).
+    const bool integral = m.integral && !m.real_power;
     Dual one = dconst(1.0);
     Dual x = Px, y = Py, z = Pz, dr = one, r = dconst(0.0);
     uint32_t k = 0;
     for (uint64_t i = 0; i < m.max_iters; ++i) {
         r = dmag(x, y, z);
         if (r.v > double(m.bailout)) break;
-        dr = (dpow_or_powi(r, P - one, m.Pi - 1, m.integral) * P) * dr + one;
+        dr = (dpow_or_powi(r, P - one, m.Pi - 1, integral) * P) * dr + one;
         Dual nx, ny, nz;
         if (x.v == 0.0 && y.v == 0.0) {
             Dual th = dacos(z / r);
-            Dual R = dpow_or_powi(r, P, m.Pi, m.integral);
+            Dual R = dpow_or_powi(r, P, m.Pi, integral);
             nx = dconst(0.0);
             ny = dconst(0.0);
             nz = R * dcos(th * P);
@@ -391,7 +396,7 @@ inline Dual mandel_de_dual(double px, double py, double pz, const MandelParams& 
         } else {
             Dual th = dacos(z / r);
             Dual ph = datan2(y, x);
-            Dual R = dpow_or_powi(r, P, m.Pi, m.integral);
+            Dual R = dpow_or_powi(r, P, m.Pi, integral);
             th = th * P;
             ph = ph * P;
             nx = (dsin(th) * dcos(ph)) * R;

@@ -1,0 +1,136 @@
+"""Adjoint (reverse-sweep) gradients and normals against the FP64 dual-number
+oracle, central differences and the 6-tap FD baseline."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+F32 = np.float32
+
+
+def gpu(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def angle(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    c = (a * b).sum(1) / (np.linalg.norm(a, axis=1) * np.linalg.norm(b, axis=1))
+    return np.arccos(np.clip(c, -1, 1))
+
+
+def test_trig_vjp_matches_dual_oracle(ctx, torch_cuda):
+    """C4 inputs: shell 0.9 < |p| < 1.3 (seed 1), power U[6, 10], 15 iterations."""
+    import cantucci_amd as ca
+    from cantucci_amd import device as dev
+    from cantucci_amd.workloads import shell_points
+    torch = torch_cuda
+    pts, powers = shell_points(50000, seed=1)
+    shape = ca.MandelbulbTrig(15, 2.0, 8.0, ctx=ctx)
+    d, gp, gpow = dev.de_vjp(shape, gpu(torch, pts), gpu(torch, powers), with_power=True)
+    torch.cuda.synchronize()
+    d, gp, gpow = d.cpu().numpy(), gp.cpu().numpy(), gpow.cpu().numpy()
+    d64, g64, it64 = O.de_grad_f64(O.mandelbulb_trig(15, 2.0), pts, powers)
+    d_o, it_o, _, _ = O.de_points(O.mandelbulb_trig(15, 2.0), pts, powers)
+    # the adjoint's forward is the parity forward
+    assert np.mean(d.view(np.uint32) == d_o.view(np.uint32)) > 0.99
+    same = it_o == it64
+    ang = angle(gp[same], g64[same, :3])
+    relp = np.abs(gpow[same] - g64[same, 3]) / np.maximum(np.abs(g64[same, 3]), 1e-12)
+    print(f"trig vjp: iters f32==f64 {same.mean():.4f}, angle p50 {np.median(ang):.2e} p99 {np.percentile(ang, 99):.2e} "
+          f"max {ang.max():.2e}; dpower rel p50 {np.median(relp):.2e} p99 {np.percentile(relp, 99):.2e}")
+    assert np.median(ang) < 1e-5
+    assert np.mean(ang <= 1e-3) > 0.99
+    assert np.median(relp) < 1e-4
+
+
+@pytest.mark.parametrize("P", [8, 5])
+def test_integer_power_normals_match_dual_oracle(ctx, torch_cuda, P):
+    """Mandelbulb<P> (P8 polynomial for P = 8, trig map otherwise): fused
+    normals agree with the FP64 dual gradient direction."""
+    import cantucci_amd as ca
+    from cantucci_amd import device as dev
+    from cantucci_amd.workloads import shell_points
+    torch = torch_cuda
+    pts, _ = shell_points(50000, seed=2, r_lo=0.6, r_hi=1.25)
+    shape = ca.Mandelbulb(15, 2.0, P, ctx=ctx)
+    d, nrm = dev.normals(shape, gpu(torch, pts))
+    _, gp = dev.de_vjp(shape, gpu(torch, pts))
+    torch.cuda.synchronize()
+    nrm, gp = nrm.cpu().numpy(), gp.cpu().numpy()
+    _, g64, it64 = O.de_grad_f64(O.mandelbulb(15, 2.0, P), pts)
+    _, it_o, _, _ = O.de_points(O.mandelbulb(15, 2.0, P), pts)
+    same = it_o == it64
+    ang = angle(nrm[same], g64[same, :3])
+    print(f"P={P} normals: angle p50 {np.median(ang):.2e} p99 {np.percentile(ang, 99):.2e}")
+    assert np.abs(np.linalg.norm(nrm, axis=1) - 1).max() < 1e-5
+    assert np.median(ang) < 1e-5
+    assert np.mean(ang <= 1e-3) > 0.99
+    assert np.median(angle(gp[same], g64[same, :3])) < 1e-5
+
+
+def test_vjp_is_linear_in_grad_d(ctx, torch_cuda):
+    import cantucci_amd as ca
+    from cantucci_amd import device as dev
+    torch = torch_cuda
+    pts = np.random.default_rng(3).uniform(-1.2, 1.2, (4096, 3)).astype(F32)
+    shape = ca.Mandelbulb.classic(15, 2.0, ctx=ctx)
+    g = torch.full((4096,), 2.0, device="cuda")
+    _, g1 = dev.de_vjp(shape, gpu(torch, pts))
+    _, g2 = dev.de_vjp(shape, gpu(torch, pts), grad_d=g)
+    torch.cuda.synchronize()
+    fin = torch.isfinite(g1).all(1)
+    assert torch.equal(g2[fin], 2 * g1[fin])
+
+
+def test_zero_iteration_gradient_closed_form(ctx, torch_cuda):
+    """|p| > bailout: d = 0.5 r ln r, grad = 0.5 (ln r + 1) p / r."""
+    import cantucci_amd as ca
+    from cantucci_amd import device as dev
+    torch = torch_cuda
+    v = np.random.default_rng(4).normal(size=(5000, 3))
+    pts = (v / np.linalg.norm(v, axis=1, keepdims=True) * np.random.default_rng(5).uniform(2.1, 4, (5000, 1)))
+    pts = pts.astype(F32)
+    _, gp = dev.de_vjp(ca.Mandelbulb.classic(15, 2.0, ctx=ctx), gpu(torch, pts))
+    torch.cuda.synchronize()
+    r = np.linalg.norm(pts.astype(np.float64), axis=1, keepdims=True)
+    expect = 0.5 * (np.log(r) + 1) * pts / r
+    assert np.allclose(gp.cpu().numpy(), expect, rtol=2e-6, atol=1e-7)
+
+
+def test_fd_normals_match_oracle_stencil(ctx, torch_cuda):
+    """The 6-tap baseline reproduces the reference stencil: differences of
+    parity distances at p +- h e_a (src/mesh/buffer.rs:256-266)."""
+    import cantucci_amd as ca
+    from cantucci_amd import device as dev
+    from cantucci_amd.workloads import shell_points
+    torch = torch_cuda
+    pts, powers = shell_points(20000, seed=6)
+    h = 1e-3
+    for shape, oshape, pw in ((ca.MandelbulbTrig(15, 2.0, 8.0, ctx=ctx), O.mandelbulb_trig(15, 2.0), powers),
+                              (ca.Mandelbulb.classic(15, 2.0, ctx=ctx), O.mandelbulb(15, 2.0), None)):
+        grad = torch.empty((len(pts), 3), device="cuda")
+        d, nrm = dev.normals_fd(shape, gpu(torch, pts), h, None if pw is None else gpu(torch, pw), grad=grad)
+        torch.cuda.synchronize()
+        diff_o = O.fd_diff(oshape, pts, h, pw)
+        diff = grad.cpu().numpy() * np.float32(2 * h)
+        close = np.abs(diff - diff_o) <= 1e-5 * np.abs(diff_o) + 1e-9
+        print(f"fd taps: {close.mean():.5f} of differences within 1e-5")
+        assert close.mean() > 0.98
+
+
+def test_autograd_function(ctx, torch_cuda):
+    import cantucci_amd as ca
+    from cantucci_amd import device as dev
+    torch = torch_cuda
+    pts = torch.tensor(np.random.default_rng(7).uniform(-1.2, 1.2, (256, 3)).astype(F32), device="cuda",
+                       requires_grad=True)
+    powers = torch.full((256,), 7.5, device="cuda", requires_grad=True)
+    shape = ca.MandelbulbTrig(15, 2.0, 8.0, ctx=ctx)
+    d = dev.distance(shape, pts, powers)
+    (d * 3.0).sum().backward()
+    _, gp, gpow = dev.de_vjp(shape, pts.detach(), powers.detach(), torch.full((256,), 3.0, device="cuda"),
+                             with_power=True)
+    torch.cuda.synchronize()
+    assert torch.equal(pts.grad, gp) and torch.equal(powers.grad, gpow)
