@@ -329,8 +329,24 @@ int cth_normals_dev(cth_ctx* c, const cth_shape* s, const float* pts, const floa
     if (int rc = check_shape(s)) return rc;
     if (n && (!pts || !normals)) return fail(CTH_E_INVALID, "null buffer");
     if (n >= (size_t(1) << 31)) return fail(CTH_E_INVALID, "batch larger than 2^31");
-    return launch_adjoint(*c, *s, pts, powers, (uint32_t)n, nullptr, d, nullptr, nullptr, normals,
+    return launch_tangent(*c, *s, pts, powers, (uint32_t)n, nullptr, d, nullptr, nullptr, normals,
                           pick_stream(c, stream));
+}
+
+int cth_de_grad_dev(cth_ctx* c, const cth_shape* s, const float* pts, const float* powers, size_t n,
+                    const float* grad_d, float* d, float* grad_p, float* grad_power, void* stream) {
+    if (!c) return fail(CTH_E_INVALID, "null context");
+    if (int rc = check_shape(s)) return rc;
+    if (n && !pts) return fail(CTH_E_INVALID, "null buffer");
+    if (n >= (size_t(1) << 31)) return fail(CTH_E_INVALID, "batch larger than 2^31");
+    hipStream_t st = pick_stream(c, stream);
+    float* gp = grad_power;
+    if (grad_power && s->kind != CTH_SHAPE_MANDELBULB_TRIG) {
+        int rc = check_hip(hipMemsetAsync(grad_power, 0, n * sizeof(float), st), "memset grad_power");
+        if (rc) return rc;
+        gp = nullptr;
+    }
+    return launch_tangent(*c, *s, pts, powers, (uint32_t)n, grad_d, d, grad_p, gp, nullptr, st);
 }
 
 int cth_normals_fd_dev(cth_ctx* c, const cth_shape* s, const float* pts, const float* powers, size_t n, float h,

@@ -80,6 +80,8 @@ int launch_trace_rays(Ctx& c, const cth_shape& s, const float* origins, int per_
                       uint32_t* steps, float* hit_pos, hipStream_t st);
 int launch_adjoint(Ctx& c, const cth_shape& s, const float* pts, const float* powers, uint32_t n,
                    const float* grad_d, float* d, float* grad_p, float* grad_power, float* normals, hipStream_t st);
+int launch_tangent(Ctx& c, const cth_shape& s, const float* pts, const float* powers, uint32_t n,
+                   const float* grad_d, float* d, float* grad_p, float* grad_power, float* normals, hipStream_t st);
 int launch_fd_normals(Ctx& c, const cth_shape& s, const float* pts, const float* powers, uint32_t n, float h,
                       float* d, float* grad_p, float* normals, hipStream_t st);
 int launch_contains_sphere(Ctx& c, const cth_shape& s, const float* pts, uint32_t n, uint8_t* out, hipStream_t st);

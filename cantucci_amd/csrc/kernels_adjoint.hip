@@ -23,6 +23,7 @@
 #include "internal.h"
 #include "launch.h"
 #include "point_sources.h"
+#include "tangent.h"
 #include "work_loop.h"
 
 namespace cth {
@@ -440,6 +441,29 @@ int launch_adjoint(Ctx& c, const cth_shape& s, const float* pts, const float* po
     if (orbit) (void)hipFreeAsync(orbit, st);
     if (rec) (void)hipFreeAsync(rec, st);
     return rc;
+}
+
+// Forward-mode normals/gradients: the tangent evaluators in the lane-refill
+// loop; one launch, no workspace.
+int launch_tangent(Ctx& c, const cth_shape& s, const float* pts, const float* powers, uint32_t n,
+                   const float* grad_d, float* d, float* grad_p, float* grad_power, float* normals, hipStream_t st) {
+    if (n == 0) return CTH_OK;
+    if (s.kind == CTH_SHAPE_SPHERE) return fail(CTH_E_UNSUPPORTED, "use the closed form for a sphere");
+    MandelUniform u;
+    int family = 0;
+    int rc = resolve_mandel(s, u, family);
+    if (rc) return rc;
+    SrcPoints src{pts, powers, s.power};
+    SinkGrad sink{d, normals, grad_p, grad_power, grad_d, u.max_iters};
+    uint32_t blocks = 0;
+    WorkQueue q = make_queue(c, n, c.shape_adj, blocks);
+    if (family == FAM_P8)
+        hipLaunchKernelGGL((k_refill<P8Tangent, SrcPoints, SinkGrad>), dim3(blocks), dim3(c.shape_adj.block), 0, st, src,
+                           u, sink, q);
+    else
+        hipLaunchKernelGGL((k_refill<TrigTangent, SrcPoints, SinkGrad>), dim3(blocks), dim3(c.shape_adj.block), 0, st,
+                           src, u, sink, q);
+    return check_hip(hipGetLastError(), "tangent launch");
 }
 
 int launch_fd_normals(Ctx& c, const cth_shape& s, const float* pts, const float* powers, uint32_t n, float h,

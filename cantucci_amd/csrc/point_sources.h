@@ -118,7 +118,8 @@ struct SrcSpans {
 struct SinkDist {
     float* out;
     uint32_t* iters;
-    __device__ __forceinline__ void put(uint32_t i, float d, uint32_t k, float, float) const {
+    template <class E>
+    __device__ __forceinline__ void put(uint32_t i, float d, uint32_t k, float, float, const E&) const {
         out[i] = d;
         if (iters) iters[i] = k;
     }
@@ -128,10 +129,42 @@ struct SinkState {
     float* r;
     float* dr;
     uint32_t* iters;
-    __device__ __forceinline__ void put(uint32_t i, float, uint32_t k, float rr, float drr) const {
+    template <class E>
+    __device__ __forceinline__ void put(uint32_t i, float, uint32_t k, float rr, float drr, const E&) const {
         r[i] = rr;
         dr[i] = drr;
         iters[i] = k;
+    }
+};
+
+// Gradient sink for the tangent evaluators (tangent.h): distance, unit normal,
+// grad_d-scaled gradient with respect to p and the power.
+struct SinkGrad {
+    float* d;
+    float* normals;
+    float* grad_p;
+    float* grad_power;
+    const float* grad_d;
+    uint32_t max_iters;
+    template <class E>
+    __device__ __forceinline__ void put(uint32_t i, float dv, uint32_t k, float r, float, const E& ev) const {
+        float g[4];
+        ev.grad(r, k == max_iters, g);
+        if (d) d[i] = dv;
+        if (normals) {
+            float m = __builtin_sqrtf(__builtin_fmaf(g[0], g[0], __builtin_fmaf(g[1], g[1], g[2] * g[2])));
+            float im = 1.0f / m;
+            normals[3 * (size_t)i] = g[0] * im;
+            normals[3 * (size_t)i + 1] = g[1] * im;
+            normals[3 * (size_t)i + 2] = g[2] * im;
+        }
+        float s = grad_d ? grad_d[i] : 1.0f;
+        if (grad_p) {
+            grad_p[3 * (size_t)i] = g[0] * s;
+            grad_p[3 * (size_t)i + 1] = g[1] * s;
+            grad_p[3 * (size_t)i + 2] = g[2] * s;
+        }
+        if (grad_power) grad_power[i] = g[3] * s;
     }
 };
 

@@ -72,7 +72,7 @@ __device__ __forceinline__ void wave_retire(const WorkQueue& q) {
 // Drive evaluator `Ev` over the indices of `q`, reading points from `src`
 // and writing results through `sink`.
 //   src.get(i, x, y, z, power)
-//   sink.put(i, d, iters, r, dr)
+//   sink.put(i, d, iters, r, dr, ev)  (ev: the evaluator, for derivative sinks)
 //
 // Each tick runs one loop body on every lane (results committed where the
 // lane's point is live).  A lane whose point finishes holds (r, dr) until at
@@ -95,7 +95,7 @@ __device__ __forceinline__ void refill_loop(const Src& src, const MandelUniform&
         uint64_t idle = __ballot(!live);
         if ((uint32_t)__popcll(idle) >= q.refill_min) {
             if (hold) {
-                sink.put(idx, ev.finish(r), k, r, ev.dr);
+                sink.put(idx, ev.finish(r), k, r, ev.dr, ev);
                 hold = false;
             }
             // Hand new points to idle lanes.  A point that bails before its
@@ -125,7 +125,7 @@ __device__ __forceinline__ void refill_loop(const Src& src, const MandelUniform&
                         k = 0;
                         r = ev.mag();
                         if (r > u.bailout) {
-                            sink.put(idx, ev.finish(r), 0u, r, ev.dr);
+                            sink.put(idx, ev.finish(r), 0u, r, ev.dr, ev);
                         } else {
                             live = true;
                         }

@@ -156,6 +156,20 @@ def de_vjp(shape: Shape, pts: torch.Tensor, powers: torch.Tensor | None = None, 
     return (d, gp, gpow) if with_power else (d, gp)
 
 
+def de_grad(shape: Shape, pts: torch.Tensor, powers: torch.Tensor | None = None, grad_d: torch.Tensor | None = None,
+            with_power: bool = False, ctx: Context | None = None):
+    """(d, grad_p[, grad_power]) in forward (tangent-linear) mode."""
+    ctx = ctx or shape.ctx
+    pts = _f32(pts).contiguous()
+    n = pts.shape[0]
+    d = torch.empty(n, dtype=torch.float32, device=pts.device)
+    gp = torch.empty((n, 3), dtype=torch.float32, device=pts.device)
+    gpow = torch.empty(n, dtype=torch.float32, device=pts.device) if with_power else None
+    check(lib.cth_de_grad_dev(ctx.handle, C.byref(shape.desc), _ptr(pts), _ptr(powers), n, _ptr(grad_d), _ptr(d),
+                              _ptr(gp), _ptr(gpow), _stream()))
+    return (d, gp, gpow) if with_power else (d, gp)
+
+
 def normals(shape: Shape, pts: torch.Tensor, powers: torch.Tensor | None = None, ctx: Context | None = None,
             out: torch.Tensor | None = None, d: torch.Tensor | None = None):
     """Unit normals (n, 3) from the fused forward + adjoint; returns (d, normals)."""

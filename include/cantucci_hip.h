@@ -248,9 +248,15 @@ int cth_render_depth_dev(cth_ctx* ctx, const cth_shape* shape, const cth_camera*
 int cth_de_vjp_dev(cth_ctx* ctx, const cth_shape* shape, const float* pts, const float* powers, size_t n,
                    const float* grad_d, float* d, float* grad_p, float* grad_power, void* stream);
 
-/* Surface normals from the fused forward + adjoint: normals[i] = grad / |grad|
- * (n x 3); the replacement for the 6-tap difference normal of
- * src/mesh/buffer.rs:256-266.  d (nullable) receives the distances. */
+/* The same product as cth_de_vjp_dev, computed in forward (tangent-linear)
+ * mode: the Jacobian of the orbit with respect to p (and the power) is carried
+ * through every loop body, so no orbit workspace is needed.  One launch. */
+int cth_de_grad_dev(cth_ctx* ctx, const cth_shape* shape, const float* pts, const float* powers, size_t n,
+                    const float* grad_d, float* d, float* grad_p, float* grad_power, void* stream);
+
+/* Surface normals from one fused launch (parity forward + tangent Jacobian):
+ * normals[i] = grad / |grad| (n x 3); the replacement for the 6-tap difference
+ * normal of src/mesh/buffer.rs:256-266.  d (nullable) receives the distances. */
 int cth_normals_dev(cth_ctx* ctx, const cth_shape* shape, const float* pts, const float* powers, size_t n, float* d,
                     float* normals, void* stream);
 
