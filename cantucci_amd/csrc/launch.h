@@ -40,7 +40,11 @@ inline WorkQueue make_queue(Ctx& c, uint32_t total, const LaunchShape& ls, uint3
     q.done = pair + 1;
     q.waves = blocks * waves_per_block;
     q.total = total;
-    q.chunk = ls.chunk;
+    // At least ~8 chunks per wave, so that small launches still balance:
+    // chunk = clamp(total / (8 waves), 64, configured), a multiple of 64.
+    uint64_t fair = (uint64_t)total / ((uint64_t)q.waves * 8u);
+    uint64_t ch = std::min<uint64_t>(ls.chunk, std::max<uint64_t>(64, fair));
+    q.chunk = (uint32_t)(ch & ~63ull);
     q.refill_min = ls.refill_min < 1 ? 1 : (ls.refill_min > 64 ? 64 : ls.refill_min);
     q.dynamic = ls.dynamic ? 1u : 0u;
     return q;
