@@ -60,6 +60,12 @@ CTH_KERNEL_DE = 0
 CTH_KERNEL_TRACE = 1
 CTH_KERNEL_ADJOINT = 2
 CTH_KERNEL_SPANS = 3
+CTH_KERNEL_JOBS = 4
+
+
+class CthJob(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("power", C.c_float), ("max_iters", C.c_uint32), ("start", C.c_float * 3),
+                ("end", C.c_float * 3), ("divisor", C.c_float)]
 
 
 class CthError(RuntimeError):
@@ -121,6 +127,16 @@ _SIGS = {
     "cth_normals_dev": (_I, [_P, _SH, _P, _P, _S, _P, _P, _P]),
     "cth_de_grad_dev": (_I, [_P, _SH, _P, _P, _S, _P, _P, _P, _P, _P]),
     "cth_normals_fd_dev": (_I, [_P, _SH, _P, _P, _S, _F, _P, _P, _P, _P]),
+    "cth_jobs_create": (_I, [_P, _I, _F, _P, _U, C.POINTER(_P)]),
+    "cth_jobs_destroy": (None, [_P]),
+    "cth_jobs_total": (_S, [_P]),
+    "cth_jobs_offset": (_S, [_P, _U]),
+    "cth_jobs_run_dev": (_I, [_P, _P, _P, _P, _P, _P, _P]),
+    "cth_wall_clock_khz": (_I, [_P]),
+    "cth_graph_begin": (_I, [_P]),
+    "cth_graph_end": (_I, [_P, C.POINTER(_P)]),
+    "cth_graph_launch": (_I, [_P, _P]),
+    "cth_graph_destroy": (None, [_P]),
     "cth_render_depth_dev": (_I, [_P, _SH, _P, _U, _U, _U, _U, _F, _F, _F, _P, _P, _P]),
 }
 

@@ -358,6 +358,79 @@ int cth_normals_fd_dev(cth_ctx* c, const cth_shape* s, const float* pts, const f
     return launch_fd_normals(*c, *s, pts, powers, (uint32_t)n, h, d, grad_p, normals, pick_stream(c, stream));
 }
 
+struct cth_jobset {
+    cth::JobSet* js;
+};
+
+int cth_jobs_create(cth_ctx* c, int kind, float bailout, const cth_job* jobs, uint32_t njobs, cth_jobset** out) {
+    if (!c || !jobs || !out) return fail(CTH_E_INVALID, "null argument");
+    *out = nullptr;
+    cth::JobSet* js = nullptr;
+    int rc = jobs_prepare(*c, kind, bailout, jobs, njobs, &js);
+    if (rc) return rc;
+    *out = new cth_jobset{js};
+    return CTH_OK;
+}
+
+void cth_jobs_destroy(cth_jobset* j) {
+    if (!j) return;
+    jobs_destroy(j->js);
+    delete j;
+}
+
+size_t cth_jobs_total(const cth_jobset* j) { return j ? jobs_total(j->js) : 0; }
+size_t cth_jobs_offset(const cth_jobset* j, uint32_t i) { return j ? jobs_offset(j->js, i) : 0; }
+
+int cth_jobs_run_dev(cth_ctx* c, const cth_jobset* j, float* dists, float* normals, uint64_t* done_ticks,
+                     uint64_t* start_tick, void* stream) {
+    if (!c || !j || !dists || !done_ticks || !start_tick) return fail(CTH_E_INVALID, "null argument");
+    return jobs_run(*c, *j->js, dists, normals, (unsigned long long*)done_ticks, (unsigned long long*)start_tick,
+                    pick_stream(c, stream));
+}
+
+int cth_wall_clock_khz(const cth_ctx* c) {
+    if (!c) return 0;
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeWallClockRate, c->device) != hipSuccess) return 0;
+    return v;
+}
+
+struct cth_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+int cth_graph_begin(void* stream) {
+    if (!stream) return fail(CTH_E_INVALID, "graph capture needs a non-null stream");
+    return check_hip(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal), "begin capture");
+}
+
+int cth_graph_end(void* stream, cth_graph** out) {
+    if (!stream || !out) return fail(CTH_E_INVALID, "null argument");
+    *out = nullptr;
+    cth_graph* g = new cth_graph();
+    int rc = check_hip(hipStreamEndCapture((hipStream_t)stream, &g->graph), "end capture");
+    if (!rc) rc = check_hip(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0), "graph instantiate");
+    if (rc) {
+        cth_graph_destroy(g);
+        return rc;
+    }
+    *out = g;
+    return CTH_OK;
+}
+
+int cth_graph_launch(cth_graph* g, void* stream) {
+    if (!g || !g->exec) return fail(CTH_E_INVALID, "null graph");
+    return check_hip(hipGraphLaunch(g->exec, (hipStream_t)stream), "graph launch");
+}
+
+void cth_graph_destroy(cth_graph* g) {
+    if (!g) return;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+}
+
 int cth_sample_spans(cth_ctx* c, const cth_shape* s, const cth_span* spans, size_t nspans, uint32_t res,
                      float* dists, uint64_t* cross_bits) {
     if (!c) return fail(CTH_E_INVALID, "null context");

@@ -41,12 +41,14 @@ struct Ctx {
     }
     LaunchShape shape_trace{256, 16, 256, 32, 0};
     LaunchShape shape_adj{256, 16, 256, 32, 0};
+    LaunchShape shape_jobs{256, 8, 256, 32, 0};
     LaunchShape* launch_shape(int kernel) {
         switch (kernel) {
             case CTH_KERNEL_DE: return &shape_de;
             case CTH_KERNEL_TRACE: return &shape_trace;
             case CTH_KERNEL_ADJOINT: return &shape_adj;
             case CTH_KERNEL_SPANS: return &shape_spans;
+            case CTH_KERNEL_JOBS: return &shape_jobs;
             default: return nullptr;
         }
     }
@@ -84,6 +86,13 @@ int launch_tangent(Ctx& c, const cth_shape& s, const float* pts, const float* po
                    const float* grad_d, float* d, float* grad_p, float* grad_power, float* normals, hipStream_t st);
 int launch_fd_normals(Ctx& c, const cth_shape& s, const float* pts, const float* powers, uint32_t n, float h,
                       float* d, float* grad_p, float* normals, hipStream_t st);
+struct JobSet;
+int jobs_prepare(Ctx& c, int kind, float bailout, const cth_job* jobs, uint32_t njobs, JobSet** out);
+void jobs_destroy(JobSet* js);
+uint32_t jobs_total(const JobSet* js);
+uint32_t jobs_offset(const JobSet* js, uint32_t i);
+int jobs_run(Ctx& c, const JobSet& js, float* dists, float* normals, unsigned long long* done,
+             unsigned long long* start, hipStream_t st);
 int launch_contains_sphere(Ctx& c, const cth_shape& s, const float* pts, uint32_t n, uint8_t* out, hipStream_t st);
 int launch_less_than_zero(Ctx& c, const float* d, uint32_t n, uint8_t* out, hipStream_t st);
 int launch_dup_pairs(Ctx& c, const float* d, uint32_t n, float* pairs, hipStream_t st);

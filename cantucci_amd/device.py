@@ -223,3 +223,70 @@ class MandelbulbDistance(torch.autograd.Function):
 def distance(shape: Shape, pts: torch.Tensor, powers: torch.Tensor | None = None) -> torch.Tensor:
     """Differentiable distance estimate (autograd through the adjoint kernel)."""
     return MandelbulbDistance.apply(pts, powers, shape)
+
+
+class JobSet:
+    """A batch of independent grid jobs (`cth_jobset`)."""
+
+    def __init__(self, ctx: Context, kind: int, bailout: float, jobs):
+        arr = (_lib.CthJob * len(jobs))()
+        for a, j in zip(arr, jobs):
+            a.n = j["n"]
+            a.power = j["power"]
+            a.max_iters = j["max_iters"]
+            a.start[:] = list(j.get("start", (-1.5, -1.5, -1.5)))
+            a.end[:] = list(j.get("end", (1.5, 1.5, 1.5)))
+            a.divisor = j.get("divisor", j["n"] - 1.0)
+        h = C.c_void_p()
+        check(lib.cth_jobs_create(ctx.handle, kind, bailout, arr, len(jobs), C.byref(h)))
+        self.ctx = ctx
+        self.handle = h
+        self.njobs = len(jobs)
+        self.total = lib.cth_jobs_total(h)
+        self.offsets = [lib.cth_jobs_offset(h, i) for i in range(self.njobs)] + [self.total]
+
+    def run(self, dists: torch.Tensor, normals: torch.Tensor | None, done: torch.Tensor, start: torch.Tensor,
+            stream: int | None = None):
+        check(lib.cth_jobs_run_dev(self.ctx.handle, self.handle, _ptr(dists), _ptr(normals), _ptr(done), _ptr(start),
+                                   _stream() if stream is None else stream))
+
+    def close(self):
+        if self.handle:
+            lib.cth_jobs_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Graph:
+    """HIP graph captured from a callable that issues `_dev` calls on `stream`."""
+
+    def __init__(self, fn, stream: torch.cuda.Stream):
+        self.stream = stream
+        with torch.cuda.stream(stream):
+            check(lib.cth_graph_begin(stream.cuda_stream))
+            try:
+                fn()
+            finally:
+                h = C.c_void_p()
+                rc = lib.cth_graph_end(stream.cuda_stream, C.byref(h))
+            check(rc)
+        self.handle = h
+
+    def launch(self):
+        check(lib.cth_graph_launch(self.handle, self.stream.cuda_stream))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib.cth_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

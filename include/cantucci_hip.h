@@ -132,7 +132,7 @@ typedef struct cth_launch {
     uint32_t dynamic;
 } cth_launch;
 
-enum { CTH_KERNEL_DE = 0, CTH_KERNEL_TRACE = 1, CTH_KERNEL_ADJOINT = 2, CTH_KERNEL_SPANS = 3 };
+enum { CTH_KERNEL_DE = 0, CTH_KERNEL_TRACE = 1, CTH_KERNEL_ADJOINT = 2, CTH_KERNEL_SPANS = 3, CTH_KERNEL_JOBS = 4 };
 
 int cth_ctx_set_launch(cth_ctx* ctx, int kernel, const cth_launch* launch);
 int cth_ctx_get_launch(const cth_ctx* ctx, int kernel, cth_launch* launch);
@@ -266,6 +266,52 @@ int cth_normals_dev(cth_ctx* ctx, const cth_shape* shape, const float* pts, cons
  * grad_p = differences / (2h) (nullable), d = DE(p) (nullable). */
 int cth_normals_fd_dev(cth_ctx* ctx, const cth_shape* shape, const float* pts, const float* powers, size_t n, float h,
                        float* d, float* normals, float* grad_p, void* stream);
+
+/* ---- batched jobs and graphs -------------------------------------------------- */
+
+/* One independent grid job of a batch: an n^3 grid over [start, end] in
+ * GridTable order, p = start + (end - start) * (i / divisor), with its own
+ * power and iteration cap.  The batch generalises the reference's per-leaf
+ * thread-pool jobs (src/mesh/mod.rs:129-161). */
+typedef struct cth_job {
+    uint32_t n;
+    float power;
+    uint32_t max_iters;
+    float start[3];
+    float end[3];
+    float divisor;
+} cth_job;
+
+typedef struct cth_jobset cth_jobset;
+
+/* Upload a batch (1..1024 jobs, < 2^31 points).  kind CTH_SHAPE_MANDELBULB_TRIG
+ * (real powers) or CTH_SHAPE_MANDELBULB (every power 8: the P8 polynomial);
+ * bailout is shared. */
+int cth_jobs_create(cth_ctx* ctx, int kind, float bailout, const cth_job* jobs, uint32_t njobs, cth_jobset** out);
+void cth_jobs_destroy(cth_jobset* jobs);
+/* Total points, and the first index of job i in the concatenated outputs. */
+size_t cth_jobs_total(const cth_jobset* jobs);
+size_t cth_jobs_offset(const cth_jobset* jobs, uint32_t i);
+
+/* Run every job in one persistent launch: dists (total floats), normals
+ * (nullable, 3 x total; forward-mode gradient, unit length), done_ticks
+ * (device, one per job: wall-clock stamp of the job's last write-back),
+ * start_tick (device, one: earliest wave start).  Capture-safe. */
+int cth_jobs_run_dev(cth_ctx* ctx, const cth_jobset* jobs, float* dists, float* normals, uint64_t* done_ticks,
+                     uint64_t* start_tick, void* stream);
+
+/* Wall-clock counter rate of the device (kHz), for done_ticks. */
+int cth_wall_clock_khz(const cth_ctx* ctx);
+
+/* Stream capture into a HIP graph.  Between begin and end on a non-null
+ * stream, the capture-safe `_dev` entry points (distance, grid, span
+ * sampling, tracing, tangent gradients/normals, jobs) only record their
+ * launches; the graph then replays them with one launch call. */
+typedef struct cth_graph cth_graph;
+int cth_graph_begin(void* stream);
+int cth_graph_end(void* stream, cth_graph** out);
+int cth_graph_launch(cth_graph* graph, void* stream);
+void cth_graph_destroy(cth_graph* graph);
 
 /* Host-buffer variant of cth_sample_spans_dev (synchronous). */
 int cth_sample_spans(cth_ctx* ctx, const cth_shape* shape, const cth_span* spans, size_t nspans, uint32_t res,
