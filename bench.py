@@ -16,7 +16,6 @@ from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -60,8 +59,11 @@ def main() -> None:
     from cantucci_amd import device as dev
     from cantucci_amd import workloads as wl
 
+    from cantucci_amd import autotune
+
     c2 = wl.C2
     ctx = ca.Context(0, args.mode)
+    tuned = autotune.apply(ctx, "spans")
     shape = ca.Mandelbulb.classic(c2["max_iters"], c2["bailout"], ctx=ctx)
     hd = wl.half_diagonals(3.0, c2["max_depth"])
     res = c2["res"]
@@ -154,6 +156,13 @@ def main() -> None:
                "sample": f"{m2} of {nleaves} depth-8 surface leaves (seeded), 9^3 corners each, "
                          f"oracle restatement, {threads} threads, {dt:.1f} s"}
 
+    extra = {}
+    for name, fn in (("c1_grid", bench_c1), ("c3_trace", bench_c3), ("c4_normals", bench_c4), ("c5_jobs", bench_c5)):
+        try:
+            extra[name] = fn(ctx, args)
+        except Exception as e:  # a secondary leg must not hide the headline
+            extra[name] = {"error": f"{type(e).__name__}: {e}"}
+
     line = {
         "metric": "Mandelbulb DE evals/s (p=8,15 it) + 1080p rays/s; fwd+adjoint vs fwd+6-tap FD",
         "value": value,
@@ -175,12 +184,189 @@ def main() -> None:
             "center_evals": center_evals,
             "corner_evals": corner_evals,
             "level_counts": [int(x) for x in lc],
+            "launch": tuned,
             "parallelism": "single GPU",
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "extra": extra,
     }
     print(json.dumps(line))
+
+
+def _time_ms(fn, reps):
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return 1e3 * (time.perf_counter() - t) / reps
+
+
+def bench_c1(ctx, args):
+    """64^3 grid over [-1.5,1.5]^3 (divisor 63), P8 / 15 / 2: one launch is a
+    few microseconds of work, so 100 launches are captured in a HIP graph."""
+    import ctypes as C
+
+    import torch
+
+    import cantucci_amd as ca
+    from cantucci_amd import autotune
+    from cantucci_amd import device as dev
+    from cantucci_amd._lib import check, lib
+
+    autotune.apply(ctx, "grid")
+    shape = ca.Mandelbulb.classic(15, 2.0, ctx=ctx)
+    n = 64
+    out = torch.empty(n ** 3, device="cuda")
+    s = torch.cuda.Stream()
+    st = (C.c_float * 3)(-1.5, -1.5, -1.5)
+    en = (C.c_float * 3)(1.5, 1.5, 1.5)
+    launches = 100
+
+    def body():
+        for _ in range(launches):
+            check(lib.cth_de_grid_dev(ctx.handle, C.byref(shape.desc), st, en, n, 63.0, out.data_ptr(), None,
+                                      s.cuda_stream))
+
+    g = dev.Graph(body, s)
+    g.launch()
+    s.synchronize()
+    reps = 5
+    t = time.perf_counter()
+    for _ in range(reps):
+        g.launch()
+    s.synchronize()
+    per_launch = (time.perf_counter() - t) / (reps * launches)
+    single = _time_ms(lambda: dev.de_grid(shape, (-1.5,) * 3, (1.5,) * 3, n, 63.0), 20)
+    g.close()
+    return {"workload": "64^3 grid, P8, 15 it, bailout 2", "evals_per_s": n ** 3 / per_launch,
+            "us_per_launch_graph": per_launch * 1e6, "us_per_launch_eager": single * 1e3}
+
+
+def bench_c3(ctx, args):
+    """32 seeded orbit poses x 1920x1080 primary rays, 256 steps, hit d < 1e-4 t, escape t > 10."""
+    import torch
+
+    import cantucci_amd as ca
+    from cantucci_amd import autotune
+    from cantucci_amd import device as dev
+    from cantucci_amd import workloads as wl
+
+    autotune.apply(ctx, "trace")
+    c3 = wl.C3
+    shape = ca.Mandelbulb.classic(c3["max_iters"], c3["bailout"], ctx=ctx)
+    cams = torch.from_numpy(wl.orbit_poses(c3["poses"], c3["seed"], c3["W"], c3["H"], c3["fov_deg"])).cuda()
+    depth = torch.empty((c3["poses"], c3["H"], c3["W"]), device="cuda")
+    steps = torch.empty((c3["poses"], c3["H"], c3["W"]), dtype=torch.int32, device="cuda")
+
+    def run():
+        dev.render_depth(shape, cams, c3["W"], c3["H"], c3["max_steps"], 0.0, c3["eps_rel"], c3["t_max"],
+                         depth=depth, steps=steps)
+
+    ms = _time_ms(run, 5)
+    rays = c3["poses"] * c3["W"] * c3["H"]
+    return {"workload": "32 poses x 1920x1080, P8 15 it, 256 steps, eps 1e-4 t, t_max 10",
+            "rays_per_s": rays / (ms / 1e3), "ms": ms,
+            "hit_fraction": float(torch.isfinite(depth).float().mean().item()),
+            "mean_steps": float(steps.float().mean().item()),
+            "de_evals_per_s": float(steps.to(torch.float64).sum().item()) / (ms / 1e3)}
+
+
+def bench_c4(ctx, args):
+    """2^22 points in the shell 0.9 < |p| < 1.3 (seed 1), power U[6,10], 15 it:
+    distance + normal by the fused forward+tangent kernel, by the reverse-sweep
+    VJP, and by the 6-tap central difference (h = 1e-3)."""
+    import numpy as np
+    import torch
+
+    import cantucci_amd as ca
+    from cantucci_amd import autotune
+    from cantucci_amd import device as dev
+    from cantucci_amd import workloads as wl
+
+    c4 = wl.C4
+    pts, powers = wl.shell_points(c4["n"], c4["seed"])
+    P = torch.from_numpy(pts).cuda()
+    W = torch.from_numpy(powers).cuda()
+    shape = ca.MandelbulbTrig(c4["max_iters"], c4["bailout"], 8.0, ctx=ctx)
+    nrm = torch.empty((c4["n"], 3), device="cuda")
+    d = torch.empty(c4["n"], device="cuda")
+    autotune.apply(ctx, "tangent")
+    t_tan = _time_ms(lambda: dev.normals(shape, P, W, out=nrm, d=d), 5)
+    n_tan = nrm.clone()
+    t_vjp = _time_ms(lambda: dev.de_vjp(shape, P, W), 3)
+    autotune.apply(ctx, "fd")
+    t_fwd = _time_ms(lambda: dev.de_points(shape, P, W), 5)
+    t_fd = _time_ms(lambda: dev.normals_fd(shape, P, c4["fd_h"], W, out=nrm, d=d), 5)
+    n_fd = nrm.clone()
+    # angular error against the FP64 dual-number oracle on a seeded subsample
+    res = {"workload": "2^22 shell points, power U[6,10], 15 it", "ms_forward": t_fwd,
+           "ms_fwd_tangent_normals": t_tan, "ms_fwd_reverse_vjp": t_vjp, "ms_fwd_6tap_fd": t_fd,
+           "speedup_tangent_vs_fd": t_fd / t_tan, "speedup_vjp_vs_fd": t_fd / t_vjp}
+    try:
+        import oracle as O
+        idx = np.sort(np.random.default_rng(0).choice(c4["n"], 20000, replace=False))
+        _, g64, it64 = O.de_grad_f64(O.mandelbulb_trig(c4["max_iters"], c4["bailout"]), pts[idx], powers[idx])
+        _, it_o, _, _ = O.de_points(O.mandelbulb_trig(c4["max_iters"], c4["bailout"]), pts[idx], powers[idx])
+
+        def ang(a):
+            a = a.astype(np.float64)
+            b = g64[:, :3]
+            c = (a * b).sum(1) / (np.linalg.norm(a, axis=1) * np.linalg.norm(b, axis=1))
+            return np.arccos(np.clip(c, -1, 1))
+
+        for tag, nn in (("tangent", n_tan), ("fd", n_fd)):
+            a = ang(nn.cpu().numpy()[idx])
+            res[f"angle_rad_{tag}"] = {"p50": float(np.median(a)), "p99": float(np.percentile(a, 99)),
+                                       "frac_le_1e-3": float(np.mean(a <= 1e-3))}
+        res["oracle_sample"] = "20000 seeded points, FP64 forward-mode dual numbers"
+    except Exception as e:
+        res["angle_error"] = str(e)
+    return res
+
+
+def bench_c5(ctx, args):
+    """64 jobs: N from {32,48,64,96,128}, power U[2,12], cap U{8..30} (seed 5);
+    distances + unit normals in one persistent launch (captured in a HIP graph)."""
+    import numpy as np
+    import torch
+
+    from cantucci_amd import _lib, autotune
+    from cantucci_amd import device as dev
+    from cantucci_amd import workloads as wl
+
+    autotune.apply(ctx, "jobs")
+    N, Pw, cap = wl.mixed_jobs(**{k: v for k, v in wl.C5.items() if k not in ("bailout",)})
+    js = dev.JobSet(ctx, _lib.CTH_SHAPE_MANDELBULB_TRIG, wl.C5["bailout"],
+                    [dict(n=int(n), power=float(p), max_iters=int(c)) for n, p, c in zip(N, Pw, cap)])
+    dists = torch.empty(js.total, device="cuda")
+    nrm = torch.empty((js.total, 3), device="cuda")
+    done = torch.zeros(js.njobs, dtype=torch.int64, device="cuda")
+    start = torch.zeros(1, dtype=torch.int64, device="cuda")
+    s = torch.cuda.Stream()
+    g = dev.Graph(lambda: js.run(dists, nrm, done, start, stream=s.cuda_stream), s)
+    g.launch()
+    s.synchronize()
+    reps = 5
+    lat = []
+    t = time.perf_counter()
+    for _ in range(reps):
+        g.launch()
+        s.synchronize()
+        ticks = done.cpu().numpy().astype(np.float64) - float(start.cpu().numpy()[0])
+        lat.append(ticks)
+    elapsed = (time.perf_counter() - t) / reps
+    khz = _lib.lib.cth_wall_clock_khz(ctx.handle) or 100000
+    lat_ms = np.concatenate(lat) / khz
+    g.close()
+    js.close()
+    return {"workload": "64 mixed jobs (seed 5), trig real power, f32 distances + normals",
+            "points": int(js.total), "evals_per_s": js.total / elapsed, "ms_per_batch": elapsed * 1e3,
+            "job_latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99)),
+                               "max": float(lat_ms.max())}}
 
 
 if __name__ == "__main__":

@@ -36,7 +36,7 @@ KERNEL_OF = {
     "spans": _lib.CTH_KERNEL_SPANS,
     "trace": _lib.CTH_KERNEL_TRACE,
     "tangent": _lib.CTH_KERNEL_ADJOINT,
-    "jobs": _lib.CTH_KERNEL_JOBS if hasattr(_lib, "CTH_KERNEL_JOBS") else _lib.CTH_KERNEL_DE,
+    "jobs": _lib.CTH_KERNEL_JOBS,
 }
 
 
@@ -133,13 +133,22 @@ def _workloads(ctx):
     dd = torch.empty(P.shape[0], device="cuda")
     runs["tangent"] = lambda: dev.normals(tr, P, W, out=nrm, d=dd)
     runs["fd"] = lambda: dev.normals_fd(tr, P, 1e-3, W, out=nrm, d=dd)
+    N, Pw, cap = wl.mixed_jobs(16, seed=5)
+    js = dev.JobSet(ctx, _lib.CTH_SHAPE_MANDELBULB_TRIG, 2.0,
+                    [dict(n=int(n), power=float(p), max_iters=int(c)) for n, p, c in zip(N, Pw, cap)])
+    jd = torch.empty(js.total, device="cuda")
+    jn = torch.empty((js.total, 3), device="cuda")
+    done = torch.zeros(js.njobs, dtype=torch.int64, device="cuda")
+    st = torch.zeros(1, dtype=torch.int64, device="cuda")
+    runs["jobs"] = lambda: js.run(jd, jn, done, st)
+    runs["_keep"] = js
     return runs
 
 
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=str(CACHE))
-    ap.add_argument("--workloads", default="grid,spans,trace,tangent,fd")
+    ap.add_argument("--workloads", default="grid,spans,trace,tangent,fd,jobs")
     ap.add_argument("--mode", default="parity")
     args = ap.parse_args()
     from .shape import Context
@@ -153,7 +162,6 @@ def main() -> None:
     for name in args.workloads.split(","):
         t = time.time()
         best, results = tune(ctx, KERNEL_OF[name], runs[name])
-        default_ms = None
         cache[key][name] = {
             "config": best,
             "best_ms": results[0][0],
@@ -163,7 +171,6 @@ def main() -> None:
         }
         print(f"{name}: best {best} {results[0][0]:.3f} ms (worst {results[-1][0]:.3f} ms, "
               f"{len(results)} shapes, {time.time() - t:.1f} s)", flush=True)
-        del default_ms
     path.write_text(json.dumps(cache, indent=1, sort_keys=True))
     print(f"wrote {path}")
 
