@@ -16,6 +16,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import re
 import os
 import sys
 import time
@@ -114,12 +115,12 @@ def main() -> None:
     del iters
     achieved = flop / kernel_s
     traffic = None
-    tf = ROOT / "profiles" / "spans_c2_traffic.json"
+    # HBM bytes per launch of the span kernel from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE summary (copy-kernel calibrated).
+    tf = ROOT / "profiles" / "spans_c2_traffic.md"
     if tf.exists():
-        try:
-            traffic = json.loads(tf.read_text()).get("bytes_per_launch")
-        except Exception:
-            traffic = None
+        m = re.search(r"^bytes_per_launch:\s*(\d+)", tf.read_text(), re.M)
+        traffic = int(m.group(1)) if m else None
     roofline = {
         "bound": "valu",
         "achieved": achieved / 1e12,
