@@ -316,7 +316,7 @@ int cth_de_vjp_dev(cth_ctx* c, const cth_shape* s, const float* pts, const float
     hipStream_t st = pick_stream(c, stream);
     float* gp = grad_power;
     if (grad_power && s->kind != CTH_SHAPE_MANDELBULB_TRIG) {
-        int rc = check_hip(hipMemsetAsync(grad_power, 0, n * sizeof(float), st), "memset grad_power");
+        int rc = launch_fill_u32(reinterpret_cast<uint32_t*>(grad_power), n, 0u, st);
         if (rc) return rc;
         gp = nullptr;
     }
@@ -342,7 +342,7 @@ int cth_de_grad_dev(cth_ctx* c, const cth_shape* s, const float* pts, const floa
     hipStream_t st = pick_stream(c, stream);
     float* gp = grad_power;
     if (grad_power && s->kind != CTH_SHAPE_MANDELBULB_TRIG) {
-        int rc = check_hip(hipMemsetAsync(grad_power, 0, n * sizeof(float), st), "memset grad_power");
+        int rc = launch_fill_u32(reinterpret_cast<uint32_t*>(grad_power), n, 0u, st);
         if (rc) return rc;
         gp = nullptr;
     }
@@ -457,6 +457,74 @@ int cth_sample_spans(cth_ctx* c, const cth_shape* s, const cth_span* spans, size
         rc = check_hip(hipMemcpyAsync(cross_bits, dc.p, nspans * words * sizeof(uint64_t), hipMemcpyDeviceToHost,
                                       c->stream),
                        "D2H");
+    int rs = check_hip(hipStreamSynchronize(c->stream), "sync");
+    return rc ? rc : rs;
+}
+
+int cth_mesh_count_dev(cth_ctx* c, const float* dists, size_t nspans, uint32_t res, uint64_t* voff, uint64_t* ioff,
+                       uint64_t* nverts, uint64_t* nidx, void* stream) {
+    if (!c) return fail(CTH_E_INVALID, "null context");
+    if (!voff || !ioff || !nverts || !nidx || (nspans && !dists)) return fail(CTH_E_INVALID, "null buffer");
+    if (nspans >= (size_t(1) << 31)) return fail(CTH_E_INVALID, "more than 2^31 spans");
+    uint64_t tot[2] = {0, 0};
+    int rc = mesh_count(*c, dists, (uint32_t)nspans, res, voff, ioff, tot, pick_stream(c, stream));
+    *nverts = tot[0];
+    *nidx = tot[1];
+    return rc;
+}
+
+int cth_mesh_emit_dev(cth_ctx* c, const cth_shape* s, const float* spans, const float* dists, size_t nspans,
+                      uint32_t res, int normals, const uint64_t* voff, const uint64_t* ioff, uint64_t nverts,
+                      cth_vertex* vertices, uint32_t* indices, void* stream) {
+    if (!c) return fail(CTH_E_INVALID, "null context");
+    if (int rc = check_shape(s)) return rc;
+    if (nspans && (!spans || !dists || !voff || !ioff || !indices || (nverts && !vertices)))
+        return fail(CTH_E_INVALID, "null buffer");
+    if (nspans >= (size_t(1) << 31)) return fail(CTH_E_INVALID, "more than 2^31 spans");
+    return mesh_emit(*c, *s, spans, dists, (uint32_t)nspans, res, normals, voff, ioff, nverts,
+                     reinterpret_cast<float*>(vertices), indices, pick_stream(c, stream));
+}
+
+int cth_mesh_span(cth_ctx* c, const cth_shape* s, const cth_span* span, uint32_t res, int normals,
+                  cth_vertex* vertices, size_t vert_cap, uint32_t* indices, size_t idx_cap, size_t* nverts,
+                  size_t* nidx) {
+    if (!c) return fail(CTH_E_INVALID, "null context");
+    if (int rc = check_shape(s)) return rc;
+    if (!span || !nverts || !nidx) return fail(CTH_E_INVALID, "null buffer");
+    for (int a = 0; a < 3; ++a)
+        if (!(span->start[a] < span->end[a])) return fail(CTH_E_INVALID, "span start must be below end on every axis");
+    if (res == 0 || (res & (res - 1)) != 0 || res > 64)
+        return fail(CTH_E_INVALID, "mesh resolution must be a power of two in [1, 64]");
+    if (normals != CTH_NORMALS_FD6 && normals != CTH_NORMALS_TANGENT) return fail(CTH_E_INVALID, "unknown normals mode");
+    if (normals == CTH_NORMALS_TANGENT && s->kind == CTH_SHAPE_SPHERE)
+        return fail(CTH_E_UNSUPPORTED, "tangent normals need a Mandelbulb kind");
+    const size_t per = size_t(res + 1) * (res + 1) * (res + 1);
+    DevBuf dsp, dd, doff, dv, di;
+    int rc = dsp.alloc(sizeof(cth_span));
+    if (!rc) rc = dd.alloc(per * sizeof(float));
+    if (!rc) rc = doff.alloc(4 * sizeof(uint64_t));
+    if (rc) return rc;
+    rc = check_hip(hipMemcpyAsync(dsp.p, span, sizeof(cth_span), hipMemcpyHostToDevice, c->stream), "H2D");
+    if (!rc) rc = cth_sample_spans_dev(c, s, (const float*)dsp.p, 1, res, (float*)dd.p, nullptr, nullptr, c->stream);
+    uint64_t nv = 0, ni = 0;
+    uint64_t* off = (uint64_t*)doff.p;
+    if (!rc) rc = cth_mesh_count_dev(c, (const float*)dd.p, 1, res, off, off + 2, &nv, &ni, c->stream);
+    if (rc) return rc;
+    *nverts = (size_t)nv;
+    *nidx = (size_t)ni;
+    if (!vertices && !indices) return CTH_OK;
+    if ((vertices && vert_cap < nv) || (indices && idx_cap < ni) || (nv && !vertices) || (ni && !indices))
+        return fail(CTH_E_INVALID, "vertex or index buffer too small for the mesh");
+    rc = dv.alloc(nv * sizeof(cth_vertex));
+    if (!rc) rc = di.alloc(ni * sizeof(uint32_t));
+    if (!rc)
+        rc = cth_mesh_emit_dev(c, s, (const float*)dsp.p, (const float*)dd.p, 1, res, normals, off, off + 2, nv,
+                               (cth_vertex*)dv.p, (uint32_t*)di.p, c->stream);
+    if (!rc && nv)
+        rc = check_hip(hipMemcpyAsync(vertices, dv.p, nv * sizeof(cth_vertex), hipMemcpyDeviceToHost, c->stream),
+                       "D2H");
+    if (!rc && ni)
+        rc = check_hip(hipMemcpyAsync(indices, di.p, ni * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream), "D2H");
     int rs = check_hip(hipStreamSynchronize(c->stream), "sync");
     return rc ? rc : rs;
 }

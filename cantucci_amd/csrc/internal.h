@@ -93,6 +93,12 @@ uint32_t jobs_total(const JobSet* js);
 uint32_t jobs_offset(const JobSet* js, uint32_t i);
 int jobs_run(Ctx& c, const JobSet& js, float* dists, float* normals, unsigned long long* done,
              unsigned long long* start, hipStream_t st);
+int mesh_count(Ctx& c, const float* dists, uint32_t nleaves, uint32_t res, uint64_t* voff, uint64_t* ioff,
+               uint64_t totals[2], hipStream_t st);
+int mesh_emit(Ctx& c, const cth_shape& s, const float* spans, const float* dists, uint32_t nleaves, uint32_t res,
+              int normals, const uint64_t* voff, const uint64_t* ioff, uint64_t nverts, float* verts,
+              uint32_t* idx, hipStream_t st);
+int launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t st);
 int launch_contains_sphere(Ctx& c, const cth_shape& s, const float* pts, uint32_t n, uint8_t* out, hipStream_t st);
 int launch_less_than_zero(Ctx& c, const float* d, uint32_t n, uint8_t* out, hipStream_t st);
 int launch_dup_pairs(Ctx& c, const float* d, uint32_t n, float* pairs, hipStream_t st);

@@ -1,0 +1,510 @@
+// Naive surface nets over a batch of sampled octree leaves: steps 2 and 3 of
+// MeshBuffer::naive_surface_nets (src/mesh/buffer.rs:97-372), given the
+// step-1 corner distances that cth_sample_spans_dev wrote.
+//
+// Two passes over the leaves, one workgroup per leaf:
+//   count: per leaf, the number of crossing cells (one vertex each) and of
+//          emitted quads (6 indices each); an exclusive scan turns the counts
+//          into the leaves' offsets in the concatenated outputs.
+//   emit:  per leaf, the crossing mask of every cell (one ballot per 64
+//          cells, kept in LDS with its running popcount), so that the vertex
+//          index of any cell -- `points[(x, y, z)]` of the reference -- is a
+//          popcount; then the cells in cube() order, tile by tile: vertex
+//          positions (edge crossings in EDGES order, inverted weights,
+//          centroid) and the quads of the +x/+y/+z edges of each corner.
+// The vertex metadata -- DE at the vertex and its normal -- are distance
+// evaluations again, so they run in the lane-refill loop afterwards: either
+// the reference's 6-tap central difference (7 evaluations per vertex,
+// :254-266) or the forward-mode gradient (one tangent evaluation).
+//
+// Vertices use the reference's `Vertex` layout (src/mesh/mod.rs:255-261):
+// position[3], normal[3], distance_from_surface, 28 bytes.  Indices are u32
+// local to the leaf, exactly as each leaf's MeshBuffer holds them.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+
+#include "internal.h"
+#include "launch.h"
+#include "point_sources.h"
+#include "tangent.h"
+#include "work_loop.h"
+
+namespace cth {
+
+namespace {
+
+constexpr uint32_t kMeshBlock = 256;
+
+__device__ __forceinline__ uint32_t sign_bit(float d) { return __float_as_uint(d) >> 31; }
+
+// The eight corner distances of cell (x, y, z) in the reference's corner
+// order (src/mesh/buffer.rs:116-125) and their IEEE sign bits (bit k =
+// corner k is negative, `!is_sign_positive`).
+__device__ __forceinline__ uint32_t load_cell(const float* __restrict__ D, uint32_t n1, uint32_t x, uint32_t y,
+                                              uint32_t z, float d[8]) {
+    const size_t sy = n1, sx = (size_t)n1 * n1;
+    const size_t b = (size_t)x * sx + (size_t)y * sy + z;
+    d[0] = D[b];
+    d[1] = D[b + 1];
+    d[2] = D[b + sy];
+    d[3] = D[b + sy + 1];
+    d[4] = D[b + sx];
+    d[5] = D[b + sx + 1];
+    d[6] = D[b + sx + sy];
+    d[7] = D[b + sx + sy + 1];
+    uint32_t m = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) m |= sign_bit(d[k]) << k;
+    return m;
+}
+
+__device__ __forceinline__ bool crossing(uint32_t m) { return m != 0u && m != 0xFFu; }
+
+// Quads of corner (x, y, z) (src/mesh/buffer.rs:299-363): bit 0 the +x edge
+// (y, z > 0), bit 1 +y (x, z > 0), bit 2 +z (x, y > 0); an edge emits when
+// the signs of its two corners differ (corners 0-4, 0-2, 0-1 of the cell).
+__device__ __forceinline__ uint32_t face_bits(uint32_t m, uint32_t x, uint32_t y, uint32_t z) {
+    uint32_t b0 = m & 1u;
+    uint32_t fx = (y > 0 && z > 0 && b0 != ((m >> 4) & 1u)) ? 1u : 0u;
+    uint32_t fy = (x > 0 && z > 0 && b0 != ((m >> 2) & 1u)) ? 2u : 0u;
+    uint32_t fz = (x > 0 && y > 0 && b0 != ((m >> 1) & 1u)) ? 4u : 0u;
+    return fx | fy | fz;
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    const uint32_t lane = __lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t t = __shfl_up(v, d, 64);
+        if (lane >= (uint32_t)d) v += t;
+    }
+    return v;
+}
+
+// Exclusive scan over the workgroup (all threads must call it); `total`
+// receives the sum.  `wsum` holds one slot per wave.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wsum, uint32_t& total) {
+    const uint32_t inc = wave_incl_scan(v);
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    if (lane == 63u) wsum[wave] = inc;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+    const uint32_t nw = blockDim.x >> 6;
+    for (uint32_t w = 0; w < nw; ++w) {
+        uint32_t s = wsum[w];
+        off += (w < wave) ? s : 0u;
+        tot += s;
+    }
+    __syncthreads();
+    total = tot;
+    return off + inc - v;
+}
+
+__device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* wsum) {
+    uint32_t t;
+    (void)block_excl_scan(v, wsum, t);
+    return t;
+}
+
+struct MeshDims {
+    uint32_t res, lg, n1, cells;
+    float fres;
+};
+
+__device__ __forceinline__ void cell_xyz(const MeshDims& m, uint32_t c, uint32_t& x, uint32_t& y, uint32_t& z) {
+    z = c & (m.res - 1u);
+    y = (c >> m.lg) & (m.res - 1u);
+    x = c >> (2u * m.lg);
+}
+
+// Per leaf: vertex count and index count (6 per quad).  Slot `nleaves` of
+// both count arrays is zeroed so that an exclusive scan over nleaves + 1
+// entries ends with the totals.
+__global__ __launch_bounds__(kMeshBlock) void k_mesh_count(const float* __restrict__ dists, MeshDims m,
+                                                           uint32_t nleaves, uint64_t* __restrict__ vcnt,
+                                                           uint64_t* __restrict__ icnt) {
+    __shared__ uint32_t wsum[kMeshBlock / 64];
+    const uint32_t leaf = blockIdx.x;
+    const float* D = dists + (size_t)leaf * m.n1 * m.n1 * m.n1;
+    uint32_t nv = 0, nf = 0;
+    for (uint32_t c = threadIdx.x; c < m.cells; c += blockDim.x) {
+        uint32_t x, y, z;
+        cell_xyz(m, c, x, y, z);
+        float d[8];
+        uint32_t mk = load_cell(D, m.n1, x, y, z, d);
+        nv += crossing(mk) ? 1u : 0u;
+        nf += (uint32_t)__popc(face_bits(mk, x, y, z));
+    }
+    nv = block_sum(nv, wsum);
+    nf = block_sum(nf, wsum);
+    if (threadIdx.x == 0) {
+        vcnt[leaf] = nv;
+        icnt[leaf] = 6ull * nf;
+        if (leaf == 0) {
+            vcnt[nleaves] = 0;
+            icnt[nleaves] = 0;
+        }
+    }
+}
+
+// Vertex position of a crossing cell (src/mesh/buffer.rs:149-250), in the
+// reference's f32 operation order.  s1: adjusted span start; step: cell size.
+__device__ __forceinline__ void place_vertex(const float d[8], uint32_t x, uint32_t y, uint32_t z, const float s1[3],
+                                             const float step[3], float p[3]) {
+    const float p0[3] = {s1[0] + (float)x * step[0], s1[1] + (float)y * step[1], s1[2] + (float)z * step[2]};
+    // EDGES (src/mesh/buffer.rs:155-176), encoded as from | to << 4.
+    const uint8_t edges[12] = {0x40, 0x51, 0x62, 0x73, 0x20, 0x31, 0x64, 0x75, 0x10, 0x32, 0x54, 0x76};
+    float sum[3] = {0.0f, 0.0f, 0.0f};
+    uint32_t count = 0;
+#pragma unroll
+    for (int ei = 0; ei < 12; ++ei) {
+        const uint32_t from = edges[ei] & 7u, to = edges[ei] >> 4;
+        if (sign_bit(d[from]) == sign_bit(d[to])) continue;
+        float d_from, d_to;
+        if (d[from] < 0.0f) {
+            d_from = d[from];
+            d_to = d[to];
+        } else {
+            d_from = -d[from];
+            d_to = -d[to];
+        }
+        float w;
+        if (d_to == d_from) {
+            w = 0.5f;
+        } else {
+            float dl = d_to - d_from;
+            w = div_rn(d_from + dl, dl);
+        }
+        // lerp(p0 + off[from], p0 + off[to], w) = a * (1 - w) + b * w (src/math.rs:45-49);
+        // corner k has +x for bit 2, +y for bit 1, +z for bit 0.
+        const float om = 1.0f - w;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const uint32_t bit = 2u - (uint32_t)a;
+            float pa = p0[a] + (((from >> bit) & 1u) ? step[a] : 0.0f);
+            float pb = p0[a] + (((to >> bit) & 1u) ? step[a] : 0.0f);
+            sum[a] = sum[a] + (pa * om + pb * w);
+        }
+        ++count;
+    }
+    const float fc = (float)count;
+    // Point3::origin() + sum / count
+#pragma unroll
+    for (int a = 0; a < 3; ++a) p[a] = 0.0f + div_rn(sum[a], fc);
+}
+
+__device__ __forceinline__ uint32_t vertex_of(const uint64_t* words, const uint32_t* wpre, const MeshDims& m,
+                                              uint32_t x, uint32_t y, uint32_t z) {
+    const uint32_t c = (((x << m.lg) | y) << m.lg) | z;
+    const uint32_t w = c >> 6, b = c & 63u;
+    const uint64_t below = (b == 0) ? 0ull : (words[w] & ((~0ull) >> (64u - b)));
+    return wpre[w] + (uint32_t)__popcll(below);
+}
+
+// One workgroup per leaf.  Dynamic LDS: ceil(cells / 64) mask words and
+// their exclusive popcount prefix.  For the normal passes that follow, each
+// vertex record temporarily holds the leaf's FD half-steps in its normal
+// slots (src/mesh/buffer.rs:257: 0.7 * (e - s) / res per axis).
+__global__ __launch_bounds__(kMeshBlock) void k_mesh_emit(const float* __restrict__ spans,
+                                                          const float* __restrict__ dists, MeshDims m,
+                                                          const uint64_t* __restrict__ voff,
+                                                          const uint64_t* __restrict__ ioff, float* __restrict__ verts,
+                                                          uint32_t* __restrict__ idx) {
+    extern __shared__ uint64_t lds_words[];
+    __shared__ uint32_t wsum[kMeshBlock / 64];
+    const uint32_t leaf = blockIdx.x;
+    const uint32_t nwords = (m.cells + 63u) >> 6;
+    uint64_t* words = lds_words;
+    uint32_t* wpre = reinterpret_cast<uint32_t*>(lds_words + nwords);
+    const float* D = dists + (size_t)leaf * m.n1 * m.n1 * m.n1;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+
+    // Crossing mask, one wave per 64 cells.
+    for (uint32_t w = wave; w < nwords; w += nwaves) {
+        const uint32_t c = (w << 6) | lane;
+        bool cr = false;
+        if (c < m.cells) {
+            uint32_t x, y, z;
+            cell_xyz(m, c, x, y, z);
+            float d[8];
+            cr = crossing(load_cell(D, m.n1, x, y, z, d));
+        }
+        const uint64_t bits = __ballot(cr);
+        if (lane == 0) words[w] = bits;
+    }
+    __syncthreads();
+    // Exclusive popcount prefix of the words: contiguous segments per thread.
+    {
+        const uint32_t seg = (nwords + blockDim.x - 1) / blockDim.x;
+        const uint32_t lo = std::min(threadIdx.x * seg, nwords), hi = std::min(lo + seg, nwords);
+        uint32_t s = 0;
+        for (uint32_t w = lo; w < hi; ++w) s += (uint32_t)__popcll(words[w]);
+        uint32_t tot;
+        uint32_t acc = block_excl_scan(s, wsum, tot);
+        for (uint32_t w = lo; w < hi; ++w) {
+            wpre[w] = acc;
+            acc += (uint32_t)__popcll(words[w]);
+        }
+    }
+    __syncthreads();
+
+    // Adjusted span (src/mesh/buffer.rs:64-67), cell step (:101), FD half-step (:257).
+    const float* sp = spans + 6 * (size_t)leaf;
+    float s1[3], step[3], delta[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float s0 = sp[a], e0 = sp[3 + a];
+        const float of = div_rn(e0 - s0, m.fres);
+        const float sa = s0 + -of;
+        const float ea = e0 + of;
+        s1[a] = sa;
+        step[a] = div_rn(ea - sa, m.fres);
+        delta[a] = div_rn(0.7f * (ea - sa), m.fres);
+    }
+    const uint64_t vbase = voff[leaf];
+    const uint64_t ibase = ioff[leaf];
+
+    uint32_t fcarry = 0;
+    for (uint32_t base = 0; base < m.cells; base += blockDim.x) {
+        const uint32_t c = base + threadIdx.x;
+        uint32_t x = 0, y = 0, z = 0, mk = 0, fb = 0;
+        float d[8];
+        if (c < m.cells) {
+            cell_xyz(m, c, x, y, z);
+            mk = load_cell(D, m.n1, x, y, z, d);
+            fb = face_bits(mk, x, y, z);
+        }
+        uint32_t tile_faces;
+        const uint32_t frank = block_excl_scan((uint32_t)__popc(fb), wsum, tile_faces);
+        if (c < m.cells && crossing(mk)) {
+            float p[3];
+            place_vertex(d, x, y, z, s1, step, p);
+            float* v = verts + 7 * (vbase + vertex_of(words, wpre, m, x, y, z));
+            v[0] = p[0];
+            v[1] = p[1];
+            v[2] = p[2];
+            v[3] = delta[0];
+            v[4] = delta[1];
+            v[5] = delta[2];
+            v[6] = 0.0f;
+        }
+        if (fb) {
+            uint32_t* out = idx + ibase + 6ull * (fcarry + frank);
+            const bool neg = d[0] < 0.0f;  // dists[(x, y, z)] < 0.0
+            const uint32_t v3 = vertex_of(words, wpre, m, x, y, z);
+            if (fb & 1u) {  // +x edge
+                const uint32_t v0 = vertex_of(words, wpre, m, x, y - 1, z - 1);
+                const uint32_t v1 = vertex_of(words, wpre, m, x, y - 1, z);
+                const uint32_t v2 = vertex_of(words, wpre, m, x, y, z - 1);
+                const uint32_t q[6] = {v0, v2, v1, v1, v2, v3}, r[6] = {v0, v1, v2, v1, v3, v2};
+#pragma unroll
+                for (int k = 0; k < 6; ++k) out[k] = neg ? q[k] : r[k];
+                out += 6;
+            }
+            if (fb & 2u) {  // +y edge
+                const uint32_t v0 = vertex_of(words, wpre, m, x - 1, y, z - 1);
+                const uint32_t v1 = vertex_of(words, wpre, m, x - 1, y, z);
+                const uint32_t v2 = vertex_of(words, wpre, m, x, y, z - 1);
+                const uint32_t q[6] = {v0, v1, v2, v1, v3, v2}, r[6] = {v0, v2, v1, v1, v2, v3};
+#pragma unroll
+                for (int k = 0; k < 6; ++k) out[k] = neg ? q[k] : r[k];
+                out += 6;
+            }
+            if (fb & 4u) {  // +z edge
+                const uint32_t v0 = vertex_of(words, wpre, m, x - 1, y - 1, z);
+                const uint32_t v1 = vertex_of(words, wpre, m, x - 1, y, z);
+                const uint32_t v2 = vertex_of(words, wpre, m, x, y - 1, z);
+                const uint32_t q[6] = {v0, v2, v1, v1, v2, v3}, r[6] = {v0, v1, v2, v1, v3, v2};
+#pragma unroll
+                for (int k = 0; k < 6; ++k) out[k] = neg ? q[k] : r[k];
+            }
+        }
+        fcarry += tile_faces;
+    }
+}
+
+// 6-tap stencil points of vertex v = i / 7 (src/mesh/buffer.rs:254-264):
+// tap 0 = p, tap 1 + 2a = p + unit_a * delta_a, tap 2 + 2a = p + unit_a * -delta_a.
+struct SrcMeshTaps {
+    const float* verts;  // 7 floats per vertex; slots 3..5 hold delta
+    __device__ __forceinline__ void get(uint32_t i, float& x, float& y, float& z, float& pw) const {
+        const uint32_t v = i / 7u, t = i - 7u * v;
+        const float* r = verts + 7 * (size_t)v;
+        float c[3] = {r[0], r[1], r[2]};
+        if (t > 0) {
+            const int a = (int)(t - 1) >> 1;
+            const float h = (t & 1u) ? r[3 + a] : -r[3 + a];
+#pragma unroll
+            for (int b = 0; b < 3; ++b) c[b] = c[b] + ((a == b) ? 1.0f : 0.0f) * h;
+        }
+        x = c[0];
+        y = c[1];
+        z = c[2];
+        pw = 0.0f;
+    }
+};
+
+struct SrcMeshVerts {
+    const float* verts;
+    float power;
+    __device__ __forceinline__ void get(uint32_t i, float& x, float& y, float& z, float& pw) const {
+        const float* r = verts + 7 * (size_t)i;
+        x = r[0];
+        y = r[1];
+        z = r[2];
+        pw = power;
+    }
+};
+
+// Forward-mode normal and distance straight into the vertex record.
+struct SinkMeshVerts {
+    float* verts;
+    uint32_t max_iters;
+    template <class E>
+    __device__ __forceinline__ void put(uint32_t i, float dv, uint32_t k, float r, float, const E& ev) const {
+        float g[4];
+        ev.grad(r, k == max_iters, g);
+        const float mg = sqrt_rn((g[0] * g[0] + g[1] * g[1]) + g[2] * g[2]);
+        const float im = div_rn(1.0f, mg);
+        float* v = verts + 7 * (size_t)i;
+        v[3] = g[0] * im;
+        v[4] = g[1] * im;
+        v[5] = g[2] * im;
+        v[6] = dv;
+    }
+};
+
+// normal = normalize((DE(p+dx) - DE(p-dx), ...)) = g * (1 / |g|) (cgmath
+// normalize); distance_from_surface = DE(p) (src/mesh/buffer.rs:254-272).
+__global__ void k_mesh_fd_combine(const float* __restrict__ taps, uint32_t n, float* __restrict__ verts) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* t = taps + 7 * (size_t)i;
+    const float g[3] = {t[1] - t[2], t[3] - t[4], t[5] - t[6]};
+    const float mg = sqrt_rn((g[0] * g[0] + g[1] * g[1]) + g[2] * g[2]);
+    const float im = div_rn(1.0f, mg);
+    float* v = verts + 7 * (size_t)i;
+    v[3] = g[0] * im;
+    v[4] = g[1] * im;
+    v[5] = g[2] * im;
+    v[6] = t[0];
+}
+
+inline uint32_t cdiv_u32(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+int mesh_dims(uint32_t res, MeshDims& m) {
+    if (res == 0 || (res & (res - 1)) != 0 || res > 64)
+        return fail(CTH_E_INVALID, "mesh resolution must be a power of two in [1, 64]");
+    m.res = res;
+    m.lg = (uint32_t)__builtin_ctz(res);
+    m.n1 = res + 1;
+    m.cells = res * res * res;
+    m.fres = (float)res;
+    return CTH_OK;
+}
+
+}  // namespace
+
+int mesh_count(Ctx&, const float* dists, uint32_t nleaves, uint32_t res, uint64_t* voff, uint64_t* ioff,
+               uint64_t totals[2], hipStream_t st) {
+    MeshDims m;
+    if (int rc = mesh_dims(res, m)) return rc;
+    totals[0] = totals[1] = 0;
+    if (nleaves == 0) {
+        const uint64_t zero[2] = {0, 0};
+        int rc = check_hip(hipMemcpyAsync(voff, zero, 8, hipMemcpyHostToDevice, st), "H2D offsets");
+        if (!rc) rc = check_hip(hipMemcpyAsync(ioff, zero + 1, 8, hipMemcpyHostToDevice, st), "H2D offsets");
+        if (!rc) rc = check_hip(hipStreamSynchronize(st), "mesh count sync");
+        return rc;
+    }
+    const size_t cnt = (size_t)nleaves + 1;
+    uint64_t* counts = nullptr;
+    void* tmp = nullptr;
+    size_t tmp_bytes = 0;
+    int rc = check_hip(hipMallocAsync((void**)&counts, 2 * cnt * sizeof(uint64_t), st), "hipMallocAsync counts");
+    if (!rc) {
+        hipLaunchKernelGGL(k_mesh_count, dim3(nleaves), dim3(kMeshBlock), 0, st, dists, m, nleaves, counts,
+                           counts + cnt);
+        rc = check_hip(hipGetLastError(), "mesh count launch");
+    }
+    if (!rc)
+        rc = check_hip(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, counts, voff, (int)cnt, st),
+                       "scan size");
+    if (!rc) rc = check_hip(hipMallocAsync(&tmp, tmp_bytes ? tmp_bytes : 4, st), "hipMallocAsync scan");
+    if (!rc) rc = check_hip(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, counts, voff, (int)cnt, st), "scan");
+    if (!rc)
+        rc = check_hip(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, counts + cnt, ioff, (int)cnt, st), "scan");
+    if (!rc)
+        rc = check_hip(hipMemcpyAsync(&totals[0], voff + nleaves, 8, hipMemcpyDeviceToHost, st), "D2H totals");
+    if (!rc)
+        rc = check_hip(hipMemcpyAsync(&totals[1], ioff + nleaves, 8, hipMemcpyDeviceToHost, st), "D2H totals");
+    if (tmp) (void)hipFreeAsync(tmp, st);
+    if (counts) (void)hipFreeAsync(counts, st);
+    int rs = check_hip(hipStreamSynchronize(st), "mesh count sync");
+    return rc ? rc : rs;
+}
+
+int mesh_emit(Ctx& c, const cth_shape& s, const float* spans, const float* dists, uint32_t nleaves, uint32_t res,
+              int normals, const uint64_t* voff, const uint64_t* ioff, uint64_t nverts, float* verts,
+              uint32_t* idx, hipStream_t st) {
+    MeshDims m;
+    if (int rc = mesh_dims(res, m)) return rc;
+    if (normals != CTH_NORMALS_FD6 && normals != CTH_NORMALS_TANGENT) return fail(CTH_E_INVALID, "unknown normals mode");
+    if (normals == CTH_NORMALS_TANGENT && s.kind == CTH_SHAPE_SPHERE)
+        return fail(CTH_E_UNSUPPORTED, "tangent normals need a Mandelbulb kind");
+    if (nleaves == 0) return CTH_OK;
+    const uint32_t nwords = (m.cells + 63u) >> 6;
+    const size_t lds = (size_t)nwords * (sizeof(uint64_t) + sizeof(uint32_t));
+    hipLaunchKernelGGL(k_mesh_emit, dim3(nleaves), dim3(kMeshBlock), lds, st, spans, dists, m, voff, ioff, verts, idx);
+    int rc = check_hip(hipGetLastError(), "mesh emit launch");
+    if (rc || nverts == 0) return rc;
+
+    MandelUniform u{};
+    int family = FAM_SPHERE;
+    if (s.kind != CTH_SHAPE_SPHERE) {
+        rc = resolve_mandel(s, u, family);
+        if (rc) return rc;
+    }
+    if (normals == CTH_NORMALS_TANGENT) {
+        if (nverts >= (1ull << 31)) return fail(CTH_E_INVALID, "more than 2^31 vertices");
+        uint32_t blocks = 0;
+        WorkQueue q = make_queue(c, (uint32_t)nverts, c.shape_adj, blocks);
+        SrcMeshVerts src{verts, s.power};
+        SinkMeshVerts sink{verts, u.max_iters};
+        if (family == FAM_P8)
+            hipLaunchKernelGGL((k_refill<P8Tangent, SrcMeshVerts, SinkMeshVerts>), dim3(blocks),
+                               dim3(c.shape_adj.block), 0, st, src, u, sink, q);
+        else
+            hipLaunchKernelGGL((k_refill<TrigTangent, SrcMeshVerts, SinkMeshVerts>), dim3(blocks),
+                               dim3(c.shape_adj.block), 0, st, src, u, sink, q);
+        return check_hip(hipGetLastError(), "mesh tangent normals launch");
+    }
+
+    // 6-tap central differences: 7 evaluations per vertex, in batches whose
+    // index space stays below 2^31.
+    const uint64_t batch = std::min<uint64_t>(nverts, ((1ull << 31) - 1) / 7);
+    float* taps = nullptr;
+    rc = check_hip(hipMallocAsync((void**)&taps, batch * 7 * sizeof(float), st), "hipMallocAsync taps");
+    for (uint64_t off = 0; off < nverts && !rc; off += batch) {
+        const uint32_t nb = (uint32_t)std::min<uint64_t>(batch, nverts - off);
+        float* vb = verts + 7 * off;
+        SrcMeshTaps src{vb};
+        SinkDist sink{taps, nullptr};
+        if (family == FAM_SPHERE) {
+            hipLaunchKernelGGL((k_sphere<SrcMeshTaps>), dim3(cdiv_u32((uint64_t)nb * 7, 256)), dim3(256), 0, st, src,
+                               nb * 7, s.center[0], s.center[1], s.center[2], s.radius, taps, nullptr);
+            rc = check_hip(hipGetLastError(), "mesh taps launch");
+        } else {
+            rc = run_refill(c, family, u, src, sink, nb * 7, st, c.shape_de);
+        }
+        if (rc) break;
+        hipLaunchKernelGGL(k_mesh_fd_combine, dim3(cdiv_u32(nb, 256)), dim3(256), 0, st, taps, nb, vb);
+        rc = check_hip(hipGetLastError(), "mesh normals launch");
+    }
+    if (taps) (void)hipFreeAsync(taps, st);
+    return rc;
+}
+
+}  // namespace cth

@@ -60,6 +60,19 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v) {
     if (i < n) p[i] = v;
 }
 
+// Fill kernel for code that may be captured into a graph (a recorded memset
+// node is avoided there).
+int launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t st) {
+    const size_t step = size_t(1) << 30;
+    for (size_t off = 0; off < n; off += step) {
+        uint32_t m = (uint32_t)((n - off) < step ? (n - off) : step);
+        hipLaunchKernelGGL(k_fill_u32, dim3((m + 255) / 256), dim3(256), 0, st, p + off, m, v);
+        int rc = check_hip(hipGetLastError(), "fill launch");
+        if (rc) return rc;
+    }
+    return CTH_OK;
+}
+
 // One thread per (node, child): child c of a split node i lands at
 // 8 * offs[i] + c.  create_spans (src/octree/mod.rs:315-329): centre
 // = start + (end - start) / 2; child bit 2 = +x half, bit 1 = +y, bit 0 = +z.

@@ -91,6 +91,35 @@ def sample_spans(shape: Shape, spans: torch.Tensor, res: int, with_cross: bool =
     return dists, cross
 
 
+_NORMALS = {"fd6": _lib.CTH_NORMALS_FD6, "tangent": _lib.CTH_NORMALS_TANGENT}
+
+
+def mesh(shape: Shape, spans: torch.Tensor, dists: torch.Tensor, res: int, normals: str = "fd6",
+         ctx: Context | None = None):
+    """Naive surface nets, steps 2-3, over leaves already sampled by
+    `sample_spans` at the same `res`.  Returns (vert_offsets (n+1,) int64,
+    idx_offsets (n+1,) int64, vertices (nv, 7) f32 in the `Vertex` layout,
+    indices (ni,) int32 holding u32 leaf-local vertex numbers)."""
+    ctx = ctx or shape.ctx
+    spans = _f32(spans).contiguous()
+    dists = _f32(dists).contiguous()
+    m = spans.shape[0]
+    if dists.numel() != m * (res + 1) ** 3:
+        raise ValueError("dists must hold (res+1)^3 distances per span")
+    voff = torch.empty(m + 1, dtype=torch.int64, device=spans.device)
+    ioff = torch.empty(m + 1, dtype=torch.int64, device=spans.device)
+    nv = C.c_uint64()
+    ni = C.c_uint64()
+    check(lib.cth_mesh_count_dev(ctx.handle, _ptr(dists), m, res, _ptr(voff), _ptr(ioff), C.byref(nv), C.byref(ni),
+                                 _stream()))
+    verts = torch.empty((max(nv.value, 1), 7), dtype=torch.float32, device=spans.device)
+    idx = torch.empty(max(ni.value, 1), dtype=torch.int32, device=spans.device)
+    check(lib.cth_mesh_emit_dev(ctx.handle, C.byref(shape.desc), _ptr(spans), _ptr(dists), m, res,
+                                _NORMALS[normals], _ptr(voff), _ptr(ioff), nv.value, _ptr(verts), _ptr(idx),
+                                _stream()))
+    return voff, ioff, verts[:nv.value], idx[:ni.value]
+
+
 def refine(shape: Shape, root, min_depth: int, max_depth: int, half_diag, cap: int | None = None,
            leaves: torch.Tensor | None = None, ctx: Context | None = None, device="cuda"):
     """Level-synchronous refinement; returns (leaves (m, 6) f32 on device, level_counts)."""

@@ -11,6 +11,8 @@
  *   Shape::contains                                     src/shape/mod.rs:78-80
  *   Shape::bounding_box                                 src/shape/mod.rs:39
  *   MeshBuffer step 1 (corner sampling) + sign test     src/mesh/buffer.rs:63-147
+ *   MeshBuffer steps 2-3 (vertices, normals, quads)     src/mesh/buffer.rs:97-372
+ *   MeshBuffer::generate_for_box                        src/mesh/buffer.rs:30-42
  *   ShapeMesh::get_focii (sphere tracing)               src/mesh/mod.rs:205-243
  *
  * Conventions
@@ -312,6 +314,59 @@ int cth_graph_begin(void* stream);
 int cth_graph_end(void* stream, cth_graph** out);
 int cth_graph_launch(cth_graph* graph, void* stream);
 void cth_graph_destroy(cth_graph* graph);
+
+/* ---- surface nets (mesh extraction) ---------------------------------------------- */
+
+/* Vertex normals of the mesh entry points.
+ *   CTH_NORMALS_FD6      the reference's 6-tap central difference with half-step
+ *                        0.7 * (e' - s') / res per axis, normalised
+ *                        (src/mesh/buffer.rs:256-266): 7 distance evaluations
+ *                        per vertex, distance_from_surface = DE(p) (:254).
+ *   CTH_NORMALS_TANGENT  the forward-mode gradient of DE at the vertex,
+ *                        normalised (one tangent evaluation per vertex; the
+ *                        distance is the parity DE(p)).  Mandelbulb kinds only. */
+enum { CTH_NORMALS_FD6 = 0, CTH_NORMALS_TANGENT = 1 };
+
+/* The reference's `Vertex` (src/mesh/mod.rs:255-261, #[repr(C)]), 28 bytes. */
+typedef struct cth_vertex {
+    float position[3];
+    float normal[3];
+    float distance_from_surface;
+} cth_vertex;
+
+/* Surface nets, pass 1, over a batch of leaves whose step-1 corner distances
+ * `dists` (nspans x (res+1)^3, GridTable order, as cth_sample_spans_dev writes
+ * them) are on the device.  Per leaf: one vertex per cell whose corners have
+ * both IEEE signs (src/mesh/buffer.rs:127-147), six indices per corner edge
+ * whose end signs differ (:299-363).  vert_offsets / idx_offsets (device,
+ * nspans + 1 each) receive the exclusive prefix sums; *nverts / *nidx (host)
+ * the totals.  res: power of two in [1, 64].  Returns after the totals are
+ * known (kernels run on `stream`). */
+int cth_mesh_count_dev(cth_ctx* ctx, const float* dists, size_t nspans, uint32_t res, uint64_t* vert_offsets,
+                       uint64_t* idx_offsets, uint64_t* nverts, uint64_t* nidx, void* stream);
+
+/* Surface nets, pass 2: leaf i's vertices go to vertices[vert_offsets[i] ...]
+ * in cube() cell order (x slowest, z fastest; src/mesh/buffer.rs:113), its
+ * indices to indices[idx_offsets[i] ...], local to the leaf (0-based, as each
+ * leaf's MeshBuffer holds them), in the reference's emission order and
+ * winding (:288-372).  Vertex position = centroid of the edge crossings in
+ * EDGES order with the reference's distance weights (:149-250).  `nverts` is
+ * the total from pass 1.  spans: 6 floats per leaf (the unadjusted span, as
+ * given to cth_sample_spans_dev).  Uses a stream-ordered workspace of 28 bytes
+ * per vertex for CTH_NORMALS_FD6. */
+int cth_mesh_emit_dev(cth_ctx* ctx, const cth_shape* shape, const float* spans, const float* dists, size_t nspans,
+                      uint32_t res, int normals, const uint64_t* vert_offsets, const uint64_t* idx_offsets,
+                      uint64_t nverts, cth_vertex* vertices, uint32_t* indices, void* stream);
+
+/* MeshBuffer::generate_for_box for one span (host buffers, synchronous):
+ * steps 1-3 on the device.  *nverts / *nidx receive the mesh size; vertices /
+ * indices may be NULL to query it, and when non-NULL must hold vert_cap /
+ * idx_cap entries of at least that size (else CTH_E_INVALID).  Like the
+ * reference (src/mesh/buffer.rs:35-39), start < end on every axis and a
+ * power-of-two res are required. */
+int cth_mesh_span(cth_ctx* ctx, const cth_shape* shape, const cth_span* span, uint32_t res, int normals,
+                  cth_vertex* vertices, size_t vert_cap, uint32_t* indices, size_t idx_cap, size_t* nverts,
+                  size_t* nidx);
 
 /* Host-buffer variant of cth_sample_spans_dev (synchronous). */
 int cth_sample_spans(cth_ctx* ctx, const cth_shape* shape, const cth_span* spans, size_t nspans, uint32_t res,

@@ -21,7 +21,7 @@ K_SPHERE = 2
 
 __all__ = [
     "OrcShape", "mandelbulb", "mandelbulb_trig", "sphere", "de_points", "de_points_f64", "de_grad_f64",
-    "de_grid", "sample_spans", "surface_nets", "create_spans", "refine", "ray_dirs", "sphere_trace", "fd_diff",
+    "de_grid", "sample_spans", "surface_nets", "mesh_batch", "create_spans", "refine", "ray_dirs", "sphere_trace", "fd_diff",
     "powi", "build", "K_MANDELBULB", "K_MANDELBULB_TRIG", "K_SPHERE", "default_threads",
 ]
 
@@ -63,6 +63,7 @@ for _name, _res, _args in [
     ("orc_de_grid_f32", None, [_SH, _P, _P, _U, _F, _P, _P, _I]),
     ("orc_sample_spans", None, [_SH, _P, _S, _U, _P, _P, _P, _I]),
     ("orc_surface_nets", None, [_SH, _P, _U, _P, _P, _S, _P, _S, C.POINTER(_S), C.POINTER(_S)]),
+    ("orc_mesh_batch", None, [_SH, _P, _S, _U, _P, _I, _P, _P, _P, _S, _P, _S]),
     ("orc_create_spans", None, [_P, _P]),
     ("orc_refine", _S, [_SH, _P, _U, _U, _P, _P, _S, _P, _I]),
     ("orc_ray_dirs", None, [_P, _U, _U, _U, _U, _P]),
@@ -190,6 +191,25 @@ def surface_nets(shape: OrcShape, span, res: int):
     _lib.orc_surface_nets(C.byref(shape), _ptr(sp), res, _ptr(d), _ptr(v), nv.value, _ptr(ix), ni.value,
                           C.byref(nv), C.byref(ni))
     return d, v, ix
+
+
+def mesh_batch(shape: OrcShape, spans, res: int, dists=None, threads=None):
+    """Surface nets steps 2-3 (or the whole algorithm when `dists` is None)
+    over a batch of spans.  Returns (voff[n+1], ioff[n+1], vertices[nv,7],
+    indices) with span-local u32 indices, concatenated in span order."""
+    sp = _f32(spans).reshape(-1, 6)
+    n = sp.shape[0]
+    d = None if dists is None else _f32(dists).reshape(n, (res + 1) ** 3)
+    voff = np.zeros(n + 1, np.uint64)
+    ioff = np.zeros(n + 1, np.uint64)
+    t = threads or default_threads()
+    _lib.orc_mesh_batch(C.byref(shape), _ptr(sp), n, res, None if d is None else _ptr(d), t, _ptr(voff),
+                        _ptr(ioff), None, 0, None, 0)
+    v = np.empty((int(voff[-1]), 7), np.float32)
+    ix = np.empty(int(ioff[-1]), np.uint32)
+    _lib.orc_mesh_batch(C.byref(shape), _ptr(sp), n, res, None if d is None else _ptr(d), t, _ptr(voff),
+                        _ptr(ioff), _ptr(v), v.shape[0], _ptr(ix), ix.shape[0])
+    return voff, ioff, v, ix
 
 
 def create_spans(span) -> np.ndarray:

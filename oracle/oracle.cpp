@@ -451,25 +451,14 @@ struct Vertex {
     float distance_from_surface;
 };
 
-// The reference's whole `naive_surface_nets` (src/mesh/buffer.rs:58-391).
-// `dists` is filled (step 1), then vertices (step 2) and indices (step 3).
-void surface_nets(const orc_shape& shape, const float* span_start, const float* span_end, uint32_t res,
-                  std::vector<float>& dists, std::vector<Vertex>& verts, std::vector<uint32_t>& indices) {
+// Steps 2 and 3 of `naive_surface_nets` (src/mesh/buffer.rs:97-372) given the
+// step-1 corner distances `dists` ((res+1)^3, GridTable order) of the span.
+void mesh_from_dists(const orc_shape& shape, const float* span_start, const float* span_end, uint32_t res,
+                     const float* dists, std::vector<Vertex>& verts, std::vector<uint32_t>& indices) {
     const uint32_t n = res + 1;
     float s[3], e[3];
     adjust_span(span_start, span_end, res, s, e);
     const float fres = float(res);
-    float across[3] = {e[0] - s[0], e[1] - s[1], e[2] - s[2]};
-    dists.assign(size_t(n) * n * n, 0.0f);
-    // Step 1: corner DE grid, cube() order, z fastest (src/util/iter.rs:49-64).
-    for (uint32_t x = 0; x < n; ++x)
-        for (uint32_t y = 0; y < n; ++y)
-            for (uint32_t z = 0; z < n; ++z) {
-                float px = grid_coord(s[0], across[0], x, fres);
-                float py = grid_coord(s[1], across[1], y, fres);
-                float pz = grid_coord(s[2], across[2], z, fres);
-                dists[(size_t(x) * n + y) * n + z] = shape_de(shape, px, py, pz);
-            }
     auto D = [&](uint32_t x, uint32_t y, uint32_t z) { return dists[(size_t(x) * n + y) * n + z]; };
 
     // Step 2 (src/mesh/buffer.rs:97-275).
@@ -583,6 +572,33 @@ void surface_nets(const orc_shape& shape, const float* span_start, const float* 
                     indices.insert(indices.end(), neg ? q : r, (neg ? q : r) + 6);
                 }
             }
+}
+
+// Step 1 of `naive_surface_nets` (src/mesh/buffer.rs:63-83): corner DE grid
+// over the adjusted span, cube() order, z fastest (src/util/iter.rs:49-64).
+void sample_span(const orc_shape& shape, const float* span_start, const float* span_end, uint32_t res,
+                 std::vector<float>& dists) {
+    const uint32_t n = res + 1;
+    float s[3], e[3];
+    adjust_span(span_start, span_end, res, s, e);
+    const float fres = float(res);
+    float across[3] = {e[0] - s[0], e[1] - s[1], e[2] - s[2]};
+    dists.assign(size_t(n) * n * n, 0.0f);
+    for (uint32_t x = 0; x < n; ++x)
+        for (uint32_t y = 0; y < n; ++y)
+            for (uint32_t z = 0; z < n; ++z) {
+                float px = grid_coord(s[0], across[0], x, fres);
+                float py = grid_coord(s[1], across[1], y, fres);
+                float pz = grid_coord(s[2], across[2], z, fres);
+                dists[(size_t(x) * n + y) * n + z] = shape_de(shape, px, py, pz);
+            }
+}
+
+// The reference's whole `naive_surface_nets` (src/mesh/buffer.rs:58-391).
+void surface_nets(const orc_shape& shape, const float* span_start, const float* span_end, uint32_t res,
+                  std::vector<float>& dists, std::vector<Vertex>& verts, std::vector<uint32_t>& indices) {
+    sample_span(shape, span_start, span_end, res, dists);
+    mesh_from_dists(shape, span_start, span_end, res, dists.data(), verts, indices);
 }
 
 // `create_spans` (src/octree/mod.rs:315-329): center = start + (end - start) / 2;
@@ -774,6 +790,41 @@ void orc_surface_nets(const orc_shape* shape, const float* span, uint32_t res, f
     *nidx = indices.size();
     if (verts_out && verts_cap >= verts.size()) std::memcpy(verts_out, verts.data(), verts.size() * sizeof(Vertex));
     if (idx_out && idx_cap >= indices.size()) std::memcpy(idx_out, indices.data(), indices.size() * 4);
+}
+
+// Surface nets over a batch of spans, threaded over spans (the reference's
+// per-leaf jobs, src/mesh/mod.rs:129-161).  `dists_in` (nullable) holds each
+// span's (res+1)^3 step-1 distances; when null they are computed.  Fills the
+// per-span exclusive offsets voff/ioff (n+1 each; vertex records of 7 floats,
+// u32 indices local to the span) and, when verts/idx are non-null and their
+// capacities suffice, the concatenated vertices and indices.
+void orc_mesh_batch(const orc_shape* shape, const float* spans, size_t nspans, uint32_t res, const float* dists_in,
+                    int threads, uint64_t* voff, uint64_t* ioff, float* verts, size_t vcap, uint32_t* idx,
+                    size_t icap) {
+    std::vector<std::vector<Vertex>> V(nspans);
+    std::vector<std::vector<uint32_t>> I(nspans);
+    const size_t per = size_t(res + 1) * (res + 1) * (res + 1);
+    parallel_for(nspans, threads, 1, [&](size_t li) {
+        const float* sp = spans + 6 * li;
+        if (dists_in) {
+            mesh_from_dists(*shape, sp, sp + 3, res, dists_in + li * per, V[li], I[li]);
+        } else {
+            std::vector<float> d;
+            surface_nets(*shape, sp, sp + 3, res, d, V[li], I[li]);
+        }
+    });
+    voff[0] = 0;
+    ioff[0] = 0;
+    for (size_t i = 0; i < nspans; ++i) {
+        voff[i + 1] = voff[i] + V[i].size();
+        ioff[i + 1] = ioff[i] + I[i].size();
+    }
+    if (verts && vcap >= voff[nspans])
+        for (size_t i = 0; i < nspans; ++i)
+            if (!V[i].empty()) std::memcpy(verts + 7 * voff[i], V[i].data(), V[i].size() * sizeof(Vertex));
+    if (idx && icap >= ioff[nspans])
+        for (size_t i = 0; i < nspans; ++i)
+            if (!I[i].empty()) std::memcpy(idx + ioff[i], I[i].data(), I[i].size() * 4);
 }
 
 void orc_create_spans(const float* span, float* out48) {

@@ -1,0 +1,228 @@
+"""GPU surface nets (naive_surface_nets steps 2-3) against the oracle.
+
+Bar: with the same corner distances, vertex counts, offsets and every index
+(topology, emission order and winding) are bit-identical, and so are the
+vertex positions (the reference's f32 operation order).  The 6-tap normals and
+distance_from_surface are distance evaluations again: bit-identical for the
+sphere, and for the Mandelbulb equal except where the final ln differs by an
+ulp between glibc's logf and the correctly rounded device log.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+F32 = np.float32
+ROOT = [-1.5, -1.5, -1.5, 1.5, 1.5, 1.5]
+
+
+def same_bits(a, b):
+    return np.asarray(a, F32).view(np.uint32) == np.asarray(b, F32).view(np.uint32)
+
+
+def gpu(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def angle(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    c = (a * b).sum(1) / (np.linalg.norm(a, axis=1) * np.linalg.norm(b, axis=1))
+    return np.arccos(np.clip(c, -1, 1))
+
+
+def c2_leaves(count=2000, seed=0, depth=8):
+    """A seeded sample of the depth-8 surface leaves of the octree workload."""
+    from cantucci_amd.workloads import half_diagonals
+    leaves, _ = O.refine(O.mandelbulb(15, 2.0), ROOT, 2, depth, half_diagonals(3.0, depth))
+    idx = np.random.default_rng(seed).choice(len(leaves), min(count, len(leaves)), replace=False)
+    return leaves[np.sort(idx)]
+
+
+def run_gpu_mesh(torch, shape, spans, res, normals="fd6"):
+    from cantucci_amd import device as dev
+    sp = gpu(torch, spans)
+    d, _ = dev.sample_spans(shape, sp, res, with_cross=False)
+    voff, ioff, v, ix = dev.mesh(shape, sp, d, res, normals)
+    torch.cuda.synchronize()
+    return (d.cpu().numpy(), voff.cpu().numpy().astype(np.uint64), ioff.cpu().numpy().astype(np.uint64),
+            v.cpu().numpy(), ix.cpu().numpy().view(np.uint32))
+
+
+def assert_topology(voff, ioff, ix, voff_o, ioff_o, ix_o):
+    assert np.array_equal(voff, voff_o), "per-leaf vertex counts differ"
+    assert np.array_equal(ioff, ioff_o), "per-leaf index counts differ"
+    assert np.array_equal(ix, ix_o), "indices differ"
+
+
+@pytest.mark.parametrize("res", [8, 1, 2, 16])
+def test_sphere_mesh_bitexact(ctx, torch_cuda, res):
+    """Sphere: exact closed-form distances, so the whole mesh -- positions,
+    6-tap normals and distances -- is bit-identical."""
+    import cantucci_amd as ca
+    torch = torch_cuda
+    center, radius = (0.1, 0.0, -0.05), 0.8
+    shape = ca.Sphere(center, radius, ctx=ctx)
+    from cantucci_amd.workloads import half_diagonals
+    leaves, _ = O.refine(O.sphere(center, radius), [-1, -1, -1, 1, 1, 1], 1, 4, half_diagonals(2.0, 4))
+    d, voff, ioff, v, ix = run_gpu_mesh(torch, shape, leaves, res)
+    voff_o, ioff_o, v_o, ix_o = O.mesh_batch(O.sphere(center, radius), leaves, res, dists=d)
+    assert_topology(voff, ioff, ix, voff_o, ioff_o, ix_o)
+    assert voff[-1] > 0 and (ioff[-1] > 0 or res == 1)
+    assert same_bits(v, v_o).all()
+    # end to end: the oracle's own corner sampling gives the same mesh
+    voff_e, ioff_e, v_e, ix_e = O.mesh_batch(O.sphere(center, radius), leaves, res)
+    assert_topology(voff, ioff, ix, voff_e, ioff_e, ix_e)
+    assert same_bits(v, v_e).all()
+
+
+def test_c2_leaf_mesh_parity(ctx, torch_cuda):
+    """2,000 seeded depth-8 surface leaves of the octree workload (P8, 15 it,
+    bailout 2, 8^3 cells per leaf)."""
+    import cantucci_amd as ca
+    torch = torch_cuda
+    spans = c2_leaves(2000)
+    shape = ca.Mandelbulb.classic(15, 2.0, ctx=ctx)
+    d, voff, ioff, v, ix = run_gpu_mesh(torch, shape, spans, 8)
+    voff_o, ioff_o, v_o, ix_o = O.mesh_batch(O.mandelbulb(15, 2.0), spans, 8, dists=d)
+    assert_topology(voff, ioff, ix, voff_o, ioff_o, ix_o)
+    nv = int(voff[-1])
+    print(f"{len(spans)} leaves: {nv} vertices, {int(ioff[-1]) // 6} quads")
+    assert nv > 10 * len(spans)
+    assert same_bits(v[:, :3], v_o[:, :3]).all(), "vertex positions differ"
+    dist_same = same_bits(v[:, 6], v_o[:, 6]) | (np.isnan(v[:, 6]) & np.isnan(v_o[:, 6]))
+    nrm_same = same_bits(v[:, 3:6], v_o[:, 3:6]).all(1)
+    print(f"distance bit-exact {dist_same.mean():.5f}, normal bit-exact {nrm_same.mean():.5f}")
+    # The distance differs from the oracle's only through the final ln: it is
+    # the correctly rounded f32 ln of the (bit-identical) loop state, where
+    # glibc's logf is off by an ulp on a few percent of these near-surface r.
+    _, _, r_o, dr_o = O.de_points(O.mandelbulb(15, 2.0), v[:, :3])
+    ln_cr = np.log(r_o.astype(np.float64)).astype(F32)
+    with np.errstate(all="ignore"):
+        d_cr = (F32(0.5) * (ln_cr * r_o)) / dr_o
+    cr_same = same_bits(v[:, 6], d_cr) | (np.isnan(v[:, 6]) & np.isnan(d_cr))
+    print(f"distance = correctly rounded formula on {cr_same.mean():.6f}")
+    assert dist_same.mean() > 0.97 and cr_same.mean() > 0.9999
+    fin = np.isfinite(v[:, 6]) & np.isfinite(v_o[:, 6])
+    rel = np.abs(v[fin, 6].astype(np.float64) - v_o[fin, 6]) / np.maximum(np.abs(v_o[fin, 6].astype(np.float64)), 1e-30)
+    assert rel.max() < 1e-5
+    # 6-tap normals: the same up to those ulps of the tap distances
+    assert nrm_same.mean() > 0.95
+    fin = np.isfinite(v[:, 3:6]).all(1) & np.isfinite(v_o[:, 3:6]).all(1)
+    ang = angle(v[fin, 3:6], v_o[fin, 3:6])
+    print(f"6-tap normal angle vs oracle: p99 {np.percentile(ang, 99):.2e} p99.9 {np.percentile(ang, 99.9):.2e}")
+    assert np.percentile(ang, 99.9) < 1e-3
+    # with the oracle's own corner distances the sign field, and so the
+    # topology, is the same
+    d_o, _, _ = O.sample_spans(O.mandelbulb(15, 2.0), spans, 8)
+    assert (np.signbit(d.reshape(d_o.shape)) == np.signbit(d_o)).all()
+    voff_e, ioff_e, _, ix_e = O.mesh_batch(O.mandelbulb(15, 2.0), spans, 8)
+    assert_topology(voff, ioff, ix, voff_e, ioff_e, ix_e)
+
+
+def test_tangent_normals_on_mesh(ctx, torch_cuda):
+    """Forward-mode vertex normals: same vertices and distances as the 6-tap
+    mesh, normals within 1e-3 rad of the FP64 dual-number gradient."""
+    import cantucci_amd as ca
+    torch = torch_cuda
+    spans = c2_leaves(500, seed=3)
+    shape = ca.Mandelbulb.classic(15, 2.0, ctx=ctx)
+    _, voff, ioff, v_fd, ix_fd = run_gpu_mesh(torch, shape, spans, 8, "fd6")
+    _, voff_t, ioff_t, v_t, ix_t = run_gpu_mesh(torch, shape, spans, 8, "tangent")
+    assert_topology(voff_t, ioff_t, ix_t, voff, ioff, ix_fd)
+    assert same_bits(v_t[:, :3], v_fd[:, :3]).all()
+    dsame = same_bits(v_t[:, 6], v_fd[:, 6]) | (np.isnan(v_t[:, 6]) & np.isnan(v_fd[:, 6]))
+    assert dsame.all()
+    pts = v_t[:, :3]
+    sh = O.mandelbulb(15, 2.0)
+    _, g64, _ = O.de_grad_f64(sh, pts.astype(np.float64))
+    # Conditioning of each vertex: how far the exact gradient direction turns
+    # when the f32 point moves by one ulp along an axis.  Vertices sit on the
+    # fractal surface, where many are so ill-conditioned that no f32
+    # evaluation can recover the FP64 direction; the bar applies to the rest.
+    turn = np.zeros(len(pts))
+    for a in range(3):
+        for dirn in (np.inf, -np.inf):
+            q = pts.copy()
+            q[:, a] = np.nextafter(q[:, a], F32(dirn))
+            _, gq, _ = O.de_grad_f64(sh, q.astype(np.float64))
+            with np.errstate(all="ignore"):
+                turn = np.maximum(turn, np.nan_to_num(angle(gq[:, :3], g64[:, :3]), nan=np.pi))
+    ok = np.isfinite(g64[:, :3]).all(1) & (np.linalg.norm(g64[:, :3], axis=1) > 0) & np.isfinite(v_t[:, 3:6]).all(1)
+    ang_t = angle(v_t[ok, 3:6], g64[ok, :3])
+    ang_fd = angle(v_fd[ok, 3:6], g64[ok, :3])
+    stable = turn[ok] < 1e-4
+    print(f"tangent p50 {np.median(ang_t):.2e} p99 {np.percentile(ang_t, 99):.2e} "
+          f"(<=1e-3 on {np.mean(ang_t <= 1e-3):.4f}); well-conditioned {stable.mean():.4f}: "
+          f"p50 {np.median(ang_t[stable]):.2e} p99 {np.percentile(ang_t[stable], 99):.2e} "
+          f"(<=1e-3 on {np.mean(ang_t[stable] <= 1e-3):.4f}); "
+          f"6-tap (a difference at 0.7 cells, not a point gradient) p50 {np.median(ang_fd):.2e}")
+    assert stable.mean() > 0.3
+    assert np.median(ang_t[stable]) < 5e-5 and np.mean(ang_t[stable] <= 1e-3) > 0.99
+
+
+def _host_sample(shape, sp, res):
+    """Corner distances through the host-buffer span-sampling entry point."""
+    import ctypes as C
+    from cantucci_amd import _lib
+    d = np.empty((res + 1) ** 3, F32)
+    _lib.check(_lib.lib.cth_sample_spans(shape.ctx.handle, C.byref(shape.desc), np.ascontiguousarray(sp).ctypes.data,
+                                         1, res, d.ctypes.data, None))
+    return d
+
+
+@pytest.mark.parametrize("res,span", [
+    (64, ((-0.6, -0.6, 0.7), (-0.3, -0.3, 1.0))),
+    (32, ((0.2, 0.4, -0.3), (0.5, 0.7, 0.0))),
+])
+def test_generate_for_box(ctx, res, span):
+    """MeshBuffer::generate_for_box mirror at the reference's own resolution
+    (64) with the app's shape, classic(6, 2.5)."""
+    import cantucci_amd as ca
+    from cantucci_amd.mesh import MeshBuffer
+    shape = ca.Mandelbulb.classic(6, 2.5, ctx=ctx)
+    buf, tm = MeshBuffer.generate_for_box(span, shape, res)
+    sp = np.array(span[0] + span[1], F32)
+    d_o, v_o, ix_o = O.surface_nets(O.mandelbulb(6, 2.5), sp, res)
+    assert tm.vertices == len(v_o) and tm.faces * 6 == len(ix_o) and tm.vertices > 100
+    assert np.array_equal(buf.indices, ix_o)
+    pos = buf.vertices["position"]
+    # end to end, positions move only through the corner distances' last-ulp
+    # differences (the final ln); with the device's corner distances they
+    # are bit-identical
+    cell = (sp[3] - sp[0]) / res
+    assert np.abs(pos - v_o[:, :3]).max() < 1e-4 * cell
+    d_dev = _host_sample(shape, sp, res)
+    _, _, v_same, ix_same = O.mesh_batch(O.mandelbulb(6, 2.5), sp[None], res, dists=d_dev)
+    assert np.array_equal(buf.indices, ix_same)
+    assert same_bits(pos, v_same[:, :3]).all()
+    dist_same = same_bits(buf.vertices["distance_from_surface"], v_o[:, 6])
+    assert dist_same.mean() > 0.97
+
+
+def test_mesh_edge_cases(ctx, torch_cuda):
+    """No crossing cells (a leaf far outside / inside), an empty batch, and
+    the reference's asserts as errors."""
+    import cantucci_amd as ca
+    from cantucci_amd import device as dev
+    from cantucci_amd.mesh import MeshBuffer
+    torch = torch_cuda
+    shape = ca.Sphere((0.0, 0.0, 0.0), 0.5, ctx=ctx)
+    spans = np.array([[2, 2, 2, 3, 3, 3], [-0.1, -0.1, -0.1, 0.1, 0.1, 0.1], [0.3, -0.2, -0.2, 0.7, 0.2, 0.2]], F32)
+    d, voff, ioff, v, ix = run_gpu_mesh(torch, shape, spans, 4)
+    assert voff[0] == 0 and voff[1] == 0 and voff[2] == 0 and voff[3] > 0
+    voff_o, ioff_o, v_o, ix_o = O.mesh_batch(O.sphere((0, 0, 0), 0.5), spans, 4, dists=d)
+    assert_topology(voff, ioff, ix, voff_o, ioff_o, ix_o)
+    assert same_bits(v, v_o).all()
+    empty = torch.empty((0, 6), device="cuda")
+    de = torch.empty((0, 125), device="cuda")
+    voff, ioff, v, ix = dev.mesh(shape, empty, de, 4)
+    assert voff.cpu().tolist() == [0] and v.shape[0] == 0 and ix.shape[0] == 0
+    with pytest.raises(ca.CthError):
+        MeshBuffer.generate_for_box(((0, 0, 0), (0, 1, 1)), shape, 8)  # start == end on x
+    with pytest.raises(ca.CthError):
+        MeshBuffer.generate_for_box(((0, 0, 0), (1, 1, 1)), shape, 12)  # not a power of two
+    with pytest.raises(ca.CthError):
+        MeshBuffer.generate_for_box(((0, 0, 0), (1, 1, 1)), shape, 8, normals="tangent")  # sphere: closed form

@@ -245,6 +245,25 @@ def test_surface_nets_sphere_watertight():
     assert np.abs(np.linalg.norm(pos, axis=1) - 0.55).max() < 2.0 / 16
 
 
+def test_mesh_batch_matches_per_span_surface_nets():
+    """The batched, threaded mesh export equals the per-span restatement,
+    both end to end and from given corner distances; offsets are exclusive
+    prefix sums of the per-span sizes."""
+    sh = O.sphere((0.1, 0.0, -0.05), 0.8)
+    spans = np.array([[-1, -1, -1, 0, 0, 0], [0, -1, -1, 1, 0, 0], [2, 2, 2, 3, 3, 3], [-1, 0, 0, 0, 1, 1]], F32)
+    voff, ioff, v, ix = O.mesh_batch(sh, spans, 8)
+    d_all, _, _ = O.sample_spans(sh, spans, 8)
+    voff2, ioff2, v2, ix2 = O.mesh_batch(sh, spans, 8, dists=d_all)
+    assert np.array_equal(voff, voff2) and np.array_equal(ioff, ioff2)
+    assert np.array_equal(v.view(np.uint32), v2.view(np.uint32)) and np.array_equal(ix, ix2)
+    for i in range(len(spans)):
+        d, vv, ii = O.surface_nets(sh, spans[i], 8)
+        assert np.array_equal(d.view(np.uint32), d_all[i].view(np.uint32))
+        assert np.array_equal(vv.view(np.uint32), v[int(voff[i]):int(voff[i + 1])].view(np.uint32))
+        assert np.array_equal(ii, ix[int(ioff[i]):int(ioff[i + 1])])
+    assert voff[3] == voff[2]  # the far span has no crossing cell
+
+
 def test_create_spans_tiles_parent():
     sp = np.array([-1.5, -1.5, -1.5, 1.5, 1.5, 1.5], F32)
     ch = O.create_spans(sp)
