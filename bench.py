@@ -1,13 +1,16 @@
 """Benchmark: Mandelbulb distance-estimation throughput on one MI355X.
 
-Headline workload (octree-adaptive sampling): root cube [-1.5, 1.5]^3, power 8,
-15 iterations, bailout 2; the octree is refined level-synchronously on the GPU
-to depth 8 (two forced levels, then split where |DE(centre)| < half diagonal
-or NaN) and every surface leaf is sampled as an 8^3-cell span (9^3 corners,
-the reference's one-cell overflow) with its crossing bits.  One step = the
-whole refinement + span sampling.  value = distance evaluations per second
-(centre + corner evaluations), inputs resident in HBM (they are generated on
-the device from the root span), parity arithmetic (bit-exact loop state).
+Headline workload (octree-adaptive sampling + mesh extraction): root cube
+[-1.5, 1.5]^3, power 8, 15 iterations, bailout 2; the octree is refined
+level-synchronously on the GPU to depth 8 (two forced levels, then split where
+|DE(centre)| < half diagonal or NaN), every surface leaf is sampled as an
+8^3-cell span (9^3 corners, the reference's one-cell overflow), and naive
+surface nets turns each leaf into vertices (6-tap difference normals, 7
+distance evaluations per vertex, as the reference) and quads.  One step = the
+whole refinement + span sampling + mesh extraction.  value = distance
+evaluations per second (centre + corner + vertex evaluations), inputs resident
+in HBM (they are generated on the device from the root span), parity
+arithmetic (bit-exact loop state).
 
 Prints one JSON line.  Options: --steps K --warmup W --mode parity|fast
 --cpu-seconds S (budget of the CPU baseline sample; 0 disables it).
@@ -49,8 +52,8 @@ def main() -> None:
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(every workload shards by independent leaves, rays or jobs; see DESIGN.md).", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path here "
+              "(every workload shards by independent leaves, rays or jobs).", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -78,11 +81,13 @@ def main() -> None:
     cross = torch.empty((nleaves, (res ** 3 + 63) // 64), dtype=torch.int64, device="cuda")
     center_evals = int(sum(int(lc[d]) for d in range(c2["min_depth"], c2["max_depth"] + 1)))
     corner_evals = nleaves * per
-    evals = center_evals + corner_evals
+    mesh_out = {}
 
     def step():
         lv, _ = dev.refine(shape, c2["root"], c2["min_depth"], c2["max_depth"], hd, leaves=leaves_buf)
-        dev.sample_spans(shape, lv, res, out=dists, cross=cross)
+        dev.sample_spans(shape, lv, res, with_cross=False, out=dists)
+        # the mesh's count pass classifies the cells and writes the crossing words
+        mesh_out["m"] = dev.mesh(shape, lv, dists, res, "fd6", cross=cross)
 
     for _ in range(args.warmup):
         step()
@@ -92,8 +97,15 @@ def main() -> None:
         step()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    voff, ioff, _, _ = mesh_out["m"]
+    nverts = int(voff[-1].item())
+    nquads = int(ioff[-1].item()) // 6
+    del mesh_out["m"]
+    vertex_evals = 7 * nverts
+    evals = center_evals + corner_evals + vertex_evals
     ms_per_step = 1e3 * elapsed / args.steps
     value = evals * args.steps / elapsed
+    phases = _phases(dev, torch, shape, c2, hd, leaves_buf, dists, cross, res)
 
     # Dominant kernel: the span-sampling refill kernel.  Its average launch
     # time is measured with HIP events on the stream it runs on (torch's
@@ -135,8 +147,9 @@ def main() -> None:
         "hbm_write_gbs": corner_evals * 4 / kernel_s / 1e9,
     }
 
-    # Reference CPU path (the oracle's op-exact restatement), timed on this
-    # host on a bounded sample of the same leaves.
+    # Reference CPU path (the oracle's op-exact restatement of the whole
+    # naive_surface_nets per leaf), timed on this host on a bounded sample of
+    # the same leaves, threaded over leaves like the reference's pool.
     cpu = None
     if args.cpu_seconds > 0:
         import oracle as O
@@ -146,16 +159,17 @@ def main() -> None:
         s = O.mandelbulb(c2["max_iters"], c2["bailout"])
         m = min(256, nleaves)
         t = time.perf_counter()
-        O.sample_spans(s, lv_host[rng.choice(nleaves, m, replace=False)], res, threads=threads)
+        O.mesh_batch(s, lv_host[rng.choice(nleaves, m, replace=False)], res, threads=threads)
         probe = time.perf_counter() - t
         m2 = int(min(nleaves, max(m, m * args.cpu_seconds / max(probe, 1e-3))))
         sample = lv_host[np.sort(rng.choice(nleaves, m2, replace=False))]
         t = time.perf_counter()
-        O.sample_spans(s, sample, res, threads=threads)
+        vo, _, _, _ = O.mesh_batch(s, sample, res, threads=threads)
         dt = time.perf_counter() - t
-        cpu = {"value": m2 * per / dt, "unit": "evals/s", "cores": threads, "kind": "port",
-               "sample": f"{m2} of {nleaves} depth-8 surface leaves (seeded), 9^3 corners each, "
-                         f"oracle restatement, {threads} threads, {dt:.1f} s"}
+        cpu_evals = m2 * per + 7 * int(vo[-1])
+        cpu = {"value": cpu_evals / dt, "unit": "evals/s", "cores": threads, "kind": "port",
+               "sample": f"{m2} of {nleaves} depth-8 surface leaves (seeded): 9^3 corners + 7 per vertex, "
+                         f"whole surface nets, oracle restatement, {threads} threads, {dt:.1f} s"}
 
     extra = {}
     for name, fn in (("c1_grid", bench_c1), ("c3_trace", bench_c3), ("c4_normals", bench_c4), ("c5_jobs", bench_c5)):
@@ -178,12 +192,17 @@ def main() -> None:
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "octree-adaptive span sampling: root [-1.5,1.5]^3, P8, 15 it, bailout 2, "
-                        "refine to depth 8 (|DE(centre)| < half diagonal), 8^3-cell spans per surface leaf",
+            "workload": "octree-adaptive sampling + mesh extraction: root [-1.5,1.5]^3, P8, 15 it, bailout 2, "
+                        "refine to depth 8 (|DE(centre)| < half diagonal), 8^3-cell spans per surface leaf, "
+                        "naive surface nets with 6-tap normals",
             "mode": args.mode,
             "surface_leaves": nleaves,
             "center_evals": center_evals,
             "corner_evals": corner_evals,
+            "vertex_evals": vertex_evals,
+            "vertices": nverts,
+            "quads": nquads,
+            "phases_ms": phases,
             "level_counts": [int(x) for x in lc],
             "launch": tuned,
             "parallelism": "single GPU",
@@ -193,6 +212,31 @@ def main() -> None:
         "extra": extra,
     }
     print(json.dumps(line))
+
+
+def _phases(dev, torch, shape, c2, hd, leaves_buf, dists, cross, res):
+    """Per-phase times of one step (HIP events on torch's current stream)."""
+    out = {}
+    marks = []
+
+    def mark():
+        e = torch.cuda.Event(enable_timing=True)
+        e.record()
+        marks.append(e)
+
+    torch.cuda.synchronize()
+    mark()
+    lv, _ = dev.refine(shape, c2["root"], c2["min_depth"], c2["max_depth"], hd, leaves=leaves_buf)
+    mark()
+    dev.sample_spans(shape, lv, res, with_cross=False, out=dists)
+    mark()
+    m = dev.mesh(shape, lv, dists, res, "fd6", cross=cross)
+    mark()
+    torch.cuda.synchronize()
+    for name, a, b in (("refine", 0, 1), ("sample_spans", 1, 2), ("mesh", 2, 3)):
+        out[name] = marks[a].elapsed_time(marks[b])
+    del m
+    return out
 
 
 def _time_ms(fn, reps):

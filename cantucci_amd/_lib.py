@@ -141,7 +141,7 @@ _SIGS = {
     "cth_graph_launch": (_I, [_P, _P]),
     "cth_graph_destroy": (None, [_P]),
     "cth_render_depth_dev": (_I, [_P, _SH, _P, _U, _U, _U, _U, _F, _F, _F, _P, _P, _P]),
-    "cth_mesh_count_dev": (_I, [_P, _P, _S, _U, _P, _P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), _P]),
+    "cth_mesh_count_dev": (_I, [_P, _P, _S, _U, _P, _P, _P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), _P]),
     "cth_mesh_emit_dev": (_I, [_P, _SH, _P, _P, _S, _U, _I, _P, _P, C.c_uint64, _P, _P, _P]),
     "cth_mesh_span": (_I, [_P, _SH, C.POINTER(CthSpan), _U, _I, _P, _S, _P, _S, C.POINTER(_S), C.POINTER(_S)]),
 }

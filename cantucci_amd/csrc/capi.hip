@@ -462,12 +462,12 @@ int cth_sample_spans(cth_ctx* c, const cth_shape* s, const cth_span* spans, size
 }
 
 int cth_mesh_count_dev(cth_ctx* c, const float* dists, size_t nspans, uint32_t res, uint64_t* voff, uint64_t* ioff,
-                       uint64_t* nverts, uint64_t* nidx, void* stream) {
+                       uint64_t* cross_bits, uint64_t* nverts, uint64_t* nidx, void* stream) {
     if (!c) return fail(CTH_E_INVALID, "null context");
     if (!voff || !ioff || !nverts || !nidx || (nspans && !dists)) return fail(CTH_E_INVALID, "null buffer");
     if (nspans >= (size_t(1) << 31)) return fail(CTH_E_INVALID, "more than 2^31 spans");
     uint64_t tot[2] = {0, 0};
-    int rc = mesh_count(*c, dists, (uint32_t)nspans, res, voff, ioff, tot, pick_stream(c, stream));
+    int rc = mesh_count(*c, dists, (uint32_t)nspans, res, voff, ioff, cross_bits, tot, pick_stream(c, stream));
     *nverts = tot[0];
     *nidx = tot[1];
     return rc;
@@ -508,7 +508,7 @@ int cth_mesh_span(cth_ctx* c, const cth_shape* s, const cth_span* span, uint32_t
     if (!rc) rc = cth_sample_spans_dev(c, s, (const float*)dsp.p, 1, res, (float*)dd.p, nullptr, nullptr, c->stream);
     uint64_t nv = 0, ni = 0;
     uint64_t* off = (uint64_t*)doff.p;
-    if (!rc) rc = cth_mesh_count_dev(c, (const float*)dd.p, 1, res, off, off + 2, &nv, &ni, c->stream);
+    if (!rc) rc = cth_mesh_count_dev(c, (const float*)dd.p, 1, res, off, off + 2, nullptr, &nv, &ni, c->stream);
     if (rc) return rc;
     *nverts = (size_t)nv;
     *nidx = (size_t)ni;

@@ -94,7 +94,7 @@ uint32_t jobs_offset(const JobSet* js, uint32_t i);
 int jobs_run(Ctx& c, const JobSet& js, float* dists, float* normals, unsigned long long* done,
              unsigned long long* start, hipStream_t st);
 int mesh_count(Ctx& c, const float* dists, uint32_t nleaves, uint32_t res, uint64_t* voff, uint64_t* ioff,
-               uint64_t totals[2], hipStream_t st);
+               uint64_t* cross, uint64_t totals[2], hipStream_t st);
 int mesh_emit(Ctx& c, const cth_shape& s, const float* spans, const float* dists, uint32_t nleaves, uint32_t res,
               int normals, const uint64_t* voff, const uint64_t* ioff, uint64_t nverts, float* verts,
               uint32_t* idx, hipStream_t st);

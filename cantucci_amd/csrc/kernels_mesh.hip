@@ -122,20 +122,28 @@ __device__ __forceinline__ void cell_xyz(const MeshDims& m, uint32_t c, uint32_t
 // Per leaf: vertex count and index count (6 per quad).  Slot `nleaves` of
 // both count arrays is zeroed so that an exclusive scan over nleaves + 1
 // entries ends with the totals.
+// With `cross` non-null it also writes the leaf's crossing words (bit c of
+// word c / 64 = cell c crosses), the layout cth_sample_spans_dev uses.
+// One workgroup per leaf (the large resolutions; see k_mesh_count_w).
 __global__ __launch_bounds__(kMeshBlock) void k_mesh_count(const float* __restrict__ dists, MeshDims m,
                                                            uint32_t nleaves, uint64_t* __restrict__ vcnt,
-                                                           uint64_t* __restrict__ icnt) {
+                                                           uint64_t* __restrict__ icnt, uint64_t* __restrict__ cross) {
     __shared__ uint32_t wsum[kMeshBlock / 64];
     const uint32_t leaf = blockIdx.x;
     const float* D = dists + (size_t)leaf * m.n1 * m.n1 * m.n1;
+    const uint32_t nwords = (m.cells + 63u) >> 6;
     uint32_t nv = 0, nf = 0;
+    // Wave w handles cells 64 w + lane + 256 k: one crossing word per ballot.
     for (uint32_t c = threadIdx.x; c < m.cells; c += blockDim.x) {
         uint32_t x, y, z;
         cell_xyz(m, c, x, y, z);
         float d[8];
         uint32_t mk = load_cell(D, m.n1, x, y, z, d);
-        nv += crossing(mk) ? 1u : 0u;
+        const bool cr = crossing(mk);
+        nv += cr ? 1u : 0u;
         nf += (uint32_t)__popc(face_bits(mk, x, y, z));
+        const uint64_t bits = __ballot(cr);
+        if (cross && (threadIdx.x & 63u) == 0) cross[(size_t)leaf * nwords + (c >> 6)] = bits;
     }
     nv = block_sum(nv, wsum);
     nf = block_sum(nf, wsum);
@@ -195,6 +203,60 @@ __device__ __forceinline__ void place_vertex(const float d[8], uint32_t x, uint3
     for (int a = 0; a < 3; ++a) p[a] = 0.0f + div_rn(sum[a], fc);
 }
 
+// Adjusted span (src/mesh/buffer.rs:64-67), cell step (:101) and the 6-tap
+// half-step 0.7 * (e' - s') / res (:257) of a leaf.
+__device__ __forceinline__ void leaf_frame(const float* sp, float fres, float s1[3], float step[3], float delta[3]) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float s0 = sp[a], e0 = sp[3 + a];
+        const float of = div_rn(e0 - s0, fres);
+        const float sa = s0 + -of;
+        const float ea = e0 + of;
+        s1[a] = sa;
+        step[a] = div_rn(ea - sa, fres);
+        delta[a] = div_rn(0.7f * (ea - sa), fres);
+    }
+}
+
+__device__ __forceinline__ void put_vertex(float* v, const float p[3], const float delta[3]) {
+    v[0] = p[0];
+    v[1] = p[1];
+    v[2] = p[2];
+    v[3] = delta[0];
+    v[4] = delta[1];
+    v[5] = delta[2];
+    v[6] = 0.0f;
+}
+
+// The quads of corner (x, y, z) (src/mesh/buffer.rs:299-363), in the
+// reference's order (+x, +y, +z edge) and winding (`dists[(x,y,z)] < 0.0`
+// picks the clockwise pair).  pt(x, y, z) is the cell's vertex number.
+template <class PT>
+__device__ __forceinline__ void emit_quads(uint32_t* out, uint32_t fb, bool neg, uint32_t x, uint32_t y, uint32_t z,
+                                           const PT& pt) {
+    const uint32_t v3 = pt(x, y, z);
+    if (fb & 1u) {  // +x edge
+        const uint32_t v0 = pt(x, y - 1, z - 1), v1 = pt(x, y - 1, z), v2 = pt(x, y, z - 1);
+        const uint32_t q[6] = {v0, v2, v1, v1, v2, v3}, r[6] = {v0, v1, v2, v1, v3, v2};
+#pragma unroll
+        for (int k = 0; k < 6; ++k) out[k] = neg ? q[k] : r[k];
+        out += 6;
+    }
+    if (fb & 2u) {  // +y edge
+        const uint32_t v0 = pt(x - 1, y, z - 1), v1 = pt(x - 1, y, z), v2 = pt(x, y, z - 1);
+        const uint32_t q[6] = {v0, v1, v2, v1, v3, v2}, r[6] = {v0, v2, v1, v1, v2, v3};
+#pragma unroll
+        for (int k = 0; k < 6; ++k) out[k] = neg ? q[k] : r[k];
+        out += 6;
+    }
+    if (fb & 4u) {  // +z edge
+        const uint32_t v0 = pt(x - 1, y - 1, z), v1 = pt(x - 1, y, z), v2 = pt(x, y - 1, z);
+        const uint32_t q[6] = {v0, v2, v1, v1, v2, v3}, r[6] = {v0, v1, v2, v1, v3, v2};
+#pragma unroll
+        for (int k = 0; k < 6; ++k) out[k] = neg ? q[k] : r[k];
+    }
+}
+
 __device__ __forceinline__ uint32_t vertex_of(const uint64_t* words, const uint32_t* wpre, const MeshDims& m,
                                               uint32_t x, uint32_t y, uint32_t z) {
     const uint32_t c = (((x << m.lg) | y) << m.lg) | z;
@@ -250,21 +312,11 @@ __global__ __launch_bounds__(kMeshBlock) void k_mesh_emit(const float* __restric
     }
     __syncthreads();
 
-    // Adjusted span (src/mesh/buffer.rs:64-67), cell step (:101), FD half-step (:257).
-    const float* sp = spans + 6 * (size_t)leaf;
     float s1[3], step[3], delta[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const float s0 = sp[a], e0 = sp[3 + a];
-        const float of = div_rn(e0 - s0, m.fres);
-        const float sa = s0 + -of;
-        const float ea = e0 + of;
-        s1[a] = sa;
-        step[a] = div_rn(ea - sa, m.fres);
-        delta[a] = div_rn(0.7f * (ea - sa), m.fres);
-    }
+    leaf_frame(spans + 6 * (size_t)leaf, m.fres, s1, step, delta);
     const uint64_t vbase = voff[leaf];
     const uint64_t ibase = ioff[leaf];
+    auto pt = [&](uint32_t px, uint32_t py, uint32_t pz) { return vertex_of(words, wpre, m, px, py, pz); };
 
     uint32_t fcarry = 0;
     for (uint32_t base = 0; base < m.cells; base += blockDim.x) {
@@ -281,47 +333,218 @@ __global__ __launch_bounds__(kMeshBlock) void k_mesh_emit(const float* __restric
         if (c < m.cells && crossing(mk)) {
             float p[3];
             place_vertex(d, x, y, z, s1, step, p);
-            float* v = verts + 7 * (vbase + vertex_of(words, wpre, m, x, y, z));
-            v[0] = p[0];
-            v[1] = p[1];
-            v[2] = p[2];
-            v[3] = delta[0];
-            v[4] = delta[1];
-            v[5] = delta[2];
-            v[6] = 0.0f;
+            put_vertex(verts + 7 * (vbase + pt(x, y, z)), p, delta);
         }
-        if (fb) {
-            uint32_t* out = idx + ibase + 6ull * (fcarry + frank);
-            const bool neg = d[0] < 0.0f;  // dists[(x, y, z)] < 0.0
-            const uint32_t v3 = vertex_of(words, wpre, m, x, y, z);
-            if (fb & 1u) {  // +x edge
-                const uint32_t v0 = vertex_of(words, wpre, m, x, y - 1, z - 1);
-                const uint32_t v1 = vertex_of(words, wpre, m, x, y - 1, z);
-                const uint32_t v2 = vertex_of(words, wpre, m, x, y, z - 1);
-                const uint32_t q[6] = {v0, v2, v1, v1, v2, v3}, r[6] = {v0, v1, v2, v1, v3, v2};
-#pragma unroll
-                for (int k = 0; k < 6; ++k) out[k] = neg ? q[k] : r[k];
-                out += 6;
-            }
-            if (fb & 2u) {  // +y edge
-                const uint32_t v0 = vertex_of(words, wpre, m, x - 1, y, z - 1);
-                const uint32_t v1 = vertex_of(words, wpre, m, x - 1, y, z);
-                const uint32_t v2 = vertex_of(words, wpre, m, x, y, z - 1);
-                const uint32_t q[6] = {v0, v1, v2, v1, v3, v2}, r[6] = {v0, v2, v1, v1, v2, v3};
-#pragma unroll
-                for (int k = 0; k < 6; ++k) out[k] = neg ? q[k] : r[k];
-                out += 6;
-            }
-            if (fb & 4u) {  // +z edge
-                const uint32_t v0 = vertex_of(words, wpre, m, x - 1, y - 1, z);
-                const uint32_t v1 = vertex_of(words, wpre, m, x - 1, y, z);
-                const uint32_t v2 = vertex_of(words, wpre, m, x, y - 1, z);
-                const uint32_t q[6] = {v0, v2, v1, v1, v2, v3}, r[6] = {v0, v1, v2, v1, v3, v2};
-#pragma unroll
-                for (int k = 0; k < 6; ++k) out[k] = neg ? q[k] : r[k];
-            }
-        }
+        if (fb) emit_quads(idx + ibase + 6ull * (fcarry + frank), fb, d[0] < 0.0f, x, y, z, pt);
         fcarry += tile_faces;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Wave-per-leaf kernels for res <= 32 (count) and res <= 16 (emit; the
+// octree workload meshes 8^3 cells per leaf): one wave owns a leaf, each lane a run of whole cell rows
+// (x, y, z = 0..res-1), which are contiguous in cube() order.  A lane
+// streams the four corner columns of its row along z (each a contiguous run
+// of res + 1 floats), so every corner load serves two cells, and no
+// workgroup barrier is needed: the leaf's row bookkeeping lives in the
+// wave's own slice of LDS.  The grid is persistent over leaves.
+// ---------------------------------------------------------------------------
+
+// The wave's LDS writes become visible to its other lanes.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t& total) {
+    const uint32_t inc = wave_incl_scan(v);
+    total = (uint32_t)__shfl((int)inc, 63, 64);
+    return inc - v;
+}
+
+// Calls f(z, d[8], mask) for the cells (x, y, 0..RES-1) of the leaf's
+// corner grid D ((RES+1)^3, GridTable order).
+template <int RES, class F>
+__device__ __forceinline__ void row_cells(const float* __restrict__ D, uint32_t x, uint32_t y, F&& f) {
+    constexpr uint32_t N1 = RES + 1;
+    const float* c00 = D + ((size_t)x * N1 + y) * N1;
+    const float* c01 = c00 + N1;
+    const float* c10 = c00 + N1 * N1;
+    const float* c11 = c10 + N1;
+    float p0 = c00[0], p1 = c01[0], p2 = c10[0], p3 = c11[0];
+#pragma unroll 8
+    for (uint32_t z = 0; z < (uint32_t)RES; ++z) {
+        const float q0 = c00[z + 1], q1 = c01[z + 1], q2 = c10[z + 1], q3 = c11[z + 1];
+        const float d[8] = {p0, q0, p1, q1, p2, q2, p3, q3};
+        uint32_t mk = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) mk |= sign_bit(d[k]) << k;
+        f(z, d, mk);
+        p0 = q0;
+        p1 = q1;
+        p2 = q2;
+        p3 = q3;
+    }
+}
+
+template <int RES>
+struct WaveLeaf {
+    static constexpr uint32_t N1 = RES + 1;
+    static constexpr uint32_t ROWS = RES * RES;
+    static constexpr uint32_t RPL = (ROWS + 63) / 64;  // rows per lane
+    static constexpr uint32_t CELLS = RES * RES * RES;
+    static constexpr uint32_t NWORDS = (CELLS + 63) / 64;
+    static constexpr uint32_t RPW = 64 / RES;  // rows per crossing word
+    static constexpr size_t CORNERS = (size_t)N1 * N1 * N1;
+};
+
+template <int RES>
+__global__ __launch_bounds__(kMeshBlock) void k_mesh_count_w(const float* __restrict__ dists, uint32_t nleaves,
+                                                             uint64_t* __restrict__ vcnt, uint64_t* __restrict__ icnt,
+                                                             uint64_t* __restrict__ cross) {
+    using L = WaveLeaf<RES>;
+    __shared__ uint32_t rowbits_all[kMeshBlock / 64][L::ROWS];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t* rowbits = rowbits_all[wave];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        vcnt[nleaves] = 0;
+        icnt[nleaves] = 0;
+    }
+    const uint32_t wstride = gridDim.x * (blockDim.x >> 6);
+    for (uint32_t leaf = blockIdx.x * (blockDim.x >> 6) + wave; leaf < nleaves; leaf += wstride) {
+        const float* D = dists + (size_t)leaf * L::CORNERS;
+        uint32_t nv = 0, nf = 0;
+        for (uint32_t k = 0; k < L::RPL; ++k) {
+            const uint32_t row = lane * L::RPL + k;
+            if (row < L::ROWS) {
+                const uint32_t x = row / RES, y = row % RES;
+                uint32_t bits = 0;
+                row_cells<RES>(D, x, y, [&](uint32_t z, const float*, uint32_t mk) {
+                    const bool cr = crossing(mk);
+                    bits |= (cr ? 1u : 0u) << z;
+                    nf += (uint32_t)__popc(face_bits(mk, x, y, z));
+                });
+                nv += (uint32_t)__popc(bits);
+                rowbits[row] = bits;
+            }
+        }
+        uint32_t tv, tf;
+        (void)wave_excl_scan(nv, tv);
+        (void)wave_excl_scan(nf, tf);
+        if (lane == 0) {
+            vcnt[leaf] = tv;
+            icnt[leaf] = 6ull * tf;
+        }
+        if (cross) {
+            wave_sync();
+            for (uint32_t w = lane; w < L::NWORDS; w += 64) {
+                uint64_t word = 0;
+                for (uint32_t j = 0; j < L::RPW; ++j) {
+                    const uint32_t r = w * L::RPW + j;
+                    if (r < L::ROWS) word |= (uint64_t)rowbits[r] << (j * RES);
+                }
+                cross[(size_t)leaf * L::NWORDS + w] = word;
+            }
+        }
+        wave_sync();  // rowbits is rewritten for the next leaf
+    }
+}
+
+template <int RES>
+__global__ __launch_bounds__(kMeshBlock) void k_mesh_emit_w(const float* __restrict__ spans,
+                                                            const float* __restrict__ dists, uint32_t nleaves,
+                                                            const uint64_t* __restrict__ voff,
+                                                            const uint64_t* __restrict__ ioff,
+                                                            float* __restrict__ verts, uint32_t* __restrict__ idx) {
+    using L = WaveLeaf<RES>;
+    static_assert(RES <= 16, "cell numbers and face offsets are packed into LDS lists");
+    // Per wave: row bookkeeping and the leaf's compacted vertex and quad lists.
+    struct Lists {
+        uint32_t rb[L::ROWS];     // crossing bits of the row's cells
+        uint32_t rp[L::ROWS];     // vertices before the row
+        uint32_t rn[L::ROWS];     // bit z: dists[(x, y, z)] < 0.0
+        uint64_t rf[L::ROWS];     // face bits of cell z at bits 3z..3z+2
+        uint16_t vlist[L::CELLS];  // cell of each vertex, in order
+        uint32_t flist[L::CELLS];  // cell | face bits << 12 | first quad << 16
+    };
+    __shared__ Lists lists_all[kMeshBlock / 64];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    Lists& W = lists_all[wave];
+    const uint32_t wstride = gridDim.x * (blockDim.x >> 6);
+    for (uint32_t leaf = blockIdx.x * (blockDim.x >> 6) + wave; leaf < nleaves; leaf += wstride) {
+        const float* D = dists + (size_t)leaf * L::CORNERS;
+        // Pass over the lane's rows: crossing, sign and face bits.
+        uint32_t lv = 0, lf = 0, lc = 0;
+        for (uint32_t k = 0; k < L::RPL; ++k) {
+            const uint32_t row = lane * L::RPL + k;
+            if (row < L::ROWS) {
+                const uint32_t x = row / RES, y = row % RES;
+                uint32_t bits = 0, neg = 0;
+                uint64_t fw = 0;
+                row_cells<RES>(D, x, y, [&](uint32_t z, const float* d, uint32_t mk) {
+                    bits |= (crossing(mk) ? 1u : 0u) << z;
+                    neg |= (d[0] < 0.0f ? 1u : 0u) << z;
+                    const uint32_t fb = face_bits(mk, x, y, z);
+                    fw |= (uint64_t)fb << (3u * z);
+                    lf += (uint32_t)__popc(fb);
+                    lc += fb ? 1u : 0u;
+                });
+                W.rb[row] = bits;
+                W.rn[row] = neg;
+                W.rf[row] = fw;
+                lv += (uint32_t)__popc(bits);
+            }
+        }
+        uint32_t tv, tf, tc;
+        uint32_t vex = wave_excl_scan(lv, tv);
+        uint32_t fo = wave_excl_scan(lf, tf);
+        uint32_t cex = wave_excl_scan(lc, tc);
+        // Compacted lists, in cube() cell order.
+        for (uint32_t k = 0; k < L::RPL; ++k) {
+            const uint32_t row = lane * L::RPL + k;
+            if (row < L::ROWS) {
+                W.rp[row] = vex;
+                for (uint32_t bits = W.rb[row]; bits; bits &= bits - 1u)
+                    W.vlist[vex++] = (uint16_t)(row * RES + (uint32_t)__builtin_ctz(bits));
+                const uint64_t fw = W.rf[row];
+                for (uint32_t z = 0; z < (uint32_t)RES; ++z) {
+                    const uint32_t fb = (uint32_t)(fw >> (3u * z)) & 7u;
+                    if (fb) {
+                        W.flist[cex++] = (row * RES + z) | (fb << 12) | (fo << 16);
+                        fo += (uint32_t)__popc(fb);
+                    }
+                }
+            }
+        }
+        wave_sync();
+        float s1[3], step[3], delta[3];
+        leaf_frame(spans + 6 * (size_t)leaf, (float)RES, s1, step, delta);
+        const uint64_t vbase = voff[leaf];
+        // Vertex placement, one vertex per lane per round.
+        for (uint32_t j = lane; j < tv; j += 64) {
+            const uint32_t cell = W.vlist[j];
+            const uint32_t x = cell / (RES * RES), y = (cell / RES) % RES, z = cell % RES;
+            float d[8];
+            (void)load_cell(D, L::N1, x, y, z, d);
+            float p[3];
+            place_vertex(d, x, y, z, s1, step, p);
+            put_vertex(verts + 7 * (vbase + j), p, delta);
+        }
+        // Quads, one corner per lane per round.
+        uint32_t* iout = idx + ioff[leaf];
+        auto pt = [&](uint32_t px, uint32_t py, uint32_t pz) {
+            const uint32_t r = px * RES + py;
+            return W.rp[r] + (uint32_t)__popc(W.rb[r] & ((1u << pz) - 1u));
+        };
+        for (uint32_t j = lane; j < tc; j += 64) {
+            const uint32_t e = W.flist[j];
+            const uint32_t cell = e & 0xFFFu, fb = (e >> 12) & 7u, q = e >> 16;
+            const uint32_t x = cell / (RES * RES), y = (cell / RES) % RES, z = cell % RES;
+            const bool neg = (W.rn[x * RES + y] >> z) & 1u;
+            emit_quads(iout + 6ull * q, fb, neg, x, y, z, pt);
+        }
+        wave_sync();  // the lists are rewritten for the next leaf
     }
 }
 
@@ -405,10 +628,24 @@ int mesh_dims(uint32_t res, MeshDims& m) {
     return CTH_OK;
 }
 
+// Persistent grid of wave-per-leaf kernels: at most 8 workgroups per CU.
+inline uint32_t wave_grid(const Ctx& c, uint32_t nleaves) {
+    const uint64_t want = (uint64_t)c.num_cus * 8u;
+    const uint64_t need = ((uint64_t)nleaves + 3u) / 4u;
+    return (uint32_t)std::max<uint64_t>(1, std::min(want, need));
+}
+
+#define LAUNCH_COUNT_W(R)                                                                                  \
+    hipLaunchKernelGGL(k_mesh_count_w<R>, dim3(blocks), dim3(kMeshBlock), 0, st, dists, nleaves, counts, \
+                       counts + cnt, cross)
+#define LAUNCH_EMIT_W(R)                                                                                        \
+    hipLaunchKernelGGL(k_mesh_emit_w<R>, dim3(blocks), dim3(kMeshBlock), 0, st, spans, dists, nleaves, voff, ioff, \
+                       verts, idx)
+
 }  // namespace
 
-int mesh_count(Ctx&, const float* dists, uint32_t nleaves, uint32_t res, uint64_t* voff, uint64_t* ioff,
-               uint64_t totals[2], hipStream_t st) {
+int mesh_count(Ctx& c, const float* dists, uint32_t nleaves, uint32_t res, uint64_t* voff, uint64_t* ioff,
+               uint64_t* cross, uint64_t totals[2], hipStream_t st) {
     MeshDims m;
     if (int rc = mesh_dims(res, m)) return rc;
     totals[0] = totals[1] = 0;
@@ -425,8 +662,20 @@ int mesh_count(Ctx&, const float* dists, uint32_t nleaves, uint32_t res, uint64_
     size_t tmp_bytes = 0;
     int rc = check_hip(hipMallocAsync((void**)&counts, 2 * cnt * sizeof(uint64_t), st), "hipMallocAsync counts");
     if (!rc) {
-        hipLaunchKernelGGL(k_mesh_count, dim3(nleaves), dim3(kMeshBlock), 0, st, dists, m, nleaves, counts,
-                           counts + cnt);
+        if (res <= 32) {
+            const uint32_t blocks = wave_grid(c, nleaves);
+            switch (res) {
+                case 1: LAUNCH_COUNT_W(1); break;
+                case 2: LAUNCH_COUNT_W(2); break;
+                case 4: LAUNCH_COUNT_W(4); break;
+                case 8: LAUNCH_COUNT_W(8); break;
+                case 16: LAUNCH_COUNT_W(16); break;
+                default: LAUNCH_COUNT_W(32); break;
+            }
+        } else {
+            hipLaunchKernelGGL(k_mesh_count, dim3(nleaves), dim3(kMeshBlock), 0, st, dists, m, nleaves, counts,
+                               counts + cnt, cross);
+        }
         rc = check_hip(hipGetLastError(), "mesh count launch");
     }
     if (!rc)
@@ -457,7 +706,19 @@ int mesh_emit(Ctx& c, const cth_shape& s, const float* spans, const float* dists
     if (nleaves == 0) return CTH_OK;
     const uint32_t nwords = (m.cells + 63u) >> 6;
     const size_t lds = (size_t)nwords * (sizeof(uint64_t) + sizeof(uint32_t));
-    hipLaunchKernelGGL(k_mesh_emit, dim3(nleaves), dim3(kMeshBlock), lds, st, spans, dists, m, voff, ioff, verts, idx);
+    if (res <= 16) {
+        const uint32_t blocks = wave_grid(c, nleaves);
+        switch (res) {
+            case 1: LAUNCH_EMIT_W(1); break;
+            case 2: LAUNCH_EMIT_W(2); break;
+            case 4: LAUNCH_EMIT_W(4); break;
+            case 8: LAUNCH_EMIT_W(8); break;
+            default: LAUNCH_EMIT_W(16); break;
+        }
+    } else {
+        hipLaunchKernelGGL(k_mesh_emit, dim3(nleaves), dim3(kMeshBlock), lds, st, spans, dists, m, voff, ioff, verts,
+                           idx);
+    }
     int rc = check_hip(hipGetLastError(), "mesh emit launch");
     if (rc || nverts == 0) return rc;
 

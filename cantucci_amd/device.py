@@ -95,11 +95,12 @@ _NORMALS = {"fd6": _lib.CTH_NORMALS_FD6, "tangent": _lib.CTH_NORMALS_TANGENT}
 
 
 def mesh(shape: Shape, spans: torch.Tensor, dists: torch.Tensor, res: int, normals: str = "fd6",
-         ctx: Context | None = None):
+         ctx: Context | None = None, cross: torch.Tensor | None = None):
     """Naive surface nets, steps 2-3, over leaves already sampled by
     `sample_spans` at the same `res`.  Returns (vert_offsets (n+1,) int64,
     idx_offsets (n+1,) int64, vertices (nv, 7) f32 in the `Vertex` layout,
-    indices (ni,) int32 holding u32 leaf-local vertex numbers)."""
+    indices (ni,) int32 holding u32 leaf-local vertex numbers).  `cross`
+    (optional (n, ceil(res^3/64)) int64) receives the crossing words."""
     ctx = ctx or shape.ctx
     spans = _f32(spans).contiguous()
     dists = _f32(dists).contiguous()
@@ -110,8 +111,10 @@ def mesh(shape: Shape, spans: torch.Tensor, dists: torch.Tensor, res: int, norma
     ioff = torch.empty(m + 1, dtype=torch.int64, device=spans.device)
     nv = C.c_uint64()
     ni = C.c_uint64()
-    check(lib.cth_mesh_count_dev(ctx.handle, _ptr(dists), m, res, _ptr(voff), _ptr(ioff), C.byref(nv), C.byref(ni),
-                                 _stream()))
+    if cross is not None and cross.numel() != m * ((res ** 3 + 63) // 64):
+        raise ValueError("cross must hold ceil(res^3/64) words per span")
+    check(lib.cth_mesh_count_dev(ctx.handle, _ptr(dists), m, res, _ptr(voff), _ptr(ioff), _ptr(cross), C.byref(nv),
+                                 C.byref(ni), _stream()))
     verts = torch.empty((max(nv.value, 1), 7), dtype=torch.float32, device=spans.device)
     idx = torch.empty(max(ni.value, 1), dtype=torch.int32, device=spans.device)
     check(lib.cth_mesh_emit_dev(ctx.handle, C.byref(shape.desc), _ptr(spans), _ptr(dists), m, res,

@@ -14,7 +14,7 @@ import sys
 
 def main() -> None:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--legs", default="spans,trace,tangent,fd,jobs,copy")
+    ap.add_argument("--legs", default="spans,mesh,trace,tangent,fd,jobs,copy")
     ap.add_argument("--reps", type=int, default=2)
     args = ap.parse_args()
     legs = set(args.legs.split(","))
@@ -37,6 +37,12 @@ def main() -> None:
             dev.sample_spans(m8, leaves, 8, with_cross=False, out=out)
         torch.cuda.synchronize()
         print("spans done", leaves.shape[0], flush=True)
+        if "mesh" in legs:
+            for _ in range(args.reps):
+                m = dev.mesh(m8, leaves, out, 8, "fd6")
+                del m
+            torch.cuda.synchronize()
+            print("mesh done", flush=True)
         del out
     if "trace" in legs:
         autotune.apply(ctx, "trace")

@@ -340,10 +340,11 @@ typedef struct cth_vertex {
  * both IEEE signs (src/mesh/buffer.rs:127-147), six indices per corner edge
  * whose end signs differ (:299-363).  vert_offsets / idx_offsets (device,
  * nspans + 1 each) receive the exclusive prefix sums; *nverts / *nidx (host)
- * the totals.  res: power of two in [1, 64].  Returns after the totals are
- * known (kernels run on `stream`). */
+ * the totals.  cross_bits (nullable, device) receives the same crossing words
+ * as cth_sample_spans_dev.  res: power of two in [1, 64].  Returns after the
+ * totals are known (kernels run on `stream`). */
 int cth_mesh_count_dev(cth_ctx* ctx, const float* dists, size_t nspans, uint32_t res, uint64_t* vert_offsets,
-                       uint64_t* idx_offsets, uint64_t* nverts, uint64_t* nidx, void* stream);
+                       uint64_t* idx_offsets, uint64_t* cross_bits, uint64_t* nverts, uint64_t* nidx, void* stream);
 
 /* Surface nets, pass 2: leaf i's vertices go to vertices[vert_offsets[i] ...]
  * in cube() cell order (x slowest, z fastest; src/mesh/buffer.rs:113), its

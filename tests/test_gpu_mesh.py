@@ -44,9 +44,11 @@ def c2_leaves(count=2000, seed=0, depth=8):
 def run_gpu_mesh(torch, shape, spans, res, normals="fd6"):
     from cantucci_amd import device as dev
     sp = gpu(torch, spans)
-    d, _ = dev.sample_spans(shape, sp, res, with_cross=False)
-    voff, ioff, v, ix = dev.mesh(shape, sp, d, res, normals)
+    d, cb = dev.sample_spans(shape, sp, res)
+    cb2 = torch.full_like(cb, -1)
+    voff, ioff, v, ix = dev.mesh(shape, sp, d, res, normals, cross=cb2)
     torch.cuda.synchronize()
+    assert torch.equal(cb, cb2), "crossing words of the count pass differ from span sampling's"
     return (d.cpu().numpy(), voff.cpu().numpy().astype(np.uint64), ioff.cpu().numpy().astype(np.uint64),
             v.cpu().numpy(), ix.cpu().numpy().view(np.uint32))
 
