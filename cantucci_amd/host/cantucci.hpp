@@ -1,0 +1,258 @@
+// C++17 host mirror of cantucci's `Shape` trait and mesh-buffer entry point,
+// over the C ABI of libcantucci_hip.so (include/cantucci_hip.h).
+//
+// The reference's interface (src/shape/mod.rs:24-105, src/shape/mandelbulb.rs:12-33,
+// src/shape/sphere.rs:6-52, src/mesh/buffer.rs:23-42) is kept method for method,
+// with the same names, argument meaning and results.  Where the reference
+// panics (an `assert!`, or `unwrap` on a missing upper bound) this layer throws
+// cantucci::Error carrying the library status; nothing aborts.  Every query
+// runs on the GPU: there is no CPU fallback, and constructing a Context
+// without a usable gfx950 device throws.
+//
+// Header-only; link with -lcantucci_hip.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <optional>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "cantucci_hip.h"
+
+namespace cantucci {
+
+// A failed library call.  `status` is a cth_status.
+class Error : public std::runtime_error {
+  public:
+    Error(int status, const std::string& what) : std::runtime_error(what), status(status) {}
+    int status;
+};
+
+inline void check(int rc) {
+    if (rc != CTH_OK) {
+        const char* m = cth_last_error();
+        throw Error(rc, m ? m : "cantucci_hip error");
+    }
+}
+
+// cgmath `Point3<f32>` (12 bytes, the layout of cth_point3).
+struct Point3 {
+    float x, y, z;
+};
+static_assert(sizeof(Point3) == sizeof(cth_point3), "Point3 must be 12 bytes");
+
+// octree `Span = Range<Point3<f32>>` (src/octree/mod.rs:13).
+struct Span {
+    Point3 start, end;
+};
+static_assert(sizeof(Span) == sizeof(cth_span), "Span must be 24 bytes");
+
+// A library context: one HIP stream on one device.  Calls on one context
+// must not overlap in time; use one context per host thread (the reference
+// shares one `Arc<dyn Shape>` across its pool threads, src/mesh/mod.rs:61-62).
+class Context {
+  public:
+    explicit Context(int device = 0, int mode = CTH_MODE_PARITY) {
+        check(cth_ctx_create(device, &ctx_));
+        if (mode != CTH_MODE_PARITY) {
+            int rc = cth_ctx_set_mode(ctx_, mode);
+            if (rc != CTH_OK) {
+                cth_ctx_destroy(ctx_);
+                check(rc);
+            }
+        }
+    }
+    ~Context() { cth_ctx_destroy(ctx_); }
+    Context(const Context&) = delete;
+    Context& operator=(const Context&) = delete;
+    cth_ctx* get() const { return ctx_; }
+
+  private:
+    cth_ctx* ctx_ = nullptr;
+};
+
+// The `Shape` trait (src/shape/mod.rs:24-105).  Shapes are plain descriptors
+// evaluated through a context; they are cheap to copy.
+class Shape {
+  public:
+    virtual ~Shape() = default;
+
+    // fn min_distance_from(&self, p: Point3<f32>) -> f32          (:37)
+    float min_distance_from(Point3 p) const { return batch_min_distance_from(std::vector<Point3>{p})[0]; }
+
+    // fn bounding_box(&self) -> Range<Point3<f32>>                (:39)
+    Span bounding_box() const {
+        cth_span s;
+        check(cth_bounding_box(&desc_, &s));
+        return Span{{s.start[0], s.start[1], s.start[2]}, {s.end[0], s.end[1], s.end[2]}};
+    }
+
+    // fn max_distance_from(&self, _p) -> Option<f32> { None }      (:62-64)
+    virtual std::optional<float> max_distance_from(Point3) const { return std::nullopt; }
+
+    // fn bounded_distance_from(&self, p) -> (f32, Option<f32>)     (:73-75)
+    std::pair<float, std::optional<float>> bounded_distance_from(Point3 p) const {
+        return {min_distance_from(p), max_distance_from(p)};
+    }
+
+    // fn contains(&self, p) -> bool                                (:78-80)
+    virtual bool contains(Point3 p) const { return min_distance_from(p) < 0.0f; }
+
+    // fn batch_min_distance_from(&self, &[Point3<f32>]) -> Vec<f32> (:89)
+    std::vector<float> batch_min_distance_from(const std::vector<Point3>& points) const {
+        std::vector<float> out(points.size());
+        check(cth_min_distance_batch(ctx_->get(), &desc_, reinterpret_cast<const cth_point3*>(points.data()),
+                                     points.size(), out.data()));
+        return out;
+    }
+
+    // fn batch_max_distance_from(&self, &[Point3<f32>]) -> Vec<f32> (:96)
+    // Throws (CTH_E_UNSUPPORTED) where the reference panics: a shape with no
+    // upper bound.
+    std::vector<float> batch_max_distance_from(const std::vector<Point3>& points) const {
+        std::vector<float> out(points.size());
+        check(cth_max_distance_batch(ctx_->get(), &desc_, reinterpret_cast<const cth_point3*>(points.data()),
+                                     points.size(), out.data()));
+        return out;
+    }
+
+    // fn batch_bounded_distance_from(&self, ...) -> Vec<(f32, f32)>  (:103)
+    std::vector<std::pair<float, float>> batch_bounded_distance_from(const std::vector<Point3>& points) const {
+        std::vector<float> flat(2 * points.size());
+        check(cth_bounded_distance_batch(ctx_->get(), &desc_, reinterpret_cast<const cth_point3*>(points.data()),
+                                         points.size(), flat.data()));
+        std::vector<std::pair<float, float>> out(points.size());
+        for (size_t i = 0; i < points.size(); ++i) out[i] = {flat[2 * i], flat[2 * i + 1]};
+        return out;
+    }
+
+    // Batched `contains` (one byte per point on the device, here a bool).
+    std::vector<bool> batch_contains(const std::vector<Point3>& points) const {
+        std::vector<uint8_t> flags(points.size());
+        check(cth_contains_batch(ctx_->get(), &desc_, reinterpret_cast<const cth_point3*>(points.data()),
+                                 points.size(), flags.data()));
+        return std::vector<bool>(flags.begin(), flags.end());
+    }
+
+    const cth_shape& desc() const { return desc_; }
+    Context& context() const { return *ctx_; }
+
+  protected:
+    Shape(Context& ctx, const cth_shape& desc) : ctx_(&ctx), desc_(desc) {}
+
+    Context* ctx_;
+    cth_shape desc_;
+};
+
+namespace detail {
+inline cth_shape mandel_desc(int kind, float power, uint64_t max_iters, float bailout) {
+    // assert!(max_iters >= 1)  (src/shape/mandelbulb.rs:20)
+    if (max_iters < 1) throw Error(CTH_E_INVALID, "Mandelbulb: max_iters must be >= 1");
+    if (max_iters > UINT32_MAX) throw Error(CTH_E_INVALID, "Mandelbulb: max_iters above 2^32 - 1");
+    cth_shape d{};
+    d.kind = kind;
+    d.power = power;
+    d.max_iters = (uint32_t)max_iters;
+    d.bailout = bailout;
+    return d;
+}
+}  // namespace detail
+
+// `Mandelbulb<const P: u8>` (src/shape/mandelbulb.rs:12-33): P == 8 runs the
+// reference's closed-form polynomial, any other P its trigonometric map.
+template <uint8_t P>
+class Mandelbulb : public Shape {
+  public:
+    // Mandelbulb::new(max_iters, bailout)  (:19-26)
+    Mandelbulb(Context& ctx, uint64_t max_iters, float bailout)
+        : Shape(ctx, detail::mandel_desc(CTH_SHAPE_MANDELBULB, (float)P, max_iters, bailout)) {}
+
+    // Mandelbulb::<8>::classic(max_iters, bailout)  (:28-33)
+    static Mandelbulb classic(Context& ctx, uint64_t max_iters, float bailout) {
+        static_assert(P == 8, "classic() exists for the power-8 Mandelbulb only");
+        return Mandelbulb(ctx, max_iters, bailout);
+    }
+};
+
+// The trigonometric Mandelbulb with a real power (powf): the shape-parameter
+// extension of `rotate_inner_px_generic` (src/shape/mandelbulb.rs:128-146).
+class MandelbulbTrig : public Shape {
+  public:
+    MandelbulbTrig(Context& ctx, uint64_t max_iters, float bailout, float power)
+        : Shape(ctx, detail::mandel_desc(CTH_SHAPE_MANDELBULB_TRIG, power, max_iters, bailout)) {}
+};
+
+// `Sphere` (src/shape/sphere.rs:6-52).
+class Sphere : public Shape {
+  public:
+    Sphere(Context& ctx, Point3 center, float radius) : Shape(ctx, sphere_desc(center, radius)) {}
+
+    // fn max_distance_from(&self, p) -> Option<f32> { Some(min) }  (:37-39)
+    std::optional<float> max_distance_from(Point3 p) const override { return min_distance_from(p); }
+
+    // fn contains: |c - p|^2 <= r^2  (:23-26)
+    bool contains(Point3 p) const override { return batch_contains(std::vector<Point3>{p})[0]; }
+
+  private:
+    static cth_shape sphere_desc(Point3 c, float r) {
+        cth_shape d{};
+        d.kind = CTH_SHAPE_SPHERE;
+        d.max_iters = 1;
+        d.center[0] = c.x;
+        d.center[1] = c.y;
+        d.center[2] = c.z;
+        d.radius = r;
+        return d;
+    }
+};
+
+// `Vertex` (src/mesh/mod.rs:255-261).
+struct Vertex {
+    float position[3];
+    float normal[3];
+    float distance_from_surface;
+};
+static_assert(sizeof(Vertex) == sizeof(cth_vertex), "Vertex must be 28 bytes");
+
+// The counts of `Timings` (src/mesh/buffer.rs:375-381).
+struct Timings {
+    uint32_t vertices;
+    uint32_t faces;
+};
+
+// `MeshBuffer` (src/mesh/buffer.rs:23-27).
+struct MeshBuffer {
+    std::vector<Vertex> vertices;
+    std::vector<uint32_t> indices;
+
+    // MeshBuffer::generate_for_box(span, shape, resolution)  (:30-42):
+    // naive surface nets on the device.  `normals` selects the reference's
+    // 6-tap difference (CTH_NORMALS_FD6) or the forward-mode gradient
+    // (CTH_NORMALS_TANGENT).  Throws where the reference asserts.
+    static std::pair<MeshBuffer, Timings> generate_for_box(const Span& span, const Shape& shape, uint32_t resolution,
+                                                           int normals = CTH_NORMALS_FD6) {
+        cth_span s;
+        s.start[0] = span.start.x;
+        s.start[1] = span.start.y;
+        s.start[2] = span.start.z;
+        s.end[0] = span.end.x;
+        s.end[1] = span.end.y;
+        s.end[2] = span.end.z;
+        size_t nv = 0, ni = 0;
+        cth_ctx* c = shape.context().get();
+        check(cth_mesh_span(c, &shape.desc(), &s, resolution, normals, nullptr, 0, nullptr, 0, &nv, &ni));
+        MeshBuffer buf;
+        buf.vertices.resize(nv);
+        buf.indices.resize(ni);
+        if (nv || ni)
+            check(cth_mesh_span(c, &shape.desc(), &s, resolution, normals,
+                                reinterpret_cast<cth_vertex*>(buf.vertices.data()), nv, buf.indices.data(), ni, &nv,
+                                &ni));
+        return {std::move(buf), Timings{(uint32_t)nv, (uint32_t)(ni / 6)}};
+    }
+};
+
+}  // namespace cantucci

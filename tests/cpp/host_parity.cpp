@@ -1,0 +1,206 @@
+// Parity of the C++ host mirror (cantucci_amd/host/cantucci.hpp) against the
+// oracle, through the trait methods a caller of the reference uses:
+// Shape::{min,max,bounded}_distance_from and their batch forms, contains,
+// bounding_box, Mandelbulb::new/classic asserts, and MeshBuffer::generate_for_box.
+// Needs a gfx950 device.  Prints one line per check; exits non-zero on the
+// first failure.
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "cantucci.hpp"
+
+// The oracle's C exports (test infrastructure; oracle/oracle.cpp).  The
+// shape descriptor has cth_shape's layout.
+extern "C" {
+void orc_de_points_f32(const cth_shape* shape, const float* pts, const float* powers, size_t n, float* d,
+                       uint32_t* iters, float* r_out, float* dr_out, int threads);
+void orc_surface_nets(const cth_shape* shape, const float* span, uint32_t res, float* dists_out, float* verts_out,
+                      size_t verts_cap, uint32_t* idx_out, size_t idx_cap, size_t* nverts, size_t* nidx);
+}
+
+namespace {
+
+int failures = 0;
+
+void expect(bool ok, const char* what) {
+    std::printf("%s %s\n", ok ? "ok  " : "FAIL", what);
+    std::fflush(stdout);
+    if (!ok) ++failures;
+}
+
+uint32_t bits(float f) {
+    uint32_t b;
+    std::memcpy(&b, &f, 4);
+    return b;
+}
+
+int64_t ulp_distance(float a, float b) {
+    auto key = [](float f) {
+        int32_t i;
+        std::memcpy(&i, &f, 4);
+        return i < 0 ? (int64_t)INT32_MIN - i : (int64_t)i;
+    };
+    int64_t d = key(a) - key(b);
+    return d < 0 ? -d : d;
+}
+
+std::vector<float> oracle_de(const cantucci::Shape& s, const std::vector<cantucci::Point3>& pts) {
+    std::vector<float> d(pts.size());
+    orc_de_points_f32(&s.desc(), reinterpret_cast<const float*>(pts.data()), nullptr, pts.size(), d.data(), nullptr,
+                      nullptr, nullptr, 4);
+    return d;
+}
+
+template <class F>
+bool throws_status(int status, F&& f) {
+    try {
+        f();
+    } catch (const cantucci::Error& e) {
+        return e.status == status;
+    }
+    return false;
+}
+
+}  // namespace
+
+int run() {
+    using namespace cantucci;
+    Context ctx(0);
+
+    // BENCH_POINTS-like near-surface points plus a seeded cloud.
+    std::mt19937 rng(7);
+    std::uniform_real_distribution<float> U(-1.5f, 1.5f);
+    std::vector<Point3> pts(20000);
+    for (auto& p : pts) p = {U(rng), U(rng), U(rng)};
+    pts.push_back({0.0f, 0.0f, 0.0f});   // origin: NaN with the x86 sign
+    pts.push_back({0.0f, -0.0f, 0.5f});  // z axis
+
+    // Mandelbulb<8>::classic(15, 2.0): the P8 polynomial
+    auto m8 = Mandelbulb<8>::classic(ctx, 15, 2.0f);
+    {
+        auto d = m8.batch_min_distance_from(pts);
+        auto o = oracle_de(m8, pts);
+        size_t same = 0;
+        int64_t worst = 0;
+        bool nan_ok = true;
+        for (size_t i = 0; i < pts.size(); ++i) {
+            if (std::isnan(o[i])) {
+                nan_ok &= std::isnan(d[i]) && (bits(d[i]) >> 31) == (bits(o[i]) >> 31);
+                ++same;
+                continue;
+            }
+            same += bits(d[i]) == bits(o[i]);
+            worst = std::max(worst, ulp_distance(d[i], o[i]));
+        }
+        std::printf("P8: %zu of %zu bit-identical, worst %lld ulp\n", same, pts.size(), (long long)worst);
+        expect(same >= pts.size() * 99 / 100 && worst <= 4 && nan_ok, "Mandelbulb<8> batch_min_distance_from");
+        expect(bits(m8.min_distance_from(pts[5])) == bits(d[5]), "min_distance_from == batch element");
+        auto c = m8.batch_contains(pts);
+        bool cok = true;
+        for (size_t i = 0; i < pts.size(); ++i) cok &= c[i] == (d[i] < 0.0f);
+        expect(cok && m8.contains(pts[5]) == (d[5] < 0.0f), "contains == (d < 0)");
+        auto bb = m8.bounding_box();
+        expect(bb.start.x == -1.2f && bb.end.z == 1.2f, "bounding_box [-1.2, 1.2]^3");
+        expect(!m8.max_distance_from(pts[0]).has_value(), "max_distance_from is None");
+        expect(!m8.bounded_distance_from(pts[0]).second.has_value(), "bounded_distance_from upper is None");
+        expect(throws_status(CTH_E_UNSUPPORTED, [&] { m8.batch_max_distance_from(pts); }),
+               "batch_max_distance_from refuses (the reference panics)");
+        expect(throws_status(CTH_E_UNSUPPORTED, [&] { m8.batch_bounded_distance_from(pts); }),
+               "batch_bounded_distance_from refuses (the reference panics)");
+        expect(throws_status(CTH_E_INVALID, [&] { Mandelbulb<8>::classic(ctx, 0, 2.0f); }),
+               "Mandelbulb::new asserts max_iters >= 1");
+    }
+
+    // Mandelbulb<3>: the trigonometric map at an integer power
+    {
+        Mandelbulb<3> m3(ctx, 10, 2.0f);
+        auto d = m3.batch_min_distance_from(pts);
+        auto o = oracle_de(m3, pts);
+        size_t close = 0, n = 0;
+        for (size_t i = 0; i < pts.size(); ++i) {
+            if (!std::isfinite(o[i])) continue;
+            ++n;
+            double rel = std::fabs((double)d[i] - o[i]) / std::max(std::fabs((double)o[i]), 1e-30);
+            close += rel <= 1e-5;
+        }
+        std::printf("P3 trig: %zu of %zu within 1e-5 relative\n", close, n);
+        expect(close >= n * 97 / 100, "Mandelbulb<3> batch_min_distance_from");
+    }
+
+    // Sphere: closed form, upper bound equals the lower one
+    {
+        Sphere sp(ctx, {0.1f, -0.2f, 0.3f}, 0.7f);
+        auto d = sp.batch_min_distance_from(pts);
+        auto o = oracle_de(sp, pts);
+        bool same = true;
+        for (size_t i = 0; i < pts.size(); ++i) same &= bits(d[i]) == bits(o[i]);
+        expect(same, "Sphere batch_min_distance_from bit-identical");
+        auto mx = sp.batch_max_distance_from(pts);
+        auto bd = sp.batch_bounded_distance_from(pts);
+        bool ok = true;
+        for (size_t i = 0; i < pts.size(); ++i)
+            ok &= bits(mx[i]) == bits(d[i]) && bits(bd[i].first) == bits(d[i]) && bits(bd[i].second) == bits(d[i]);
+        expect(ok && sp.max_distance_from(pts[3]).value() == d[3], "Sphere max/bounded distance");
+        auto bb = sp.bounding_box();
+        expect(bb.start.x == 0.1f + -0.7f && bb.end.y == -0.2f + 0.7f, "Sphere bounding_box");
+        auto c = sp.batch_contains(pts);
+        bool cok = true;
+        for (size_t i = 0; i < pts.size(); ++i) {
+            float dx = 0.1f - pts[i].x, dy = -0.2f - pts[i].y, dz = 0.3f - pts[i].z;
+            cok &= c[i] == (((dx * dx + dy * dy) + dz * dz) <= 0.7f * 0.7f);
+        }
+        expect(cok, "Sphere contains by squared norm");
+    }
+
+    // MeshBuffer::generate_for_box at the reference's resolution (64)
+    {
+        Sphere sp(ctx, {0.05f, -0.02f, 0.01f}, 0.55f);
+        Span box{{-1.0f, -1.0f, -1.0f}, {1.0f, 1.0f, 1.0f}};
+        auto [buf, tm] = MeshBuffer::generate_for_box(box, sp, 64);
+        const float sp6[6] = {-1.0f, -1.0f, -1.0f, 1.0f, 1.0f, 1.0f};
+        size_t nv = 0, ni = 0;
+        std::vector<float> dists(65 * 65 * 65);
+        orc_surface_nets(&sp.desc(), sp6, 64, dists.data(), nullptr, 0, nullptr, 0, &nv, &ni);
+        std::vector<float> v(7 * nv);
+        std::vector<uint32_t> ix(ni);
+        orc_surface_nets(&sp.desc(), sp6, 64, dists.data(), v.data(), nv, ix.data(), ni, &nv, &ni);
+        bool same = buf.vertices.size() == nv && buf.indices == ix && tm.faces * 6 == ni;
+        if (same) same = std::memcmp(buf.vertices.data(), v.data(), 28 * nv) == 0;
+        std::printf("sphere mesh: %zu vertices, %zu indices\n", nv, ni);
+        expect(same && nv > 1000, "MeshBuffer::generate_for_box (sphere) bit-identical");
+
+        auto m = Mandelbulb<8>::classic(ctx, 6, 2.5f);
+        Span b2{{-0.6f, -0.6f, 0.7f}, {-0.3f, -0.3f, 1.0f}};
+        auto [mb, mt] = MeshBuffer::generate_for_box(b2, m, 64);
+        const float m6[6] = {-0.6f, -0.6f, 0.7f, -0.3f, -0.3f, 1.0f};
+        orc_surface_nets(&m.desc(), m6, 64, dists.data(), nullptr, 0, nullptr, 0, &nv, &ni);
+        std::vector<uint32_t> ix2(ni);
+        std::vector<float> v2(7 * nv);
+        orc_surface_nets(&m.desc(), m6, 64, dists.data(), v2.data(), nv, ix2.data(), ni, &nv, &ni);
+        std::printf("mandelbulb mesh: %zu vertices, %zu indices\n", nv, ni);
+        expect(mb.vertices.size() == nv && mb.indices == ix2 && mt.vertices == nv,
+               "MeshBuffer::generate_for_box (Mandelbulb<8>) topology identical");
+        expect(throws_status(CTH_E_INVALID, [&] { MeshBuffer::generate_for_box(b2, m, 12); }),
+               "generate_for_box asserts a power-of-two resolution");
+        Span flat{{0.0f, 0.0f, 0.0f}, {0.0f, 1.0f, 1.0f}};
+        expect(throws_status(CTH_E_INVALID, [&] { MeshBuffer::generate_for_box(flat, m, 8); }),
+               "generate_for_box asserts start < end");
+    }
+
+    std::printf("%s: %d failure(s)\n", failures ? "FAILED" : "PASSED", failures);
+    return failures ? 1 : 0;
+}
+
+int main() {
+    try {
+        return run();
+    } catch (const cantucci::Error& e) {
+        // e.g. no gfx950 device: the library refuses to run (no CPU fallback)
+        std::printf("cantucci error %d: %s\n", e.status, e.what());
+        return 2;
+    }
+}
