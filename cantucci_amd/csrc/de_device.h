@@ -229,33 +229,58 @@ struct TrigExact {
             Pi = (floorf(power) == power && fabsf(power) < 64.0f) ? (int)power : -1;
     }
     __device__ __forceinline__ float mag() const { return sqrt_rn((x * x + y * y) + z * z); }
-    __device__ __forceinline__ void body(float r, const MandelUniform&, bool live) {
+
+    // One loop body's intermediate values, kept so that the tangent
+    // evaluator (tangent.h) differentiates the same quantities without
+    // evaluating the transcendentals a second time.
+    struct Terms {
+        float rp, R;      // r^(P-1), r^P
+        float th, ph;     // theta = acos(z / r), phi = atan2(y, x) (before scaling by P)
+        float st, ct;     // sin(P theta), cos(P theta)
+        float sp, cp;     // sin(P phi), cos(P phi)
+        float c;          // z / r
+        float nx, ny, nz, ndr;
+    };
+
+    __device__ __forceinline__ void eval(float r, Terms& t) const {
         // dr = r.powi(P - 1) * (P as f32) * dr + 1.0   (:73)
-        float rp = (Pi >= 0) ? powi_ladder(r, Pi - 1) : powf(r, P - 1.0f);
-        float ndr = ((rp * P) * dr) + 1.0f;
+        t.rp = (Pi >= 0) ? powi_ladder(r, Pi - 1) : powf(r, P - 1.0f);
+        t.ndr = ((t.rp * P) * dr) + 1.0f;
         // rotate_inner_px_generic (:128-146); on the z-axis rotate_on_z_axis
         // (:114-126) keeps only (0, 0, R cos(P theta)), the same z product.
-        float R = (Pi >= 0) ? powi_ladder(r, Pi) : powf(r, P);
-        float c = div_rn(z, r);
-        float th = acosf(c);
-        float ph = atan2f(y, x);
-        th = th * P;
-        ph = ph * P;
-        float st = sinf(th), ct = cosf(th), sp = sinf(ph), cp = cosf(ph);
-        float nx = (st * cp) * R;
-        float ny = (sp * st) * R;
-        float nz = ct * R;
+        t.R = (Pi >= 0) ? powi_ladder(r, Pi) : powf(r, P);
+        t.c = div_rn(z, r);
+        t.th = acosf(t.c);
+        t.ph = atan2f(y, x);
+        const float th = t.th * P;
+        const float ph = t.ph * P;
+        t.st = sinf(th);
+        t.ct = cosf(th);
+        t.sp = sinf(ph);
+        t.cp = cosf(ph);
+        t.nx = (t.st * t.cp) * t.R;
+        t.ny = (t.sp * t.st) * t.R;
+        t.nz = t.ct * t.R;
         if (on_z_axis(x, y)) {
-            nx = 0.0f;
-            ny = 0.0f;
+            t.nx = 0.0f;
+            t.ny = 0.0f;
         }
+    }
+
+    __device__ __forceinline__ void commit(const Terms& t, bool live) {
         if (live) {
-            if (__builtin_isinf(c)) pos_nan = true;
-            dr = ndr;
-            x = nx + px;
-            y = ny + py;
-            z = nz + pz;
+            if (__builtin_isinf(t.c)) pos_nan = true;
+            dr = t.ndr;
+            x = t.nx + px;
+            y = t.ny + py;
+            z = t.nz + pz;
         }
+    }
+
+    __device__ __forceinline__ void body(float r, const MandelUniform&, bool live) {
+        Terms t;
+        eval(r, t);
+        commit(t, live);
     }
     __device__ __forceinline__ float finish(float r) const {
         return finish_exact(r, dr, pos_nan);

@@ -152,19 +152,17 @@ struct TrigTangent : TrigExact {
         }
     }
 
-    __device__ __forceinline__ void body(float r, const MandelUniform& u, bool live) {
+    __device__ __forceinline__ void body(float r, const MandelUniform&, bool live) {
         const float zx = x, zy = y, zz = z;
+        // The forward body's own terms: R = r^P, r^(P-1), theta, phi and the
+        // sines/cosines of P theta, P phi.
+        Terms t;
+        TrigExact::eval(r, t);
         float ir = 1.0f / r;
         float rho2 = zx * zx + zy * zy;
         float rho = __builtin_sqrtf(rho2);
-        float lnr = __logf(r);
-        float th = acosf(zz * ir);
-        float ph = atan2f(zy, zx);
-        float sT, cT, sP, cP;
-        __sincosf(P * th, &sT, &cT);
-        __sincosf(P * ph, &sP, &cP);
-        float R = __expf(P * lnr);
-        float Rm1 = R * ir;
+        const float R = t.R, Rm1 = t.rp;
+        const float sT = t.st, cT = t.ct, sP = t.sp, cP = t.cp;
         // gradients of r, theta, phi with respect to z
         float gr[3] = {zx * ir, zy * ir, zz * ir};
         float kth = zz * ir * ir / rho;
@@ -190,13 +188,14 @@ struct TrigTangent : TrigExact {
         }
         if (dP) {
             // explicit power dependence of F and of the dr update
+            const float lnr = __logf(r);
 #pragma unroll
-            for (int i = 0; i < 3; ++i) NT[i][3] += R * (lnr * u0[i] + th * uT[i] + ph * uF[i]);
+            for (int i = 0; i < 3; ++i) NT[i][3] += R * (lnr * u0[i] + t.th * uT[i] + t.ph * uF[i]);
             ntdr[3] += Rm1 * dr * (1.0f + P * lnr);
         }
 #pragma unroll
         for (int i = 0; i < 3; ++i) NT[i][i] += 1.0f;  // z_{k+1} = F + p
-        TrigExact::body(r, u, live);
+        TrigExact::commit(t, live);
         if (live) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
