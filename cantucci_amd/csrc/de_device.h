@@ -47,6 +47,27 @@ __device__ __forceinline__ bool on_z_axis(float x, float y) {
 __device__ __forceinline__ float sqrt_rn(float x) { return __builtin_sqrtf(x); }
 __device__ __forceinline__ float div_rn(float a, float b) { return a / b; }
 
+// The same correctly rounded square root without the small-argument scaling
+// whenever no active lane needs it.  For x >= 2^-96 (and +inf) the compiler's
+// expansion of __builtin_sqrtf is exactly this: the hardware estimate s
+// (within an ulp), then two FMA residual tests choose s - ulp, s or s + ulp;
+// below 2^-96 it first scales x by 2^32 (and the result by 2^-16), and it
+// passes 0 and inf through.  NaN, zero, negative and tiny arguments take the
+// full sequence.
+__device__ __forceinline__ float sqrt_rn_wave(float x) {
+    if (__builtin_expect(__ballot(!(x >= 0x1p-96f)) == 0, 1)) {
+        const float s = __builtin_amdgcn_sqrtf(x);
+        const float sd = __uint_as_float(__float_as_uint(s) - 1u);
+        const float su = __uint_as_float(__float_as_uint(s) + 1u);
+        const float rd = __builtin_fmaf(-sd, s, x);
+        const float ru = __builtin_fmaf(-su, s, x);
+        float r = (rd <= 0.0f) ? sd : s;
+        r = (ru > 0.0f) ? su : r;
+        return r;
+    }
+    return __builtin_sqrtf(x);
+}
+
 // NaN sign of the reference.  x86 SSE arithmetic creates the default NaN
 // 0xFFC00000 (sign set) and propagates NaN operands unchanged; glibc's acosf
 // returns +NaN (0x7FC00000) for arguments outside [-1, 1].  In the distance
@@ -157,7 +178,7 @@ struct P8Exact {
         y2 = y * y;
         w2 = x2 + y2;
         z2 = z * z;
-        return sqrt_rn(w2 + z2);
+        return sqrt_rn_wave(w2 + z2);
     }
     // One loop body for every lane of the wave, committed only where `live`
     // (lanes without a live point compute on stale state and discard it).
@@ -177,7 +198,7 @@ struct P8Exact {
         float a = 1.0f + div_rn(num, w8);
         float nx = a * ((((x8 - (28.0f * x6) * y2) + (70.0f * x4) * y4) - (28.0f * x2) * y6) - y8);
         float ny = (((8.0f * a) * x) * y) * (((x6 - (7.0f * x4) * y2) + (7.0f * x2) * y4) - y6);
-        float nz = (((8.0f * z) * sqrt_rn(w2)) * (z2 - w2)) * ((z4 - (6.0f * z2) * w2) + w4);
+        float nz = (((8.0f * z) * sqrt_rn_wave(w2)) * (z2 - w2)) * ((z4 - (6.0f * z2) * w2) + w4);
         bool axis = live && on_z_axis(x, y);
         if (__ballot(axis)) {
             if (axis) {
@@ -228,7 +249,7 @@ struct TrigExact {
         else
             Pi = (floorf(power) == power && fabsf(power) < 64.0f) ? (int)power : -1;
     }
-    __device__ __forceinline__ float mag() const { return sqrt_rn((x * x + y * y) + z * z); }
+    __device__ __forceinline__ float mag() const { return sqrt_rn_wave((x * x + y * y) + z * z); }
 
     // One loop body's intermediate values, kept so that the tangent
     // evaluator (tangent.h) differentiates the same quantities without
