@@ -68,6 +68,7 @@ def main() -> None:
     c2 = wl.C2
     ctx = ca.Context(0, args.mode)
     tuned = autotune.apply(ctx, "spans")
+    tuned_mesh = autotune.apply(ctx, "mesh")
     shape = ca.Mandelbulb.classic(c2["max_iters"], c2["bailout"], ctx=ctx)
     hd = wl.half_diagonals(3.0, c2["max_depth"])
     res = c2["res"]
@@ -205,6 +206,7 @@ def main() -> None:
             "phases_ms": phases,
             "level_counts": [int(x) for x in lc],
             "launch": tuned,
+            "launch_mesh": tuned_mesh,
             "parallelism": "single GPU",
         },
         "roofline": roofline,

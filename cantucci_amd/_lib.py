@@ -64,6 +64,7 @@ CTH_KERNEL_TRACE = 1
 CTH_KERNEL_ADJOINT = 2
 CTH_KERNEL_SPANS = 3
 CTH_KERNEL_JOBS = 4
+CTH_KERNEL_MESH = 5
 
 
 class CthJob(C.Structure):

@@ -26,7 +26,7 @@ SPACE = {
     "block": [256],
     "blocks_per_cu": [4, 8, 16],
     "chunk": [64, 256, 1024],
-    "refill_min": [16, 32, 48],
+    "refill_min": [16, 32, 48, 56, 62],
     "dynamic": [0, 1],
 }
 
@@ -37,6 +37,7 @@ KERNEL_OF = {
     "trace": _lib.CTH_KERNEL_TRACE,
     "tangent": _lib.CTH_KERNEL_ADJOINT,
     "jobs": _lib.CTH_KERNEL_JOBS,
+    "mesh": _lib.CTH_KERNEL_MESH,
 }
 
 
@@ -107,7 +108,8 @@ def apply(ctx, workload: str, path: Path = CACHE) -> dict | None:
 
 
 def _workloads(ctx):
-    """Representative runs of each workload (sized to tune in seconds)."""
+    """The benchmark workloads at their full sizes (each runs in milliseconds,
+    so a whole sweep takes seconds)."""
     import torch
 
     from . import device as dev
@@ -116,16 +118,18 @@ def _workloads(ctx):
 
     runs = {}
     m8 = Mandelbulb.classic(15, 2.0, ctx=ctx)
-    runs["grid"] = lambda: dev.de_grid(m8, (-1.5,) * 3, (1.5,) * 3, 128, 127.0)
-    hd = wl.half_diagonals(3.0, 7)
-    leaves, _ = dev.refine(m8, wl.C2["root"], 2, 7, hd)
+    runs["grid"] = lambda: dev.de_grid(m8, (-1.5,) * 3, (1.5,) * 3, 64, 63.0)
+    hd = wl.half_diagonals(3.0, 8)
+    leaves, _ = dev.refine(m8, wl.C2["root"], 2, 8, hd)
     out = torch.empty((leaves.shape[0], 729), device="cuda")
     runs["spans"] = lambda: dev.sample_spans(m8, leaves, 8, with_cross=False, out=out)
-    cams = torch.from_numpy(wl.orbit_poses(32, 0, 480, 270)).cuda()
-    depth = torch.empty((32, 270, 480), device="cuda")
-    steps = torch.empty((32, 270, 480), dtype=torch.int32, device="cuda")
-    runs["trace"] = lambda: dev.render_depth(m8, cams, 480, 270, depth=depth, steps=steps)
-    pts, powers = wl.shell_points(1 << 21, seed=1)
+    dev.sample_spans(m8, leaves, 8, with_cross=False, out=out)
+    runs["mesh"] = lambda: dev.mesh(m8, leaves, out, 8, "fd6")
+    cams = torch.from_numpy(wl.orbit_poses(32, 0, 1920, 1080)).cuda()
+    depth = torch.empty((32, 1080, 1920), device="cuda")
+    steps = torch.empty((32, 1080, 1920), dtype=torch.int32, device="cuda")
+    runs["trace"] = lambda: dev.render_depth(m8, cams, 1920, 1080, depth=depth, steps=steps)
+    pts, powers = wl.shell_points(1 << 22, seed=1)
     P = torch.from_numpy(pts).cuda()
     W = torch.from_numpy(powers).cuda()
     tr = MandelbulbTrig(15, 2.0, 8.0, ctx=ctx)
@@ -133,7 +137,7 @@ def _workloads(ctx):
     dd = torch.empty(P.shape[0], device="cuda")
     runs["tangent"] = lambda: dev.normals(tr, P, W, out=nrm, d=dd)
     runs["fd"] = lambda: dev.normals_fd(tr, P, 1e-3, W, out=nrm, d=dd)
-    N, Pw, cap = wl.mixed_jobs(16, seed=5)
+    N, Pw, cap = wl.mixed_jobs(64, seed=5)
     js = dev.JobSet(ctx, _lib.CTH_SHAPE_MANDELBULB_TRIG, 2.0,
                     [dict(n=int(n), power=float(p), max_iters=int(c)) for n, p, c in zip(N, Pw, cap)])
     jd = torch.empty(js.total, device="cuda")
@@ -148,7 +152,7 @@ def _workloads(ctx):
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=str(CACHE))
-    ap.add_argument("--workloads", default="grid,spans,trace,tangent,fd,jobs")
+    ap.add_argument("--workloads", default="grid,spans,mesh,trace,tangent,fd,jobs")
     ap.add_argument("--mode", default="parity")
     args = ap.parse_args()
     from .shape import Context

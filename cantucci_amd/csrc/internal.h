@@ -42,6 +42,7 @@ struct Ctx {
     LaunchShape shape_trace{256, 16, 256, 32, 0};
     LaunchShape shape_adj{256, 16, 256, 32, 0};
     LaunchShape shape_jobs{256, 8, 256, 32, 0};
+    LaunchShape shape_mesh{256, 8, 1024, 32, 1};
     LaunchShape* launch_shape(int kernel) {
         switch (kernel) {
             case CTH_KERNEL_DE: return &shape_de;
@@ -49,6 +50,7 @@ struct Ctx {
             case CTH_KERNEL_ADJOINT: return &shape_adj;
             case CTH_KERNEL_SPANS: return &shape_spans;
             case CTH_KERNEL_JOBS: return &shape_jobs;
+            case CTH_KERNEL_MESH: return &shape_mesh;
             default: return nullptr;
         }
     }

@@ -731,15 +731,15 @@ int mesh_emit(Ctx& c, const cth_shape& s, const float* spans, const float* dists
     if (normals == CTH_NORMALS_TANGENT) {
         if (nverts >= (1ull << 31)) return fail(CTH_E_INVALID, "more than 2^31 vertices");
         uint32_t blocks = 0;
-        WorkQueue q = make_queue(c, (uint32_t)nverts, c.shape_adj, blocks);
+        WorkQueue q = make_queue(c, (uint32_t)nverts, c.shape_mesh, blocks);
         SrcMeshVerts src{verts, s.power};
         SinkMeshVerts sink{verts, u.max_iters};
         if (family == FAM_P8)
             hipLaunchKernelGGL((k_refill<P8Tangent, SrcMeshVerts, SinkMeshVerts>), dim3(blocks),
-                               dim3(c.shape_adj.block), 0, st, src, u, sink, q);
+                               dim3(c.shape_mesh.block), 0, st, src, u, sink, q);
         else
             hipLaunchKernelGGL((k_refill<TrigTangent, SrcMeshVerts, SinkMeshVerts>), dim3(blocks),
-                               dim3(c.shape_adj.block), 0, st, src, u, sink, q);
+                               dim3(c.shape_mesh.block), 0, st, src, u, sink, q);
         return check_hip(hipGetLastError(), "mesh tangent normals launch");
     }
 
@@ -758,7 +758,7 @@ int mesh_emit(Ctx& c, const cth_shape& s, const float* spans, const float* dists
                                nb * 7, s.center[0], s.center[1], s.center[2], s.radius, taps, nullptr);
             rc = check_hip(hipGetLastError(), "mesh taps launch");
         } else {
-            rc = run_refill(c, family, u, src, sink, nb * 7, st, c.shape_de);
+            rc = run_refill(c, family, u, src, sink, nb * 7, st, c.shape_mesh);
         }
         if (rc) break;
         hipLaunchKernelGGL(k_mesh_fd_combine, dim3(cdiv_u32(nb, 256)), dim3(256), 0, st, taps, nb, vb);

@@ -30,6 +30,7 @@ def main() -> None:
     m8 = ca.Mandelbulb.classic(15, 2.0, ctx=ctx)
     if "spans" in legs:
         autotune.apply(ctx, "spans")
+        autotune.apply(ctx, "mesh")
         c2 = wl.C2
         leaves, _ = dev.refine(m8, c2["root"], c2["min_depth"], c2["max_depth"], wl.half_diagonals(3.0, 8))
         out = torch.empty((leaves.shape[0], 729), device="cuda")

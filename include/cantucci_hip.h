@@ -134,7 +134,17 @@ typedef struct cth_launch {
     uint32_t dynamic;
 } cth_launch;
 
-enum { CTH_KERNEL_DE = 0, CTH_KERNEL_TRACE = 1, CTH_KERNEL_ADJOINT = 2, CTH_KERNEL_SPANS = 3, CTH_KERNEL_JOBS = 4 };
+/* Kernel families: point/grid distances, sphere tracing, gradients (VJP,
+ * tangent, normals), octree span sampling, batched jobs, and the mesh
+ * vertex evaluations (6-tap stencils or tangent normals). */
+enum {
+    CTH_KERNEL_DE = 0,
+    CTH_KERNEL_TRACE = 1,
+    CTH_KERNEL_ADJOINT = 2,
+    CTH_KERNEL_SPANS = 3,
+    CTH_KERNEL_JOBS = 4,
+    CTH_KERNEL_MESH = 5
+};
 
 int cth_ctx_set_launch(cth_ctx* ctx, int kernel, const cth_launch* launch);
 int cth_ctx_get_launch(const cth_ctx* ctx, int kernel, cth_launch* launch);
