@@ -15,7 +15,7 @@ import sys
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--legs", default="spans,mesh,trace,tangent,fd,jobs,copy")
-    ap.add_argument("--reps", type=int, default=2)
+    ap.add_argument("--reps", type=int, default=5)
     args = ap.parse_args()
     legs = set(args.legs.split(","))
 
