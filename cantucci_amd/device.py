@@ -204,7 +204,8 @@ def de_grad(shape: Shape, pts: torch.Tensor, powers: torch.Tensor | None = None,
 
 def normals(shape: Shape, pts: torch.Tensor, powers: torch.Tensor | None = None, ctx: Context | None = None,
             out: torch.Tensor | None = None, d: torch.Tensor | None = None):
-    """Unit normals (n, 3) from the fused forward + adjoint; returns (d, normals)."""
+    """Unit normals (n, 3) from one fused launch (parity forward + tangent
+    Jacobian); returns (d, normals)."""
     ctx = ctx or shape.ctx
     pts = _f32(pts).contiguous()
     n = pts.shape[0]
