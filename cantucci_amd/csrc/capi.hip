@@ -103,6 +103,7 @@ int cth_ctx_create(int device, cth_ctx** out) {
     if (rc == CTH_OK) rc = check_hip(hipMalloc((void**)&c->counters, 2 * COUNTER_RING * sizeof(uint32_t)), "hipMalloc");
     if (rc == CTH_OK)
         rc = check_hip(hipMemset(c->counters, 0, 2 * COUNTER_RING * sizeof(uint32_t)), "hipMemset counters");
+    if (rc == CTH_OK) rc = check_hip(hipHostMalloc((void**)&c->pinned, 8 * sizeof(uint64_t)), "hipHostMalloc");
     if (rc == CTH_OK) rc = check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
     if (rc != CTH_OK) {
         cth_ctx_destroy(c);
@@ -117,6 +118,9 @@ void cth_ctx_destroy(cth_ctx* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->counters) (void)hipFree(c->counters);
+    for (void* p : c->ws)
+        if (p) (void)hipFree(p);
+    if (c->pinned) (void)hipHostFree(c->pinned);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }

@@ -25,6 +25,11 @@ struct LaunchShape {
     uint32_t dynamic = 0;      // 0: static interleaved chunks, 1: atomic claims
 };
 
+// Device scratch slots of a context, grown on demand and kept until the
+// context is destroyed (calls on one context never overlap, and the users
+// below finish with a stream synchronisation).
+enum ScratchSlot { WS_NODES_A = 0, WS_NODES_B, WS_FLAGS, WS_OFFS, WS_SCAN, WS_COUNTS, WS_SLOTS };
+
 struct Ctx {
     int device = 0;
     int num_cus = 0;
@@ -32,6 +37,29 @@ struct Ctx {
     hipStream_t stream = nullptr;
     uint32_t* counters = nullptr;  // 2 * COUNTER_RING
     uint32_t ring_pos = 0;
+    void* ws[WS_SLOTS] = {};
+    size_t ws_bytes[WS_SLOTS] = {};
+    uint64_t* pinned = nullptr;  // 8 host-pinned words for count read-backs
+    // Slot `s` with at least `bytes` bytes (a larger buffer replaces the old
+    // one after a device-wide wait, so no launch can still be using it).
+    int scratch(int s, size_t bytes, void** out) {
+        if (bytes > ws_bytes[s]) {
+            if (ws[s]) {
+                (void)hipDeviceSynchronize();
+                (void)hipFree(ws[s]);
+                ws[s] = nullptr;
+                ws_bytes[s] = 0;
+            }
+            size_t want = bytes + bytes / 4 + 256;
+            if (hipMalloc(&ws[s], want) != hipSuccess) {
+                ws[s] = nullptr;
+                return CTH_E_NOMEM;
+            }
+            ws_bytes[s] = want;
+        }
+        *out = ws[s];
+        return CTH_OK;
+    }
     LaunchShape shape_de;
     LaunchShape shape_spans{256, 8, 1024, 32, 1};
     // Next counter pair of the ring.

@@ -660,8 +660,9 @@ int mesh_count(Ctx& c, const float* dists, uint32_t nleaves, uint32_t res, uint6
     uint64_t* counts = nullptr;
     void* tmp = nullptr;
     size_t tmp_bytes = 0;
-    int rc = check_hip(hipMallocAsync((void**)&counts, 2 * cnt * sizeof(uint64_t), st), "hipMallocAsync counts");
-    if (!rc) {
+    int rc = c.scratch(WS_COUNTS, 2 * cnt * sizeof(uint64_t), (void**)&counts);
+    if (rc) return fail(rc, "mesh count scratch allocation failed");
+    {
         if (res <= 32) {
             const uint32_t blocks = wave_grid(c, nleaves);
             switch (res) {
@@ -681,17 +682,19 @@ int mesh_count(Ctx& c, const float* dists, uint32_t nleaves, uint32_t res, uint6
     if (!rc)
         rc = check_hip(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, counts, voff, (int)cnt, st),
                        "scan size");
-    if (!rc) rc = check_hip(hipMallocAsync(&tmp, tmp_bytes ? tmp_bytes : 4, st), "hipMallocAsync scan");
+    if (!rc && c.scratch(WS_SCAN, tmp_bytes, &tmp)) rc = fail(CTH_E_NOMEM, "scan scratch allocation failed");
     if (!rc) rc = check_hip(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, counts, voff, (int)cnt, st), "scan");
     if (!rc)
         rc = check_hip(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, counts + cnt, ioff, (int)cnt, st), "scan");
     if (!rc)
-        rc = check_hip(hipMemcpyAsync(&totals[0], voff + nleaves, 8, hipMemcpyDeviceToHost, st), "D2H totals");
+        rc = check_hip(hipMemcpyAsync(c.pinned, voff + nleaves, 8, hipMemcpyDeviceToHost, st), "D2H totals");
     if (!rc)
-        rc = check_hip(hipMemcpyAsync(&totals[1], ioff + nleaves, 8, hipMemcpyDeviceToHost, st), "D2H totals");
-    if (tmp) (void)hipFreeAsync(tmp, st);
-    if (counts) (void)hipFreeAsync(counts, st);
+        rc = check_hip(hipMemcpyAsync(c.pinned + 1, ioff + nleaves, 8, hipMemcpyDeviceToHost, st), "D2H totals");
     int rs = check_hip(hipStreamSynchronize(st), "mesh count sync");
+    if (!rc && !rs) {
+        totals[0] = c.pinned[0];
+        totals[1] = c.pinned[1];
+    }
     return rc ? rc : rs;
 }
 

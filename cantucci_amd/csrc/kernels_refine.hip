@@ -74,15 +74,16 @@ int launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t st) {
 }
 
 // One thread per (node, child): child c of a split node i lands at
-// 8 * offs[i] + c.  create_spans (src/octree/mod.rs:315-329): centre
+// 8 * (incl[i] - 1) + c, incl being the inclusive prefix sum of the split
+// flags.  create_spans (src/octree/mod.rs:315-329): centre
 // = start + (end - start) / 2; child bit 2 = +x half, bit 1 = +y, bit 0 = +z.
 __global__ void k_emit_children(const float* __restrict__ nodes, const uint32_t* __restrict__ flags,
-                                const uint32_t* __restrict__ offs, uint32_t n, float* __restrict__ children) {
+                                const uint32_t* __restrict__ incl, uint32_t n, float* __restrict__ children) {
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t i = t >> 3, c = t & 7u;
     if (i >= n || !flags[i]) return;
     const float* sp = nodes + 6 * (size_t)i;
-    float* out = children + 6 * ((size_t)offs[i] * 8 + c);
+    float* out = children + 6 * ((size_t)(incl[i] - 1u) * 8 + c);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         float s = sp[a], e = sp[3 + a];
@@ -94,16 +95,21 @@ __global__ void k_emit_children(const float* __restrict__ nodes, const uint32_t*
 }
 
 __global__ void k_emit_leaves(const float* __restrict__ nodes, const uint32_t* __restrict__ flags,
-                              const uint32_t* __restrict__ offs, uint32_t n, float* __restrict__ leaves,
+                              const uint32_t* __restrict__ incl, uint32_t n, float* __restrict__ leaves,
                               uint64_t cap) {
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t i = t / 6, a = t - i * 6;
-    if (i >= n || !flags[i] || offs[i] >= cap) return;
-    leaves[6 * (size_t)offs[i] + a] = nodes[6 * (size_t)i + a];
+    if (i >= n || !flags[i] || incl[i] - 1u >= cap) return;
+    leaves[6 * (size_t)(incl[i] - 1u) + a] = nodes[6 * (size_t)i + a];
 }
 
 static inline uint32_t cdiv32(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
+// Level-synchronous refinement.  Per level: the split flags (forced, sphere
+// closed form or a DE refill launch), an inclusive scan of the flags, one
+// 4-byte read-back of the split count into pinned host memory, and the
+// children.  Node, flag, offset and scan buffers are the context's scratch
+// slots (ping-pong for the nodes), so a level allocates nothing.
 int refine_levels(Ctx& c, const cth_shape& s, const float* root, uint32_t min_depth, uint32_t max_depth,
                   const float* half_diag, float* leaves, size_t cap, size_t* nleaves, uint64_t* level_counts,
                   hipStream_t st) {
@@ -114,20 +120,11 @@ int refine_levels(Ctx& c, const cth_shape& s, const float* root, uint32_t min_de
         if (rc) return rc;
     }
     const uint64_t kMaxNodes = 1ull << 28;
-    // Scratch: two node arrays, flags, offsets, scan temp.  Sized per level.
-    std::vector<void*> owned;
-    auto alloc = [&](size_t bytes, void** p) -> int {
-        int rc = check_hip(hipMallocAsync(p, bytes ? bytes : 4, st), "hipMallocAsync");
-        if (!rc) owned.push_back(*p);
-        return rc;
-    };
-    auto release = [&]() {
-        for (void* p : owned) (void)hipFreeAsync(p, st);
-        owned.clear();
-    };
     float* cur = nullptr;
-    int rc = alloc(6 * sizeof(float), (void**)&cur);
-    if (!rc) rc = check_hip(hipMemcpyAsync(cur, root, 6 * sizeof(float), hipMemcpyHostToDevice, st), "H2D root");
+    int rc = c.scratch(WS_NODES_A, 6 * sizeof(float), (void**)&cur);
+    if (rc) return fail(rc, "refinement scratch allocation failed");
+    rc = check_hip(hipMemcpyAsync(cur, root, 6 * sizeof(float), hipMemcpyHostToDevice, st), "H2D root");
+    int slot = WS_NODES_A;
     uint64_t n = 1;
     *nleaves = 0;
     for (uint32_t depth = 0; depth <= max_depth && rc == CTH_OK; ++depth) {
@@ -137,10 +134,12 @@ int refine_levels(Ctx& c, const cth_shape& s, const float* root, uint32_t min_de
             break;
         }
         uint32_t* flags = nullptr;
-        uint32_t* offs = nullptr;
-        rc = alloc(n * sizeof(uint32_t), (void**)&flags);
-        if (!rc) rc = alloc(n * sizeof(uint32_t), (void**)&offs);
-        if (rc) break;
+        uint32_t* incl = nullptr;
+        if (c.scratch(WS_FLAGS, n * sizeof(uint32_t), (void**)&flags) ||
+            c.scratch(WS_OFFS, n * sizeof(uint32_t), (void**)&incl)) {
+            rc = fail(CTH_E_NOMEM, "refinement scratch allocation failed");
+            break;
+        }
         if (depth < min_depth) {
             hipLaunchKernelGGL(k_fill_u32, dim3(cdiv32(n, 256)), dim3(256), 0, st, flags, (uint32_t)n, 1u);
             rc = check_hip(hipGetLastError(), "fill launch");
@@ -155,45 +154,49 @@ int refine_levels(Ctx& c, const cth_shape& s, const float* root, uint32_t min_de
         }
         if (rc) break;
         size_t temp_bytes = 0;
-        rc = check_hip(hipcub::DeviceScan::ExclusiveSum(nullptr, temp_bytes, flags, offs, (int)n, st), "scan size");
+        rc = check_hip(hipcub::DeviceScan::InclusiveSum(nullptr, temp_bytes, flags, incl, (int)n, st), "scan size");
         void* temp = nullptr;
-        if (!rc) rc = alloc(temp_bytes, &temp);
-        if (!rc) rc = check_hip(hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, flags, offs, (int)n, st), "scan");
-        uint32_t last[2] = {0, 0};
-        if (!rc) rc = check_hip(hipMemcpyAsync(&last[0], offs + (n - 1), 4, hipMemcpyDeviceToHost, st), "D2H");
-        if (!rc) rc = check_hip(hipMemcpyAsync(&last[1], flags + (n - 1), 4, hipMemcpyDeviceToHost, st), "D2H");
+        if (!rc && c.scratch(WS_SCAN, temp_bytes, &temp)) rc = fail(CTH_E_NOMEM, "scan scratch allocation failed");
+        if (!rc) rc = check_hip(hipcub::DeviceScan::InclusiveSum(temp, temp_bytes, flags, incl, (int)n, st), "scan");
+        if (!rc)
+            rc = check_hip(hipMemcpyAsync(c.pinned, incl + (n - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, st),
+                           "D2H split count");
         if (!rc) rc = check_hip(hipStreamSynchronize(st), "sync");
         if (rc) break;
-        uint64_t split = (uint64_t)last[0] + last[1];
+        const uint64_t split = *reinterpret_cast<volatile uint32_t*>(c.pinned);
         if (depth == max_depth) {
             *nleaves = split;
             if (leaves && split) {
-                hipLaunchKernelGGL(k_emit_leaves, dim3(cdiv32(n * 6, 256)), dim3(256), 0, st, cur, flags, offs,
+                hipLaunchKernelGGL(k_emit_leaves, dim3(cdiv32(n * 6, 256)), dim3(256), 0, st, cur, flags, incl,
                                    (uint32_t)n, leaves, (uint64_t)cap);
                 rc = check_hip(hipGetLastError(), "leaves launch");
             }
             break;
         }
-        uint64_t next_n = split * 8;
+        const uint64_t next_n = split * 8;
         if (next_n > kMaxNodes) {
             rc = fail(CTH_E_INVALID, "refinement exceeds 2^28 nodes in one level");
             break;
         }
+        // The next level goes to the other node slot; growing that slot
+        // waits for the device, so `cur` stays valid.
+        const int next_slot = (slot == WS_NODES_A) ? WS_NODES_B : WS_NODES_A;
         float* next = nullptr;
-        rc = alloc(next_n * 6 * sizeof(float) + 4, (void**)&next);
-        if (rc) break;
+        if (c.scratch(next_slot, next_n * 6 * sizeof(float) + 4, (void**)&next)) {
+            rc = fail(CTH_E_NOMEM, "refinement scratch allocation failed");
+            break;
+        }
         if (next_n) {
-            hipLaunchKernelGGL(k_emit_children, dim3(cdiv32(n * 8, 256)), dim3(256), 0, st, cur, flags, offs,
+            hipLaunchKernelGGL(k_emit_children, dim3(cdiv32(n * 8, 256)), dim3(256), 0, st, cur, flags, incl,
                                (uint32_t)n, next);
             rc = check_hip(hipGetLastError(), "children launch");
         }
         cur = next;
+        slot = next_slot;
         n = next_n;
     }
     int rs = check_hip(hipStreamSynchronize(st), "sync");
-    release();
-    int rf = check_hip(hipStreamSynchronize(st), "free sync");
-    return rc ? rc : (rs ? rs : rf);
+    return rc ? rc : rs;
 }
 
 }  // namespace cth
