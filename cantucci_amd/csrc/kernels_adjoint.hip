@@ -460,8 +460,11 @@ int launch_tangent(Ctx& c, const cth_shape& s, const float* pts, const float* po
     if (family == FAM_P8)
         hipLaunchKernelGGL((k_refill<P8Tangent, SrcPoints, SinkGrad>), dim3(blocks), dim3(c.shape_adj.block), 0, st, src,
                            u, sink, q);
-    else
+    else if (grad_power)  // the power column only when its gradient is asked for
         hipLaunchKernelGGL((k_refill<TrigTangent, SrcPoints, SinkGrad>), dim3(blocks), dim3(c.shape_adj.block), 0, st,
+                           src, u, sink, q);
+    else
+        hipLaunchKernelGGL((k_refill<TrigNormal, SrcPoints, SinkGrad>), dim3(blocks), dim3(c.shape_adj.block), 0, st,
                            src, u, sink, q);
     return check_hip(hipGetLastError(), "tangent launch");
 }

@@ -239,7 +239,7 @@ int jobs_run(Ctx& c, const JobSet& js, float* dists, float* normals, unsigned lo
         hipLaunchKernelGGL((k_jobs<P8Tangent>), dim3(blocks), dim3(c.shape_jobs.block), 0, st, js.dev, js.njobs, u,
                            dists, normals, done, start, q);
     else
-        hipLaunchKernelGGL((k_jobs<TrigTangent>), dim3(blocks), dim3(c.shape_jobs.block), 0, st, js.dev, js.njobs, u,
+        hipLaunchKernelGGL((k_jobs<TrigNormal>), dim3(blocks), dim3(c.shape_jobs.block), 0, st, js.dev, js.njobs, u,
                            dists, normals, done, start, q);
     return check_hip(hipGetLastError(), "jobs launch");
 }

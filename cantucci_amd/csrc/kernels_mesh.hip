@@ -741,7 +741,7 @@ int mesh_emit(Ctx& c, const cth_shape& s, const float* spans, const float* dists
             hipLaunchKernelGGL((k_refill<P8Tangent, SrcMeshVerts, SinkMeshVerts>), dim3(blocks),
                                dim3(c.shape_mesh.block), 0, st, src, u, sink, q);
         else
-            hipLaunchKernelGGL((k_refill<TrigTangent, SrcMeshVerts, SinkMeshVerts>), dim3(blocks),
+            hipLaunchKernelGGL((k_refill<TrigNormal, SrcMeshVerts, SinkMeshVerts>), dim3(blocks),
                                dim3(c.shape_mesh.block), 0, st, src, u, sink, q);
         return check_hip(hipGetLastError(), "mesh tangent normals launch");
     }

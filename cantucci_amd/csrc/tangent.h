@@ -130,23 +130,26 @@ struct P8Tangent : P8Exact {
 // Trig map: F = R u(Theta, Phi), R = r^P, Theta = P theta, Phi = P phi,
 // u = (sin Theta cos Phi, sin Theta sin Phi, cos Theta),
 // theta = acos(z / r), phi = atan2(y, x) (src/shape/mandelbulb.rs:128-146).
-// Column 3 of the tangent is d/dP (real-power kind only).
+// NQ = 4 carries the d/dP column as well (real-power kind); NQ = 3 only the
+// point derivatives (normals).
 // ---------------------------------------------------------------------------
-struct TrigTangent : TrigExact {
-    float T[3][4];
-    float tdr[4];
-    float rq_prev[4];
+template <int NQ>
+struct TrigTangentN : TrigExact {
+    static_assert(NQ == 3 || NQ == 4, "3 point columns, optionally the power");
+    float T[3][NQ];
+    float tdr[NQ];
+    float rq_prev[NQ];
     bool dP;  // differentiate with respect to the power
 
     __device__ __forceinline__ void start(float ax, float ay, float az, float pw, const MandelUniform& u) {
         TrigExact::start(ax, ay, az, pw, u);
-        dP = !u.integral;
+        dP = (NQ == 4) && !u.integral;
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) T[i][j] = (i == j) ? 1.0f : 0.0f;
+            for (int j = 0; j < NQ; ++j) T[i][j] = (i == j) ? 1.0f : 0.0f;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < NQ; ++j) {
             tdr[j] = 0.0f;
             rq_prev[j] = 0.0f;
         }
@@ -174,9 +177,9 @@ struct TrigTangent : TrigExact {
         float uF[3] = {-sT * sP, sT * cP, 0.0f};
         float PR = P * R;
         float cr = P * Rm1;
-        float ntdr[4], NT[3][4], rq[4];
+        float ntdr[NQ], NT[3][NQ], rq[NQ];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < NQ; ++q) {
             float rq_ = gr[0] * T[0][q] + gr[1] * T[1][q] + gr[2] * T[2][q];
             float tq = gth[0] * T[0][q] + gth[1] * T[1][q] + gth[2] * T[2][q];
             float pq = gph[0] * T[0][q] + gph[1] * T[1][q];
@@ -186,19 +189,21 @@ struct TrigTangent : TrigExact {
 #pragma unroll
             for (int i = 0; i < 3; ++i) NT[i][q] = cr * u0[i] * rq_ + PR * (uT[i] * tq + uF[i] * pq);
         }
-        if (dP) {
-            // explicit power dependence of F and of the dr update
-            const float lnr = __logf(r);
+        if constexpr (NQ == 4) {
+            if (dP) {
+                // explicit power dependence of F and of the dr update
+                const float lnr = __logf(r);
 #pragma unroll
-            for (int i = 0; i < 3; ++i) NT[i][3] += R * (lnr * u0[i] + t.th * uT[i] + t.ph * uF[i]);
-            ntdr[3] += Rm1 * dr * (1.0f + P * lnr);
+                for (int i = 0; i < 3; ++i) NT[i][NQ - 1] += R * (lnr * u0[i] + t.th * uT[i] + t.ph * uF[i]);
+                ntdr[NQ - 1] += Rm1 * dr * (1.0f + P * lnr);
+            }
         }
 #pragma unroll
         for (int i = 0; i < 3; ++i) NT[i][i] += 1.0f;  // z_{k+1} = F + p
         TrigExact::commit(t, live);
         if (live) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
+            for (int q = 0; q < NQ; ++q) {
                 rq_prev[q] = rq[q];
                 tdr[q] = ntdr[q];
 #pragma unroll
@@ -207,20 +212,24 @@ struct TrigTangent : TrigExact {
         }
     }
 
+    // dd/dp (and dd/dP in g[3] when NQ == 4 and the power is real).
     __device__ __forceinline__ void grad(float r, bool capped, float g[4]) const {
-        float rq[4];
+        float rq[NQ];
         if (capped) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) rq[q] = rq_prev[q];
+            for (int q = 0; q < NQ; ++q) rq[q] = rq_prev[q];
         } else {
             float ir = 1.0f / r;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) rq[q] = (x * T[0][q] + y * T[1][q] + z * T[2][q]) * ir;
+            for (int q = 0; q < NQ; ++q) rq[q] = (x * T[0][q] + y * T[1][q] + z * T[2][q]) * ir;
         }
 #pragma unroll
-        for (int q = 0; q < 4; ++q) g[q] = grad_final(r, dr, rq[q], tdr[q]);
-        if (!dP) g[3] = 0.0f;
+        for (int q = 0; q < NQ; ++q) g[q] = grad_final(r, dr, rq[q], tdr[q]);
+        if (NQ == 3 || !dP) g[3] = 0.0f;
     }
 };
+
+using TrigTangent = TrigTangentN<4>;  // gradients with respect to p and the power
+using TrigNormal = TrigTangentN<3>;   // gradients with respect to p only
 
 }  // namespace cth
