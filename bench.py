@@ -168,12 +168,19 @@ def main() -> None:
         vo, _, _, _ = O.mesh_batch(s, sample, res, threads=threads)
         dt = time.perf_counter() - t
         cpu_evals = m2 * per + 7 * int(vo[-1])
+        one = sample[:2000]
+        t = time.perf_counter()
+        vo1, _, _, _ = O.mesh_batch(s, one, res, threads=1)
+        dt1 = time.perf_counter() - t
         cpu = {"value": cpu_evals / dt, "unit": "evals/s", "cores": threads, "kind": "port",
                "sample": f"{m2} of {nleaves} depth-8 surface leaves (seeded): 9^3 corners + 7 per vertex, "
-                         f"whole surface nets, oracle restatement, {threads} threads, {dt:.1f} s"}
+                         f"whole surface nets, oracle restatement, {threads} threads, {dt:.1f} s",
+               "value_1thread": (len(one) * per + 7 * int(vo1[-1])) / dt1,
+               "sample_1thread": f"the first {len(one)} leaves of that sample on one thread, {dt1:.2f} s"}
 
     extra = {}
-    for name, fn in (("c1_grid", bench_c1), ("c3_trace", bench_c3), ("c4_normals", bench_c4), ("c5_jobs", bench_c5)):
+    for name, fn in (("c1_grid", bench_c1), ("c3_trace", bench_c3), ("c4_normals", bench_c4), ("c5_jobs", bench_c5),
+                     ("reference_default_mesh", bench_ref_default)):
         try:
             extra[name] = fn(ctx, args)
         except Exception as e:  # a secondary leg must not hide the headline
@@ -291,6 +298,76 @@ def bench_c1(ctx, args):
     g.close()
     return {"workload": "64^3 grid, P8, 15 it, bailout 2", "evals_per_s": n ** 3 / per_launch,
             "us_per_launch_graph": per_launch * 1e6, "us_per_launch_eager": single * 1e3}
+
+
+def bench_ref_default(ctx, args):
+    """The reference application's own meshing workload: Mandelbulb::classic(6,
+    2.5), the bounding box [-1.2, 1.2]^3 pre-split twice into 64 leaves
+    (src/mesh/mod.rs:52-56) and every leaf meshed by naive surface nets at
+    RESOLUTION = 64 (src/mesh/mod.rs:133): 64 x 65^3 corners + 7 evaluations
+    per vertex.  GPU: one batched sample + mesh; CPU: the oracle's surface nets
+    over the same 64 leaves, threaded over leaves like the reference's pool."""
+    import numpy as np
+    import torch
+
+    import cantucci_amd as ca
+    from cantucci_amd import device as dev
+
+    shape = ca.Mandelbulb.classic(6, 2.5, ctx=ctx)
+    root = np.array([-1.2, -1.2, -1.2, 1.2, 1.2, 1.2], np.float32)
+    spans = []
+    for child in _create_spans(root):
+        spans.extend(_create_spans(child))
+    spans = np.array(spans, np.float32)
+    res = 64
+    sp = torch.from_numpy(spans).cuda()
+    out = torch.empty((len(spans), (res + 1) ** 3), device="cuda")
+
+    def run():
+        dev.sample_spans(shape, sp, res, with_cross=False, out=out)
+        return dev.mesh(shape, sp, out, res, "fd6")
+
+    m = run()
+    torch.cuda.synchronize()
+    nverts, nidx = int(m[0][-1]), int(m[1][-1])
+    del m
+    reps = 5
+    t = time.perf_counter()
+    for _ in range(reps):
+        m = run()
+        del m
+    torch.cuda.synchronize()
+    gpu_ms = 1e3 * (time.perf_counter() - t) / reps
+    evals = len(spans) * (res + 1) ** 3 + 7 * nverts
+    leg = {"workload": "64 leaves of [-1.2,1.2]^3 at res 64, classic(6, 2.5), whole surface nets",
+           "vertices": nverts, "quads": nidx // 6, "evals": evals, "gpu_ms": gpu_ms, "gpu_evals_per_s": evals / gpu_ms * 1e3}
+    if args.cpu_seconds > 0:
+        import oracle as O
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        t = time.perf_counter()
+        O.mesh_batch(O.mandelbulb(6, 2.5), spans, res, threads=threads)
+        cpu_s = time.perf_counter() - t
+        t = time.perf_counter()
+        O.mesh_batch(O.mandelbulb(6, 2.5), spans[:4], res, threads=1)
+        cpu1_s = (time.perf_counter() - t) * len(spans) / 4
+        leg.update({"cpu_ms": cpu_s * 1e3, "cpu_threads": threads, "cpu_1thread_ms_extrapolated": cpu1_s * 1e3,
+                    "cpu_1thread_sample": "4 of the 64 leaves on one thread, scaled to 64",
+                    "speedup_vs_cpu": cpu_s * 1e3 / gpu_ms})
+    return leg
+
+
+def _create_spans(span):
+    """`create_spans` (src/octree/mod.rs:315-329): centre = s + (e - s) / 2;
+    child bit 2 = +x half, bit 1 = +y, bit 0 = +z."""
+    import numpy as np
+    s, e = span[:3], span[3:]
+    c = (s + (e - s) / np.float32(2.0)).astype(np.float32)
+    out = []
+    for i in range(8):
+        lo = np.array([c[a] if (i >> (2 - a)) & 1 else s[a] for a in range(3)], np.float32)
+        hi = np.array([e[a] if (i >> (2 - a)) & 1 else c[a] for a in range(3)], np.float32)
+        out.append(np.concatenate([lo, hi]))
+    return out
 
 
 def bench_c3(ctx, args):

@@ -102,6 +102,19 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wsum, 
     return off + inc - v;
 }
 
+// The wave's LDS writes become visible to its other lanes.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t& total) {
+    const uint32_t inc = wave_incl_scan(v);
+    total = (uint32_t)__shfl((int)inc, 63, 64);
+    return inc - v;
+}
+
 __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* wsum) {
     uint32_t t;
     (void)block_excl_scan(v, wsum, t);
@@ -119,42 +132,60 @@ __device__ __forceinline__ void cell_xyz(const MeshDims& m, uint32_t c, uint32_t
     x = c >> (2u * m.lg);
 }
 
-// Per leaf: vertex count and index count (6 per quad).  Slot `nleaves` of
-// both count arrays is zeroed so that an exclusive scan over nleaves + 1
-// entries ends with the totals.
-// With `cross` non-null it also writes the leaf's crossing words (bit c of
-// word c / 64 = cell c crosses), the layout cth_sample_spans_dev uses.
-// One workgroup per leaf (the large resolutions; see k_mesh_count_w).
-__global__ __launch_bounds__(kMeshBlock) void k_mesh_count(const float* __restrict__ dists, MeshDims m,
-                                                           uint32_t nleaves, uint64_t* __restrict__ vcnt,
-                                                           uint64_t* __restrict__ icnt, uint64_t* __restrict__ cross) {
+// ---------------------------------------------------------------------------
+// Plane units for the large resolutions (res 32 and 64, the reference's own
+// RESOLUTION): unit u = leaf * res + x holds the res^2 cells (x, *, *) of a
+// leaf, so a single leaf still spreads over res workgroups.  The units are
+// counted and scanned in cube() order; a unit's vertices then start at its
+// scanned offset, and cells of plane x - 1 (the only other plane its quads
+// touch) are numbered from the previous unit's offset.
+// ---------------------------------------------------------------------------
+
+// Per unit: vertex count and index count (6 per quad); slot `units` of both
+// arrays is zeroed so that an exclusive scan over units + 1 entries ends
+// with the totals.  With `cross` non-null it also writes the leaf's
+// crossing words (bit c of word c / 64 = cell c crosses), the layout
+// cth_sample_spans_dev uses (res^2 is a multiple of 256, so every wave's
+// 64 cells are one word).
+__global__ __launch_bounds__(kMeshBlock) void k_mesh_count_p(const float* __restrict__ dists, MeshDims m,
+                                                             uint32_t units, uint64_t* __restrict__ vcnt,
+                                                             uint64_t* __restrict__ icnt, uint64_t* __restrict__ cross) {
     __shared__ uint32_t wsum[kMeshBlock / 64];
-    const uint32_t leaf = blockIdx.x;
+    const uint32_t u = blockIdx.x;
+    const uint32_t leaf = u >> m.lg, x = u & (m.res - 1u);
     const float* D = dists + (size_t)leaf * m.n1 * m.n1 * m.n1;
-    const uint32_t nwords = (m.cells + 63u) >> 6;
+    const uint32_t pc = m.res * m.res;
+    const uint32_t nwords = m.cells >> 6;
     uint32_t nv = 0, nf = 0;
-    // Wave w handles cells 64 w + lane + 256 k: one crossing word per ballot.
-    for (uint32_t c = threadIdx.x; c < m.cells; c += blockDim.x) {
-        uint32_t x, y, z;
-        cell_xyz(m, c, x, y, z);
+    for (uint32_t j = threadIdx.x; j < pc; j += blockDim.x) {
+        const uint32_t y = j >> m.lg, z = j & (m.res - 1u);
         float d[8];
-        uint32_t mk = load_cell(D, m.n1, x, y, z, d);
+        const uint32_t mk = load_cell(D, m.n1, x, y, z, d);
         const bool cr = crossing(mk);
         nv += cr ? 1u : 0u;
         nf += (uint32_t)__popc(face_bits(mk, x, y, z));
         const uint64_t bits = __ballot(cr);
-        if (cross && (threadIdx.x & 63u) == 0) cross[(size_t)leaf * nwords + (c >> 6)] = bits;
+        if (cross && (threadIdx.x & 63u) == 0) cross[(size_t)leaf * nwords + ((x * pc + j) >> 6)] = bits;
     }
     nv = block_sum(nv, wsum);
     nf = block_sum(nf, wsum);
     if (threadIdx.x == 0) {
-        vcnt[leaf] = nv;
-        icnt[leaf] = 6ull * nf;
-        if (leaf == 0) {
-            vcnt[nleaves] = 0;
-            icnt[nleaves] = 0;
+        vcnt[u] = nv;
+        icnt[u] = 6ull * nf;
+        if (u == 0) {
+            vcnt[units] = 0;
+            icnt[units] = 0;
         }
     }
+}
+
+// Per-leaf offsets from the scanned unit offsets: leaf i starts at unit i * res.
+__global__ void k_leaf_offsets(const uint64_t* __restrict__ uv, const uint64_t* __restrict__ ui, uint32_t nleaves,
+                               uint32_t res, uint64_t* __restrict__ voff, uint64_t* __restrict__ ioff) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > nleaves) return;
+    voff[i] = uv[(size_t)i * res];
+    ioff[i] = ui[(size_t)i * res];
 }
 
 // Vertex position of a crossing cell (src/mesh/buffer.rs:149-250), in the
@@ -257,92 +288,83 @@ __device__ __forceinline__ void emit_quads(uint32_t* out, uint32_t fb, bool neg,
     }
 }
 
-__device__ __forceinline__ uint32_t vertex_of(const uint64_t* words, const uint32_t* wpre, const MeshDims& m,
-                                              uint32_t x, uint32_t y, uint32_t z) {
-    const uint32_t c = (((x << m.lg) | y) << m.lg) | z;
-    const uint32_t w = c >> 6, b = c & 63u;
-    const uint64_t below = (b == 0) ? 0ull : (words[w] & ((~0ull) >> (64u - b)));
-    return wpre[w] + (uint32_t)__popcll(below);
-}
-
-// One workgroup per leaf.  Dynamic LDS: ceil(cells / 64) mask words and
-// their exclusive popcount prefix.  For the normal passes that follow, each
-// vertex record temporarily holds the leaf's FD half-steps in its normal
+// One workgroup per plane unit.  LDS: the crossing words of planes x and
+// x - 1 and their exclusive popcount prefixes (res^2 / 64 words each).  The
+// vertex records temporarily hold the leaf's 6-tap half-steps in their normal
 // slots (src/mesh/buffer.rs:257: 0.7 * (e - s) / res per axis).
-__global__ __launch_bounds__(kMeshBlock) void k_mesh_emit(const float* __restrict__ spans,
-                                                          const float* __restrict__ dists, MeshDims m,
-                                                          const uint64_t* __restrict__ voff,
-                                                          const uint64_t* __restrict__ ioff, float* __restrict__ verts,
-                                                          uint32_t* __restrict__ idx) {
+__global__ __launch_bounds__(kMeshBlock) void k_mesh_emit_p(
+    const float* __restrict__ spans, const float* __restrict__ dists, MeshDims m, const uint64_t* __restrict__ uv,
+    const uint64_t* __restrict__ ui, const uint64_t* __restrict__ voff, const uint64_t* __restrict__ ioff,
+    float* __restrict__ verts, uint32_t* __restrict__ idx) {
     extern __shared__ uint64_t lds_words[];
     __shared__ uint32_t wsum[kMeshBlock / 64];
-    const uint32_t leaf = blockIdx.x;
-    const uint32_t nwords = (m.cells + 63u) >> 6;
-    uint64_t* words = lds_words;
-    uint32_t* wpre = reinterpret_cast<uint32_t*>(lds_words + nwords);
+    const uint32_t u = blockIdx.x;
+    const uint32_t leaf = u >> m.lg, x = u & (m.res - 1u);
+    const uint32_t pc = m.res * m.res, pw = pc >> 6;
+    uint64_t* wc = lds_words;   // plane x
+    uint64_t* wp = wc + pw;     // plane x - 1
+    uint32_t* prc = reinterpret_cast<uint32_t*>(wp + pw);
+    uint32_t* prp = prc + pw;
     const float* D = dists + (size_t)leaf * m.n1 * m.n1 * m.n1;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
 
-    // Crossing mask, one wave per 64 cells.
-    for (uint32_t w = wave; w < nwords; w += nwaves) {
-        const uint32_t c = (w << 6) | lane;
-        bool cr = false;
-        if (c < m.cells) {
-            uint32_t x, y, z;
-            cell_xyz(m, c, x, y, z);
-            float d[8];
-            cr = crossing(load_cell(D, m.n1, x, y, z, d));
-        }
+    const uint32_t nplanes = (x > 0) ? 2u : 1u;
+    for (uint32_t w = wave; w < nplanes * pw; w += nwaves) {
+        const uint32_t px = (w < pw) ? x : x - 1u, ww = (w < pw) ? w : w - pw;
+        const uint32_t j = (ww << 6) | lane;
+        float d[8];
+        const bool cr = crossing(load_cell(D, m.n1, px, j >> m.lg, j & (m.res - 1u), d));
         const uint64_t bits = __ballot(cr);
-        if (lane == 0) words[w] = bits;
+        if (lane == 0) ((w < pw) ? wc : wp)[ww] = bits;
     }
     __syncthreads();
-    // Exclusive popcount prefix of the words: contiguous segments per thread.
-    {
-        const uint32_t seg = (nwords + blockDim.x - 1) / blockDim.x;
-        const uint32_t lo = std::min(threadIdx.x * seg, nwords), hi = std::min(lo + seg, nwords);
-        uint32_t s = 0;
-        for (uint32_t w = lo; w < hi; ++w) s += (uint32_t)__popcll(words[w]);
+    if (wave < nplanes) {  // pw <= 64: one wave scans one plane's words
+        const uint64_t* words = (wave == 0) ? wc : wp;
+        uint32_t* pre = (wave == 0) ? prc : prp;
+        const uint32_t v = (lane < pw) ? (uint32_t)__popcll(words[lane]) : 0u;
         uint32_t tot;
-        uint32_t acc = block_excl_scan(s, wsum, tot);
-        for (uint32_t w = lo; w < hi; ++w) {
-            wpre[w] = acc;
-            acc += (uint32_t)__popcll(words[w]);
-        }
+        const uint32_t ex = wave_excl_scan(v, tot);
+        if (lane < pw) pre[lane] = ex;
     }
     __syncthreads();
 
+    const size_t u0 = (size_t)leaf * m.res;  // the leaf's first unit
+    const uint32_t vb_x = (uint32_t)(uv[u] - uv[u0]);
+    const uint32_t vb_p = (x > 0) ? (uint32_t)(uv[u - 1] - uv[u0]) : 0u;
+    auto pt = [&](uint32_t px, uint32_t py, uint32_t pz) {
+        const bool cur = (px == x);
+        const uint32_t j = (py << m.lg) | pz, w = j >> 6, b = j & 63u;
+        const uint64_t word = cur ? wc[w] : wp[w];
+        const uint64_t below = (b == 0) ? 0ull : (word & ((~0ull) >> (64u - b)));
+        return (cur ? vb_x + prc[w] : vb_p + prp[w]) + (uint32_t)__popcll(below);
+    };
     float s1[3], step[3], delta[3];
     leaf_frame(spans + 6 * (size_t)leaf, m.fres, s1, step, delta);
-    const uint64_t vbase = voff[leaf];
-    const uint64_t ibase = ioff[leaf];
-    auto pt = [&](uint32_t px, uint32_t py, uint32_t pz) { return vertex_of(words, wpre, m, px, py, pz); };
+    float* vout = verts + 7 * voff[leaf];
+    uint32_t* iout = idx + ioff[leaf] + (ui[u] - ui[u0]);
 
     uint32_t fcarry = 0;
-    for (uint32_t base = 0; base < m.cells; base += blockDim.x) {
-        const uint32_t c = base + threadIdx.x;
-        uint32_t x = 0, y = 0, z = 0, mk = 0, fb = 0;
+    for (uint32_t base = 0; base < pc; base += blockDim.x) {
+        const uint32_t j = base + threadIdx.x;  // pc is a multiple of the block
+        const uint32_t y = j >> m.lg, z = j & (m.res - 1u);
         float d[8];
-        if (c < m.cells) {
-            cell_xyz(m, c, x, y, z);
-            mk = load_cell(D, m.n1, x, y, z, d);
-            fb = face_bits(mk, x, y, z);
-        }
+        const uint32_t mk = load_cell(D, m.n1, x, y, z, d);
+        const uint32_t fb = face_bits(mk, x, y, z);
         uint32_t tile_faces;
         const uint32_t frank = block_excl_scan((uint32_t)__popc(fb), wsum, tile_faces);
-        if (c < m.cells && crossing(mk)) {
+        if (crossing(mk)) {
             float p[3];
             place_vertex(d, x, y, z, s1, step, p);
-            put_vertex(verts + 7 * (vbase + pt(x, y, z)), p, delta);
+            put_vertex(vout + 7 * (size_t)pt(x, y, z), p, delta);
         }
-        if (fb) emit_quads(idx + ibase + 6ull * (fcarry + frank), fb, d[0] < 0.0f, x, y, z, pt);
+        if (fb) emit_quads(iout + 6ull * (fcarry + frank), fb, d[0] < 0.0f, x, y, z, pt);
         fcarry += tile_faces;
     }
 }
 
 // ---------------------------------------------------------------------------
-// Wave-per-leaf kernels for res <= 32 (count) and res <= 16 (emit; the
-// octree workload meshes 8^3 cells per leaf): one wave owns a leaf, each lane a run of whole cell rows
+// Wave-per-leaf kernels for res <= 16 (the octree workload meshes 8^3 cells
+// per leaf): one wave owns a leaf, each lane a run of whole cell rows
 // (x, y, z = 0..res-1), which are contiguous in cube() order.  A lane
 // streams the four corner columns of its row along z (each a contiguous run
 // of res + 1 floats), so every corner load serves two cells, and no
@@ -350,18 +372,6 @@ __global__ __launch_bounds__(kMeshBlock) void k_mesh_emit(const float* __restric
 // wave's own slice of LDS.  The grid is persistent over leaves.
 // ---------------------------------------------------------------------------
 
-// The wave's LDS writes become visible to its other lanes.
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t& total) {
-    const uint32_t inc = wave_incl_scan(v);
-    total = (uint32_t)__shfl((int)inc, 63, 64);
-    return inc - v;
-}
 
 // Calls f(z, d[8], mask) for the cells (x, y, 0..RES-1) of the leaf's
 // corner grid D ((RES+1)^3, GridTable order).
@@ -644,6 +654,37 @@ inline uint32_t wave_grid(const Ctx& c, uint32_t nleaves) {
 
 }  // namespace
 
+// Exclusive scans of the vertex and index counts (2 * cnt entries, vertex
+// counts first) into voff / ioff.
+static int scan_pair(Ctx& c, uint64_t* counts, size_t cnt, uint64_t* voff, uint64_t* ioff, hipStream_t st) {
+    size_t tmp_bytes = 0;
+    void* tmp = nullptr;
+    int rc = check_hip(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, counts, voff, (int)cnt, st), "scan size");
+    if (!rc && c.scratch(WS_SCAN, tmp_bytes, &tmp)) rc = fail(CTH_E_NOMEM, "scan scratch allocation failed");
+    if (!rc) rc = check_hip(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, counts, voff, (int)cnt, st), "scan");
+    if (!rc)
+        rc = check_hip(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, counts + cnt, ioff, (int)cnt, st), "scan");
+    return rc;
+}
+
+// Plane-unit counts and their exclusive scans, in the context's scratch:
+// *uoff holds units + 1 vertex offsets followed by units + 1 index offsets.
+static int plane_offsets(Ctx& c, const float* dists, const MeshDims& m, uint32_t nleaves, uint64_t* cross,
+                         uint64_t** uoff, hipStream_t st) {
+    const size_t units = (size_t)nleaves * m.res;
+    if (units >= (size_t(1) << 31)) return fail(CTH_E_INVALID, "more than 2^31 mesh plane units");
+    const size_t cnt = units + 1;
+    uint64_t* counts = nullptr;
+    if (c.scratch(WS_COUNTS, 2 * cnt * sizeof(uint64_t), (void**)&counts) ||
+        c.scratch(WS_UOFF, 2 * cnt * sizeof(uint64_t), (void**)uoff))
+        return fail(CTH_E_NOMEM, "mesh scratch allocation failed");
+    hipLaunchKernelGGL(k_mesh_count_p, dim3((uint32_t)units), dim3(kMeshBlock), 0, st, dists, m, (uint32_t)units,
+                       counts, counts + cnt, cross);
+    int rc = check_hip(hipGetLastError(), "mesh plane count launch");
+    if (!rc) rc = scan_pair(c, counts, cnt, *uoff, *uoff + cnt, st);
+    return rc;
+}
+
 int mesh_count(Ctx& c, const float* dists, uint32_t nleaves, uint32_t res, uint64_t* voff, uint64_t* ioff,
                uint64_t* cross, uint64_t totals[2], hipStream_t st) {
     MeshDims m;
@@ -656,40 +697,36 @@ int mesh_count(Ctx& c, const float* dists, uint32_t nleaves, uint32_t res, uint6
         if (!rc) rc = check_hip(hipStreamSynchronize(st), "mesh count sync");
         return rc;
     }
-    const size_t cnt = (size_t)nleaves + 1;
-    uint64_t* counts = nullptr;
-    void* tmp = nullptr;
-    size_t tmp_bytes = 0;
-    int rc = c.scratch(WS_COUNTS, 2 * cnt * sizeof(uint64_t), (void**)&counts);
-    if (rc) return fail(rc, "mesh count scratch allocation failed");
-    {
-        if (res <= 32) {
-            const uint32_t blocks = wave_grid(c, nleaves);
-            switch (res) {
-                case 1: LAUNCH_COUNT_W(1); break;
-                case 2: LAUNCH_COUNT_W(2); break;
-                case 4: LAUNCH_COUNT_W(4); break;
-                case 8: LAUNCH_COUNT_W(8); break;
-                case 16: LAUNCH_COUNT_W(16); break;
-                default: LAUNCH_COUNT_W(32); break;
-            }
-        } else {
-            hipLaunchKernelGGL(k_mesh_count, dim3(nleaves), dim3(kMeshBlock), 0, st, dists, m, nleaves, counts,
-                               counts + cnt, cross);
+    int rc = CTH_OK;
+    if (res <= 16) {
+        // wave per leaf: the counts are per leaf already
+        const size_t cnt = (size_t)nleaves + 1;
+        uint64_t* counts = nullptr;
+        if (c.scratch(WS_COUNTS, 2 * cnt * sizeof(uint64_t), (void**)&counts))
+            return fail(CTH_E_NOMEM, "mesh count scratch allocation failed");
+        const uint32_t blocks = wave_grid(c, nleaves);
+        switch (res) {
+            case 1: LAUNCH_COUNT_W(1); break;
+            case 2: LAUNCH_COUNT_W(2); break;
+            case 4: LAUNCH_COUNT_W(4); break;
+            case 8: LAUNCH_COUNT_W(8); break;
+            default: LAUNCH_COUNT_W(16); break;
         }
         rc = check_hip(hipGetLastError(), "mesh count launch");
+        if (!rc) rc = scan_pair(c, counts, cnt, voff, ioff, st);
+    } else {
+        // plane units, then the leaves' offsets from the units'
+        uint64_t* uoff = nullptr;
+        rc = plane_offsets(c, dists, m, nleaves, cross, &uoff, st);
+        if (!rc) {
+            const size_t units = (size_t)nleaves * res;
+            hipLaunchKernelGGL(k_leaf_offsets, dim3(cdiv_u32((uint64_t)nleaves + 1, 256)), dim3(256), 0, st, uoff,
+                               uoff + units + 1, nleaves, res, voff, ioff);
+            rc = check_hip(hipGetLastError(), "leaf offsets launch");
+        }
     }
-    if (!rc)
-        rc = check_hip(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, counts, voff, (int)cnt, st),
-                       "scan size");
-    if (!rc && c.scratch(WS_SCAN, tmp_bytes, &tmp)) rc = fail(CTH_E_NOMEM, "scan scratch allocation failed");
-    if (!rc) rc = check_hip(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, counts, voff, (int)cnt, st), "scan");
-    if (!rc)
-        rc = check_hip(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, counts + cnt, ioff, (int)cnt, st), "scan");
-    if (!rc)
-        rc = check_hip(hipMemcpyAsync(c.pinned, voff + nleaves, 8, hipMemcpyDeviceToHost, st), "D2H totals");
-    if (!rc)
-        rc = check_hip(hipMemcpyAsync(c.pinned + 1, ioff + nleaves, 8, hipMemcpyDeviceToHost, st), "D2H totals");
+    if (!rc) rc = check_hip(hipMemcpyAsync(c.pinned, voff + nleaves, 8, hipMemcpyDeviceToHost, st), "D2H totals");
+    if (!rc) rc = check_hip(hipMemcpyAsync(c.pinned + 1, ioff + nleaves, 8, hipMemcpyDeviceToHost, st), "D2H totals");
     int rs = check_hip(hipStreamSynchronize(st), "mesh count sync");
     if (!rc && !rs) {
         totals[0] = c.pinned[0];
@@ -707,8 +744,6 @@ int mesh_emit(Ctx& c, const cth_shape& s, const float* spans, const float* dists
     if (normals == CTH_NORMALS_TANGENT && s.kind == CTH_SHAPE_SPHERE)
         return fail(CTH_E_UNSUPPORTED, "tangent normals need a Mandelbulb kind");
     if (nleaves == 0) return CTH_OK;
-    const uint32_t nwords = (m.cells + 63u) >> 6;
-    const size_t lds = (size_t)nwords * (sizeof(uint64_t) + sizeof(uint32_t));
     if (res <= 16) {
         const uint32_t blocks = wave_grid(c, nleaves);
         switch (res) {
@@ -719,8 +754,13 @@ int mesh_emit(Ctx& c, const cth_shape& s, const float* spans, const float* dists
             default: LAUNCH_EMIT_W(16); break;
         }
     } else {
-        hipLaunchKernelGGL(k_mesh_emit, dim3(nleaves), dim3(kMeshBlock), lds, st, spans, dists, m, voff, ioff, verts,
-                           idx);
+        // the plane units' offsets (recounted: a count call leaves only the leaves')
+        uint64_t* uoff = nullptr;
+        if (int rc = plane_offsets(c, dists, m, nleaves, nullptr, &uoff, st)) return rc;
+        const size_t units = (size_t)nleaves * res;
+        const size_t lds = 2 * (size_t)(m.res * m.res / 64) * (sizeof(uint64_t) + sizeof(uint32_t));
+        hipLaunchKernelGGL(k_mesh_emit_p, dim3((uint32_t)units), dim3(kMeshBlock), lds, st, spans, dists, m, uoff,
+                           uoff + units + 1, voff, ioff, verts, idx);
     }
     int rc = check_hip(hipGetLastError(), "mesh emit launch");
     if (rc || nverts == 0) return rc;

@@ -59,7 +59,7 @@ def assert_topology(voff, ioff, ix, voff_o, ioff_o, ix_o):
     assert np.array_equal(ix, ix_o), "indices differ"
 
 
-@pytest.mark.parametrize("res", [8, 1, 2, 16])
+@pytest.mark.parametrize("res", [8, 1, 2, 16, 32, 64])
 def test_sphere_mesh_bitexact(ctx, torch_cuda, res):
     """Sphere: exact closed-form distances, so the whole mesh -- positions,
     6-tap normals and distances -- is bit-identical."""
