@@ -93,3 +93,15 @@ def test_fastdiv_magic_numbers():
                                       np.array([2 ** 31 - 1], np.uint64)]))
         q = (((n * np.uint64(m)) >> np.uint64(32)) + n) >> np.uint64(s)
         assert (q == n // np.uint64(d)).all(), d
+
+
+def test_bench_refuses_several_gpus():
+    """bench.py measures one GPU; --gpus N > 1 exits with a message before
+    touching any device."""
+    import subprocess
+    import sys
+    from pathlib import Path
+    root = Path(__file__).resolve().parents[1]
+    res = subprocess.run([sys.executable, str(root / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                         timeout=120)
+    assert res.returncode == 2 and "one GPU" in res.stderr, res.stderr
