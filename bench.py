@@ -180,7 +180,7 @@ def main() -> None:
 
     extra = {}
     for name, fn in (("c1_grid", bench_c1), ("c3_trace", bench_c3), ("c4_normals", bench_c4), ("c5_jobs", bench_c5),
-                     ("reference_default_mesh", bench_ref_default)):
+                     ("reference_default_mesh", bench_ref_default), ("point_query", bench_point_query)):
         try:
             extra[name] = fn(ctx, args)
         except Exception as e:  # a secondary leg must not hide the headline
@@ -298,6 +298,48 @@ def bench_c1(ctx, args):
     g.close()
     return {"workload": "64^3 grid, P8, 15 it, bailout 2", "evals_per_s": n ** 3 / per_launch,
             "us_per_launch_graph": per_launch * 1e6, "us_per_launch_eager": single * 1e3}
+
+
+def bench_point_query(ctx, args):
+    """Latency of one host-buffer query, the reference's per-frame calls:
+    the fly camera's min_distance_from (src/control/fly.rs:91-92) and the 25
+    focus rays of get_focii (src/mesh/mod.rs:205-243)."""
+    import numpy as np
+
+    import cantucci_amd as ca
+    from cantucci_amd import device as dev  # noqa: F401  (torch loaded first)
+
+    shape = ca.Mandelbulb.classic(6, 2.5, ctx=ctx)
+    p = (-3.0, 0.1, 0.2)
+    for _ in range(20):
+        shape.min_distance_from(p)
+    reps = 2000
+    t = time.perf_counter()
+    for _ in range(reps):
+        shape.min_distance_from(p)
+    one = (time.perf_counter() - t) / reps
+    rng = np.random.default_rng(0)
+    dirs = rng.normal(size=(25, 3)) * 0.1 + np.array([1.0, 0.0, 0.0])
+    dirs = (dirs / np.linalg.norm(dirs, axis=1, keepdims=True)).astype(np.float32)
+    origin = np.array([-3.0, 0.0, 0.0], np.float32)
+    import ctypes as C
+    from cantucci_amd._lib import check, lib
+    depth = np.empty(25, np.float32)
+    steps = np.empty(25, np.uint32)
+    pos = np.empty((25, 3), np.float32)
+
+    def focii():
+        check(lib.cth_sphere_trace(ctx.handle, C.byref(shape.desc), origin.ctypes.data, 0, dirs.ctypes.data, 25, 100,
+                                   1e-6, 0.0, float("inf"), depth.ctypes.data, steps.ctypes.data, pos.ctypes.data))
+
+    for _ in range(20):
+        focii()
+    t = time.perf_counter()
+    for _ in range(500):
+        focii()
+    trace = (time.perf_counter() - t) / 500
+    return {"workload": "classic(6, 2.5): one min_distance_from; 25 focus rays (eps 1e-6, 100 steps)",
+            "us_min_distance_from": one * 1e6, "us_focus_rays": trace * 1e6}
 
 
 def bench_ref_default(ctx, args):

@@ -63,16 +63,21 @@ static int check_shape(const cth_shape* s) {
 static hipStream_t pick_stream(Ctx*, void* stream) { return (hipStream_t)stream; }
 
 // RAII device buffer for the host-pointer entry points.
-struct DevBuf {
-    void* p = nullptr;
-    ~DevBuf() {
-        if (p) (void)hipFree(p);
+// Device staging for the host-buffer entry points: sub-buffers (256-byte
+// aligned) of one of the context's scratch slots, so that a small query --
+// one point per frame for the reference's fly camera -- allocates nothing.
+template <size_t N>
+int staging(Ctx& c, int slot, const size_t (&bytes)[N], void* (&out)[N]) {
+    size_t off[N], total = 0;
+    for (size_t i = 0; i < N; ++i) {
+        off[i] = total;
+        total += (bytes[i] + 255) & ~size_t(255);
     }
-    int alloc(size_t bytes) {
-        if (bytes == 0) bytes = 4;
-        return check_hip(hipMalloc(&p, bytes), "hipMalloc");
-    }
-};
+    void* base = nullptr;
+    if (c.scratch(slot, total ? total : 256, &base)) return fail(CTH_E_NOMEM, "staging allocation failed");
+    for (size_t i = 0; i < N; ++i) out[i] = static_cast<char*>(base) + off[i];
+    return CTH_OK;
+}
 
 }  // namespace cth
 
@@ -279,22 +284,18 @@ int cth_sphere_trace(cth_ctx* c, const cth_shape* s, const float* origins, int p
     if (n == 0) return CTH_OK;
     if (!origins || !dirs || !depth || !steps) return fail(CTH_E_INVALID, "null buffer");
     size_t no = per_ray_origin ? n : 1;
-    DevBuf dor, ddi, dde, dst, dhp;
-    int rc = dor.alloc(no * 12);
-    if (!rc) rc = ddi.alloc(n * 12);
-    if (!rc) rc = dde.alloc(n * 4);
-    if (!rc) rc = dst.alloc(n * 4);
-    if (!rc && hit_pos) rc = dhp.alloc(n * 12);
+    void* b[5];
+    int rc = staging(*c, WS_HOST_IN, {no * 12, n * 12, n * 4, n * 4, hit_pos ? n * 12 : 0}, b);
     if (rc) return rc;
-    rc = check_hip(hipMemcpyAsync(dor.p, origins, no * 12, hipMemcpyHostToDevice, c->stream), "H2D");
-    if (!rc) rc = check_hip(hipMemcpyAsync(ddi.p, dirs, n * 12, hipMemcpyHostToDevice, c->stream), "H2D");
+    rc = check_hip(hipMemcpyAsync(b[0], origins, no * 12, hipMemcpyHostToDevice, c->stream), "H2D");
+    if (!rc) rc = check_hip(hipMemcpyAsync(b[1], dirs, n * 12, hipMemcpyHostToDevice, c->stream), "H2D");
     if (!rc)
-        rc = cth_sphere_trace_dev(c, s, (const float*)dor.p, per_ray_origin, (const float*)ddi.p, n, max_steps,
-                                  eps_abs, eps_rel, t_max, (float*)dde.p, (uint32_t*)dst.p,
-                                  hit_pos ? (float*)dhp.p : nullptr, c->stream);
-    if (!rc) rc = check_hip(hipMemcpyAsync(depth, dde.p, n * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
-    if (!rc) rc = check_hip(hipMemcpyAsync(steps, dst.p, n * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
-    if (!rc && hit_pos) rc = check_hip(hipMemcpyAsync(hit_pos, dhp.p, n * 12, hipMemcpyDeviceToHost, c->stream), "D2H");
+        rc = cth_sphere_trace_dev(c, s, (const float*)b[0], per_ray_origin, (const float*)b[1], n, max_steps,
+                                  eps_abs, eps_rel, t_max, (float*)b[2], (uint32_t*)b[3],
+                                  hit_pos ? (float*)b[4] : nullptr, c->stream);
+    if (!rc) rc = check_hip(hipMemcpyAsync(depth, b[2], n * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+    if (!rc) rc = check_hip(hipMemcpyAsync(steps, b[3], n * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+    if (!rc && hit_pos) rc = check_hip(hipMemcpyAsync(hit_pos, b[4], n * 12, hipMemcpyDeviceToHost, c->stream), "D2H");
     int rs = check_hip(hipStreamSynchronize(c->stream), "sync");
     return rc ? rc : rs;
 }
@@ -445,20 +446,21 @@ int cth_sample_spans(cth_ctx* c, const cth_shape* s, const cth_span* spans, size
     if (res > 512) return fail(CTH_E_INVALID, "resolution above 512");
     size_t per = size_t(res + 1) * (res + 1) * (res + 1);
     size_t words = (size_t(res) * res * res + 63) / 64;
-    DevBuf dsp, dd, dc;
-    int rc = dsp.alloc(nspans * sizeof(cth_span));
-    if (!rc) rc = dd.alloc(nspans * per * sizeof(float));
-    if (!rc && cross_bits) rc = dc.alloc(nspans * words * sizeof(uint64_t));
+    void* b[3];
+    int rc = staging(*c, WS_HOST_IN,
+                     {nspans * sizeof(cth_span), nspans * per * sizeof(float),
+                      cross_bits ? nspans * words * sizeof(uint64_t) : 0},
+                     b);
     if (rc) return rc;
-    rc = check_hip(hipMemcpyAsync(dsp.p, spans, nspans * sizeof(cth_span), hipMemcpyHostToDevice, c->stream), "H2D");
+    rc = check_hip(hipMemcpyAsync(b[0], spans, nspans * sizeof(cth_span), hipMemcpyHostToDevice, c->stream), "H2D");
     if (!rc)
-        rc = cth_sample_spans_dev(c, s, (const float*)dsp.p, nspans, res, (float*)dd.p,
-                                  cross_bits ? (uint64_t*)dc.p : nullptr, nullptr, c->stream);
+        rc = cth_sample_spans_dev(c, s, (const float*)b[0], nspans, res, (float*)b[1],
+                                  cross_bits ? (uint64_t*)b[2] : nullptr, nullptr, c->stream);
     if (!rc)
-        rc = check_hip(hipMemcpyAsync(dists, dd.p, nspans * per * sizeof(float), hipMemcpyDeviceToHost, c->stream),
+        rc = check_hip(hipMemcpyAsync(dists, b[1], nspans * per * sizeof(float), hipMemcpyDeviceToHost, c->stream),
                        "D2H");
     if (!rc && cross_bits)
-        rc = check_hip(hipMemcpyAsync(cross_bits, dc.p, nspans * words * sizeof(uint64_t), hipMemcpyDeviceToHost,
+        rc = check_hip(hipMemcpyAsync(cross_bits, b[2], nspans * words * sizeof(uint64_t), hipMemcpyDeviceToHost,
                                       c->stream),
                        "D2H");
     int rs = check_hip(hipStreamSynchronize(c->stream), "sync");
@@ -503,32 +505,30 @@ int cth_mesh_span(cth_ctx* c, const cth_shape* s, const cth_span* span, uint32_t
     if (normals == CTH_NORMALS_TANGENT && s->kind == CTH_SHAPE_SPHERE)
         return fail(CTH_E_UNSUPPORTED, "tangent normals need a Mandelbulb kind");
     const size_t per = size_t(res + 1) * (res + 1) * (res + 1);
-    DevBuf dsp, dd, doff, dv, di;
-    int rc = dsp.alloc(sizeof(cth_span));
-    if (!rc) rc = dd.alloc(per * sizeof(float));
-    if (!rc) rc = doff.alloc(4 * sizeof(uint64_t));
+    void* b[3];
+    int rc = staging(*c, WS_HOST_IN, {sizeof(cth_span), per * sizeof(float), 4 * sizeof(uint64_t)}, b);
     if (rc) return rc;
-    rc = check_hip(hipMemcpyAsync(dsp.p, span, sizeof(cth_span), hipMemcpyHostToDevice, c->stream), "H2D");
-    if (!rc) rc = cth_sample_spans_dev(c, s, (const float*)dsp.p, 1, res, (float*)dd.p, nullptr, nullptr, c->stream);
+    rc = check_hip(hipMemcpyAsync(b[0], span, sizeof(cth_span), hipMemcpyHostToDevice, c->stream), "H2D");
+    if (!rc) rc = cth_sample_spans_dev(c, s, (const float*)b[0], 1, res, (float*)b[1], nullptr, nullptr, c->stream);
     uint64_t nv = 0, ni = 0;
-    uint64_t* off = (uint64_t*)doff.p;
-    if (!rc) rc = cth_mesh_count_dev(c, (const float*)dd.p, 1, res, off, off + 2, nullptr, &nv, &ni, c->stream);
+    uint64_t* off = (uint64_t*)b[2];
+    if (!rc) rc = cth_mesh_count_dev(c, (const float*)b[1], 1, res, off, off + 2, nullptr, &nv, &ni, c->stream);
     if (rc) return rc;
     *nverts = (size_t)nv;
     *nidx = (size_t)ni;
     if (!vertices && !indices) return CTH_OK;
     if ((vertices && vert_cap < nv) || (indices && idx_cap < ni) || (nv && !vertices) || (ni && !indices))
         return fail(CTH_E_INVALID, "vertex or index buffer too small for the mesh");
-    rc = dv.alloc(nv * sizeof(cth_vertex));
-    if (!rc) rc = di.alloc(ni * sizeof(uint32_t));
+    void* o[2];
+    rc = staging(*c, WS_HOST_OUT, {nv * sizeof(cth_vertex), ni * sizeof(uint32_t) + 4}, o);
     if (!rc)
-        rc = cth_mesh_emit_dev(c, s, (const float*)dsp.p, (const float*)dd.p, 1, res, normals, off, off + 2, nv,
-                               (cth_vertex*)dv.p, (uint32_t*)di.p, c->stream);
+        rc = cth_mesh_emit_dev(c, s, (const float*)b[0], (const float*)b[1], 1, res, normals, off, off + 2, nv,
+                               (cth_vertex*)o[0], (uint32_t*)o[1], c->stream);
     if (!rc && nv)
-        rc = check_hip(hipMemcpyAsync(vertices, dv.p, nv * sizeof(cth_vertex), hipMemcpyDeviceToHost, c->stream),
+        rc = check_hip(hipMemcpyAsync(vertices, o[0], nv * sizeof(cth_vertex), hipMemcpyDeviceToHost, c->stream),
                        "D2H");
     if (!rc && ni)
-        rc = check_hip(hipMemcpyAsync(indices, di.p, ni * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream), "D2H");
+        rc = check_hip(hipMemcpyAsync(indices, o[1], ni * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream), "D2H");
     int rs = check_hip(hipStreamSynchronize(c->stream), "sync");
     return rc ? rc : rs;
 }
@@ -545,24 +545,22 @@ static int host_query(cth_ctx* c, const cth_shape* s, const cth_point3* pts, siz
     if (!pts || !out) return fail(CTH_E_INVALID, "null buffer");
     if (n >= (size_t(1) << 31)) return fail(CTH_E_INVALID, "batch larger than 2^31");
     uint32_t m = (uint32_t)n;
-    DevBuf dp, dd, dout;
-    int rc = dp.alloc(n * 12);
-    if (!rc) rc = dd.alloc(n * 4);
     size_t out_bytes = (q == Q_BOUNDED) ? n * 8 : (q == Q_CONTAINS ? n : n * 4);
-    if (!rc && (q == Q_BOUNDED || q == Q_CONTAINS)) rc = dout.alloc(out_bytes);
+    void* b[3];  // points, distances, derived output
+    int rc = staging(*c, WS_HOST_IN, {n * 12, n * 4, (q == Q_BOUNDED || q == Q_CONTAINS) ? out_bytes : 0}, b);
     if (rc) return rc;
-    rc = check_hip(hipMemcpyAsync(dp.p, pts, n * 12, hipMemcpyHostToDevice, c->stream), "H2D");
+    rc = check_hip(hipMemcpyAsync(b[0], pts, n * 12, hipMemcpyHostToDevice, c->stream), "H2D");
     if (!rc) {
         if (q == Q_CONTAINS && s->kind == CTH_SHAPE_SPHERE) {
-            rc = launch_contains_sphere(*c, *s, (const float*)dp.p, m, (uint8_t*)dout.p, c->stream);
+            rc = launch_contains_sphere(*c, *s, (const float*)b[0], m, (uint8_t*)b[2], c->stream);
         } else {
-            rc = launch_de_points(*c, *s, (const float*)dp.p, nullptr, m, (float*)dd.p, nullptr, nullptr, nullptr,
+            rc = launch_de_points(*c, *s, (const float*)b[0], nullptr, m, (float*)b[1], nullptr, nullptr, nullptr,
                                   c->stream);
-            if (!rc && q == Q_CONTAINS) rc = launch_less_than_zero(*c, (const float*)dd.p, m, (uint8_t*)dout.p, c->stream);
-            if (!rc && q == Q_BOUNDED) rc = launch_dup_pairs(*c, (const float*)dd.p, m, (float*)dout.p, c->stream);
+            if (!rc && q == Q_CONTAINS) rc = launch_less_than_zero(*c, (const float*)b[1], m, (uint8_t*)b[2], c->stream);
+            if (!rc && q == Q_BOUNDED) rc = launch_dup_pairs(*c, (const float*)b[1], m, (float*)b[2], c->stream);
         }
     }
-    const void* src = (q == Q_MIN || q == Q_MAX) ? dd.p : dout.p;
+    const void* src = (q == Q_MIN || q == Q_MAX) ? b[1] : b[2];
     if (!rc) rc = check_hip(hipMemcpyAsync(out, src, out_bytes, hipMemcpyDeviceToHost, c->stream), "D2H");
     int rs = check_hip(hipStreamSynchronize(c->stream), "sync");
     return rc ? rc : rs;

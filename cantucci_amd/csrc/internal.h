@@ -28,7 +28,18 @@ struct LaunchShape {
 // Device scratch slots of a context, grown on demand and kept until the
 // context is destroyed (calls on one context never overlap, and the users
 // below finish with a stream synchronisation).
-enum ScratchSlot { WS_NODES_A = 0, WS_NODES_B, WS_FLAGS, WS_OFFS, WS_SCAN, WS_COUNTS, WS_UOFF, WS_SLOTS };
+enum ScratchSlot {
+    WS_NODES_A = 0,
+    WS_NODES_B,
+    WS_FLAGS,
+    WS_OFFS,
+    WS_SCAN,
+    WS_COUNTS,
+    WS_UOFF,
+    WS_HOST_IN,   // staging of the host-buffer entry points
+    WS_HOST_OUT,
+    WS_SLOTS
+};
 
 struct Ctx {
     int device = 0;
