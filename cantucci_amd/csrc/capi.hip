@@ -276,6 +276,23 @@ int cth_sphere_trace_dev(cth_ctx* c, const cth_shape* s, const float* origins, i
                              depth, steps, hit_pos, pick_stream(c, stream));
 }
 
+int cth_focus_split_dev(cth_ctx* c, const float* leaves, size_t n, const float* focii, const float* depth,
+                        size_t nfocii, const float eye[3], uint32_t* flags, void* stream) {
+    if (!c) return fail(CTH_E_INVALID, "null context");
+    if (!eye || (n && (!leaves || !flags)) || (nfocii && !focii)) return fail(CTH_E_INVALID, "null buffer");
+    if (n >= (size_t(1) << 31) || nfocii >= (size_t(1) << 31)) return fail(CTH_E_INVALID, "batch larger than 2^31");
+    return launch_focus_split(*c, leaves, (uint32_t)n, focii, depth, (uint32_t)nfocii, eye, flags,
+                              pick_stream(c, stream));
+}
+
+int cth_split_leaves_dev(cth_ctx* c, const float* leaves, size_t n, const uint32_t* flags, float* out, size_t cap,
+                         size_t* nout, void* stream) {
+    if (!c) return fail(CTH_E_INVALID, "null context");
+    if (!nout || (n && (!leaves || !flags))) return fail(CTH_E_INVALID, "null buffer");
+    if (n >= (size_t(1) << 29)) return fail(CTH_E_INVALID, "more than 2^29 leaves");
+    return split_leaves(*c, leaves, (uint32_t)n, flags, out, cap, nout, pick_stream(c, stream));
+}
+
 int cth_sphere_trace(cth_ctx* c, const cth_shape* s, const float* origins, int per_ray_origin, const float* dirs,
                      size_t n, uint32_t max_steps, float eps_abs, float eps_rel, float t_max, float* depth,
                      uint32_t* steps, float* hit_pos) {

@@ -140,6 +140,10 @@ int mesh_emit(Ctx& c, const cth_shape& s, const float* spans, const float* dists
               int normals, const uint64_t* voff, const uint64_t* ioff, uint64_t nverts, float* verts,
               uint32_t* idx, hipStream_t st);
 int launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t st);
+int launch_focus_split(Ctx& c, const float* leaves, uint32_t n, const float* focii, const float* depth, uint32_t nf,
+                       const float eye[3], uint32_t* flags, hipStream_t st);
+int split_leaves(Ctx& c, const float* leaves, uint32_t n, const uint32_t* flags, float* out, size_t cap,
+                 size_t* nout, hipStream_t st);
 int launch_contains_sphere(Ctx& c, const cth_shape& s, const float* pts, uint32_t n, uint8_t* out, hipStream_t st);
 int launch_less_than_zero(Ctx& c, const float* d, uint32_t n, uint8_t* out, hipStream_t st);
 int launch_dup_pairs(Ctx& c, const float* d, uint32_t n, float* pairs, hipStream_t st);

@@ -123,6 +123,32 @@ def mesh(shape: Shape, spans: torch.Tensor, dists: torch.Tensor, res: int, norma
     return voff, ioff, verts[:nv.value], idx[:ni.value]
 
 
+def focus_split(ctx: Context, leaves: torch.Tensor, focii: torch.Tensor, eye, depth: torch.Tensor | None = None):
+    """The reference's split rule (ShapeMesh::update): a uint32 flag per leaf,
+    1 where a focus point lies in the leaf and the eye is nearer than twice
+    the leaf's width.  `depth` (optional) skips focus rays that missed."""
+    leaves = _f32(leaves).contiguous()
+    focii = _f32(focii).contiguous()
+    flags = torch.empty(leaves.shape[0], dtype=torch.int32, device=leaves.device)
+    e = (C.c_float * 3)(*[float(v) for v in eye])
+    check(lib.cth_focus_split_dev(ctx.handle, _ptr(leaves), leaves.shape[0], _ptr(focii),
+                                  _ptr(depth.contiguous()) if depth is not None else None, focii.shape[0], e,
+                                  _ptr(flags), _stream()))
+    return flags
+
+
+def split_leaves(ctx: Context, leaves: torch.Tensor, flags: torch.Tensor) -> torch.Tensor:
+    """Replace flagged leaves by their 8 create_spans children, in place order."""
+    leaves = _f32(leaves).contiguous()
+    n = C.c_size_t()
+    check(lib.cth_split_leaves_dev(ctx.handle, _ptr(leaves), leaves.shape[0], _ptr(flags), None, 0, C.byref(n),
+                                   _stream()))
+    out = torch.empty((max(n.value, 1), 6), dtype=torch.float32, device=leaves.device)
+    check(lib.cth_split_leaves_dev(ctx.handle, _ptr(leaves), leaves.shape[0], _ptr(flags), _ptr(out), out.shape[0],
+                                   C.byref(n), _stream()))
+    return out[:n.value]
+
+
 def refine(shape: Shape, root, min_depth: int, max_depth: int, half_diag, cap: int | None = None,
            leaves: torch.Tensor | None = None, ctx: Context | None = None, device="cuda"):
     """Level-synchronous refinement; returns (leaves (m, 6) f32 on device, level_counts)."""

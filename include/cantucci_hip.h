@@ -14,6 +14,7 @@
  *   MeshBuffer steps 2-3 (vertices, normals, quads)     src/mesh/buffer.rs:97-372
  *   MeshBuffer::generate_for_box                        src/mesh/buffer.rs:30-42
  *   ShapeMesh::get_focii (sphere tracing)               src/mesh/mod.rs:205-243
+ *   ShapeMesh::update split rule, Octree::leaf_around  src/mesh/mod.rs:91-104, src/octree/mod.rs:75-89
  *
  * Conventions
  *   - Plain C types only.  Host-pointer entry points are synchronous (they use
@@ -218,6 +219,24 @@ int cth_sample_spans_dev(cth_ctx* ctx, const cth_shape* shape, const float* span
 int cth_refine_dev(cth_ctx* ctx, const cth_shape* shape, const float root[6], uint32_t min_depth, uint32_t max_depth,
                    const float* half_diag, float* leaves, size_t cap, size_t* nleaves, uint64_t* level_counts,
                    void* stream);
+
+/* The reference's own split rule (ShapeMesh::update, src/mesh/mod.rs:91-104)
+ * over a set of leaves that tiles an octree (n spans, 6 floats each, device):
+ * flags[i] = 1 when some focus point (nfocii x 3 floats, device) lies in leaf
+ * i's half-open span (SpanExt::contains, src/octree/mod.rs:25-31) and the
+ * eye is nearer to it than 2 |end.x - start.x|.  depth (nullable, nfocii)
+ * skips the focus points whose depth is not finite -- the rays get_focii
+ * drops (src/mesh/mod.rs:229-242); hit_pos and depth of
+ * cth_sphere_trace_dev can be passed straight in. */
+int cth_focus_split_dev(cth_ctx* ctx, const float* leaves, size_t n, const float* focii, const float* depth,
+                        size_t nfocii, const float eye[3], uint32_t* flags, void* stream);
+
+/* Replace every flagged leaf by its 8 children (create_spans order,
+ * src/octree/mod.rs:315-329) in place: out holds, for each leaf in order,
+ * either the leaf or its children.  *nout receives the span count; out
+ * (device, cap spans) may be NULL to query it.  Returns after *nout is known. */
+int cth_split_leaves_dev(cth_ctx* ctx, const float* leaves, size_t n, const uint32_t* flags, float* out, size_t cap,
+                         size_t* nout, void* stream);
 
 /* Sphere tracing of explicit rays: the marcher of ShapeMesh::get_focii
  * (src/mesh/mod.rs:229-241):  d = DE(pos); pos += dir * d; t += d; a ray hits

@@ -21,7 +21,7 @@ K_SPHERE = 2
 
 __all__ = [
     "OrcShape", "mandelbulb", "mandelbulb_trig", "sphere", "de_points", "de_points_f64", "de_grad_f64",
-    "de_grid", "sample_spans", "surface_nets", "mesh_batch", "create_spans", "refine", "ray_dirs", "sphere_trace", "fd_diff",
+    "de_grid", "sample_spans", "surface_nets", "mesh_batch", "focus_split", "create_spans", "refine", "ray_dirs", "sphere_trace", "fd_diff",
     "powi", "build", "K_MANDELBULB", "K_MANDELBULB_TRIG", "K_SPHERE", "default_threads",
 ]
 
@@ -65,6 +65,7 @@ for _name, _res, _args in [
     ("orc_surface_nets", None, [_SH, _P, _U, _P, _P, _S, _P, _S, C.POINTER(_S), C.POINTER(_S)]),
     ("orc_mesh_batch", None, [_SH, _P, _S, _U, _P, _I, _P, _P, _P, _S, _P, _S]),
     ("orc_create_spans", None, [_P, _P]),
+    ("orc_focus_split", None, [_P, _S, _P, _P, _S, _P, _P]),
     ("orc_refine", _S, [_SH, _P, _U, _U, _P, _P, _S, _P, _I]),
     ("orc_ray_dirs", None, [_P, _U, _U, _U, _U, _P]),
     ("orc_sphere_trace", None, [_SH, _P, _P, _S, _I, _U, _F, _F, _F, _P, _P, _P, _I]),
@@ -210,6 +211,18 @@ def mesh_batch(shape: OrcShape, spans, res: int, dists=None, threads=None):
     _lib.orc_mesh_batch(C.byref(shape), _ptr(sp), n, res, None if d is None else _ptr(d), t, _ptr(voff),
                         _ptr(ioff), _ptr(v), v.shape[0], _ptr(ix), ix.shape[0])
     return voff, ioff, v, ix
+
+
+def focus_split(leaves, focii, eye, depth=None) -> np.ndarray:
+    """ShapeMesh::update's split flags (src/mesh/mod.rs:91-104) for a flat leaf set."""
+    lv = _f32(leaves).reshape(-1, 6)
+    fc = _f32(focii).reshape(-1, 3)
+    dp = None if depth is None else _f32(depth).reshape(-1)
+    e = _f32(eye).reshape(3)
+    flags = np.zeros(lv.shape[0], np.uint32)
+    _lib.orc_focus_split(_ptr(lv), lv.shape[0], _ptr(fc), None if dp is None else _ptr(dp), fc.shape[0], _ptr(e),
+                         _ptr(flags))
+    return flags
 
 
 def create_spans(span) -> np.ndarray:

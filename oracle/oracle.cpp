@@ -827,6 +827,31 @@ void orc_mesh_batch(const orc_shape* shape, const float* spans, size_t nspans, u
             if (!I[i].empty()) std::memcpy(idx + ioff[i], I[i].data(), I[i].size() * 4);
 }
 
+// ShapeMesh::update's split rule (src/mesh/mod.rs:91-104) over a flat leaf
+// set: for each focus point (skipping rays whose depth is not finite, as
+// get_focii drops misses, :229-242) the leaf around it -- the leaf whose
+// half-open span contains it (Octree::leaf_around_mut + SpanExt::contains,
+// src/octree/mod.rs:25-31,75-89) -- splits when
+// camera.position.distance(focus) < 2 * |span.end.x - span.start.x|.
+void orc_focus_split(const float* leaves, size_t n, const float* focii, const float* depth, size_t nf,
+                     const float* eye, uint32_t* flags) {
+    for (size_t i = 0; i < n; ++i) flags[i] = 0;
+    for (size_t f = 0; f < nf; ++f) {
+        if (depth && !std::isfinite(depth[f])) continue;
+        const float* p = focii + 3 * f;
+        for (size_t i = 0; i < n; ++i) {
+            const float* s = leaves + 6 * i;
+            const bool in = s[0] <= p[0] && s[1] <= p[1] && s[2] <= p[2] && p[0] < s[3] && p[1] < s[4] && p[2] < s[5];
+            if (!in) continue;
+            const float dx = eye[0] - p[0], dy = eye[1] - p[1], dz = eye[2] - p[2];
+            const float dist = std::sqrt((dx * dx + dy * dy) + dz * dz);
+            const float threshold = 2.0f * std::fabs(s[3] - s[0]);
+            if (dist < threshold) flags[i] = 1;
+            break;  // the leaf around the point is unique
+        }
+    }
+}
+
 void orc_create_spans(const float* span, float* out48) {
     float o[8][6];
     create_spans(span, span + 3, o);

@@ -10,10 +10,3 @@ GOLDEN = Path(__file__).resolve().parent / "golden"
 # (src/shape/mod.rs:110-133), inputs without expected outputs.
 BENCH_POINTS = np.array(json.loads((GOLDEN / "bench_points.json").read_text())["points"], np.float32)
 
-
-def load_case(name: str) -> dict:
-    """A golden case: `<name>.json` manifest plus `<name>.npz` arrays."""
-    meta = json.loads((GOLDEN / f"{name}.json").read_text())
-    with np.load(GOLDEN / f"{name}.npz", allow_pickle=False) as z:
-        arrays = {k: z[k] for k in z.files}
-    return {"meta": meta, **arrays}

@@ -386,3 +386,28 @@ def test_refinement_level_counts_small():
     c = leaves[:, :3] + (leaves[:, 3:] - leaves[:, :3]) / F32(2)
     d = O.de_points(O.mandelbulb(15, 2.0), c)[0]
     assert ((np.abs(d) < half_diagonals(3.0, 5)[5]) | np.isnan(d)).all()
+
+
+def test_focus_split_rule_by_hand():
+    """ShapeMesh::update's rule (src/mesh/mod.rs:91-104) on the 64 leaves of a
+    twice-split octree: the leaf around a focus point (half-open spans) splits
+    when the eye is nearer than twice its width; strict <, boundaries go to
+    the upper leaf, points outside every leaf and missed rays change nothing."""
+    root = np.array([-1, -1, -1, 1, 1, 1], F32)
+    leaves = np.concatenate([O.create_spans(c) for c in O.create_spans(root)])
+    upper = int(np.where((leaves[:, :3] == 0).all(1))[0][0])  # [0, 0.5)^3
+    lower = int(np.where((leaves[:, 3:] == 0).all(1))[0][0])  # [-0.5, 0)^3
+    # (0.1, 0.2, 0.3) at distance 0.6 < 2 * 0.5 from the eye
+    f = O.focus_split(leaves, [[0.1, 0.2, 0.3]], [0.1, 0.2, 0.9])
+    assert f.nonzero()[0].tolist() == [upper]
+    # the origin lies on eight leaves' faces and belongs to [0, 0.5)^3 only
+    f = O.focus_split(leaves, [[0.0, 0.0, 0.0]], [0.0, 0.0, 0.5])
+    assert f.nonzero()[0].tolist() == [upper]
+    f = O.focus_split(leaves, [[-0.25, -0.25, -0.25]], [-0.25, -0.25, 0.5])
+    assert f.nonzero()[0].tolist() == [lower]
+    # exactly at the threshold (distance 1.0 = 2 * 0.5): no split
+    f = O.focus_split(leaves, [[0.25, 0.25, 0.0]], [0.25, 0.25, 1.0])
+    assert not f.any()
+    # outside the octree (half-open at +1) and a missed ray
+    assert not O.focus_split(leaves, [[1.0, 0.0, 0.0]], [1.0, 0.0, 0.1]).any()
+    assert not O.focus_split(leaves, [[0.1, 0.1, 0.1]], [0.1, 0.1, 0.2], depth=[np.inf]).any()
