@@ -611,18 +611,130 @@ struct SinkMeshVerts {
 
 // normal = normalize((DE(p+dx) - DE(p-dx), ...)) = g * (1 / |g|) (cgmath
 // normalize); distance_from_surface = DE(p) (src/mesh/buffer.rs:254-272).
-__global__ void k_mesh_fd_combine(const float* __restrict__ taps, uint32_t n, float* __restrict__ verts) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float* t = taps + 7 * (size_t)i;
+__device__ __forceinline__ void put_fd_normal(float* v, const float t[7]) {
     const float g[3] = {t[1] - t[2], t[3] - t[4], t[5] - t[6]};
     const float mg = sqrt_rn((g[0] * g[0] + g[1] * g[1]) + g[2] * g[2]);
     const float im = div_rn(1.0f, mg);
-    float* v = verts + 7 * (size_t)i;
     v[3] = g[0] * im;
     v[4] = g[1] * im;
     v[5] = g[2] * im;
     v[6] = t[0];
+}
+
+// The 6-tap stencil with the normal formed in registers: a lane evaluates
+// the 7 taps of one vertex one after another -- each tap restarts the lane
+// at the wave's batched refill, exactly like a new point of the lane-refill
+// loop (work_loop.h) -- keeps the tap distances and writes the normal and
+// distance_from_surface itself.  The arithmetic is that of SrcMeshTaps +
+// k_mesh_fd_combine; no tap buffer.  Index space: vertices.
+template <class Ev>
+__global__ __launch_bounds__(256) void k_mesh_taps(float* __restrict__ verts, MandelUniform u, WorkQueue q) {
+    Ev ev;
+    bool live = false, hold = false, has_v = false;
+    uint32_t vid = 0, t = 0, k = 0;
+    float r = 0.0f;
+    float p[3] = {0.0f, 0.0f, 0.0f}, dl[3] = {0.0f, 0.0f, 0.0f};
+    float acc[7] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    uint32_t c_next = 0, c_end = 0, round = 0;
+    bool exhausted = false;
+
+    for (;;) {
+        if ((uint32_t)__popcll(__ballot(!live)) >= q.refill_min) {
+            if (hold) {
+                const float d = ev.finish(r);
+#pragma unroll
+                for (int j = 0; j < 7; ++j) acc[j] = (t == (uint32_t)j) ? d : acc[j];
+                ++t;
+                hold = false;
+            }
+            for (;;) {
+                if (!live && has_v && t == 7u) {
+                    put_fd_normal(verts + 7 * (size_t)vid, acc);
+                    has_v = false;
+                }
+                const uint64_t need = __ballot(!live && !has_v);
+                if (need != 0 && !exhausted) {
+                    if (c_next >= c_end) {
+                        const uint32_t b = next_chunk(q, round);
+                        if (b >= q.total) {
+                            exhausted = true;
+                        } else {
+                            c_next = b;
+                            c_end = (q.total - b < q.chunk) ? q.total : b + q.chunk;
+                        }
+                    }
+                    if (!exhausted) {
+                        const uint32_t avail = c_end - c_next;
+                        const uint32_t nneed = (uint32_t)__popcll(need);
+                        if (!live && !has_v) {
+                            const uint32_t rank = lane_rank(need);
+                            if (rank < avail) {
+                                vid = c_next + rank;
+                                const float* rec = verts + 7 * (size_t)vid;
+                                p[0] = rec[0];
+                                p[1] = rec[1];
+                                p[2] = rec[2];
+                                dl[0] = rec[3];
+                                dl[1] = rec[4];
+                                dl[2] = rec[5];
+                                t = 0;
+                                has_v = true;
+                            }
+                        }
+                        c_next += (nneed < avail) ? nneed : avail;
+                    }
+                }
+                const bool start = !live && has_v;
+                if (__ballot(start) == 0) {
+                    if (exhausted || __ballot(!live && !has_v) == 0) break;
+                    continue;
+                }
+                if (start) {
+                    // tap t: p, or p + unit_a * (+-delta_a) (src/mesh/buffer.rs:259-264)
+                    float c[3] = {p[0], p[1], p[2]};
+                    if (t > 0) {
+                        const int a = (int)(t - 1) >> 1;
+                        const float da = (a == 0) ? dl[0] : ((a == 1) ? dl[1] : dl[2]);
+                        const float h = (t & 1u) ? da : -da;
+#pragma unroll
+                        for (int b = 0; b < 3; ++b) c[b] = c[b] + ((a == b) ? 1.0f : 0.0f) * h;
+                    }
+                    ev.start(c[0], c[1], c[2], 0.0f, u);
+                    k = 0;
+                    r = ev.mag();
+                    if (r > u.bailout) {
+                        const float d = ev.finish(r);
+#pragma unroll
+                        for (int j = 0; j < 7; ++j) acc[j] = (t == (uint32_t)j) ? d : acc[j];
+                        ++t;
+                    } else {
+                        live = true;
+                    }
+                }
+            }
+            if (__ballot(live) == 0) break;
+        }
+        ev.body(r, u, live);
+        k += live ? 1u : 0u;
+        const bool cap = live && (k == u.max_iters);
+        const float rn = ev.mag();
+        const bool bail = live && !cap && (rn > u.bailout);
+        // at the cap r stays the magnitude before the last body (src/shape/mandelbulb.rs:65-78)
+        if (live && !cap) r = rn;
+        if (cap || bail) {
+            live = false;
+            hold = true;
+        }
+    }
+    if (q.dynamic) wave_retire(q);
+}
+
+__global__ void k_mesh_fd_combine(const float* __restrict__ taps, uint32_t n, float* __restrict__ verts) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* t = taps + 7 * (size_t)i;
+    const float tv[7] = {t[0], t[1], t[2], t[3], t[4], t[5], t[6]};
+    put_fd_normal(verts + 7 * (size_t)i, tv);
 }
 
 inline uint32_t cdiv_u32(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
@@ -786,8 +898,25 @@ int mesh_emit(Ctx& c, const cth_shape& s, const float* spans, const float* dists
         return check_hip(hipGetLastError(), "mesh tangent normals launch");
     }
 
-    // 6-tap central differences: 7 evaluations per vertex, in batches whose
-    // index space stays below 2^31.
+    // 6-tap central differences, 7 evaluations per vertex.  Mandelbulb kinds:
+    // one launch, the lane forms the normal from its own taps.
+    if (family != FAM_SPHERE) {
+        if (nverts >= (1ull << 31)) return fail(CTH_E_INVALID, "more than 2^31 vertices");
+        uint32_t blocks = 0;
+        WorkQueue q = make_queue(c, (uint32_t)nverts, c.shape_mesh, blocks);
+        dim3 grid(blocks), block(c.shape_mesh.block);
+        if (family == FAM_P8) {
+            if (c.mode == CTH_MODE_FAST)
+                hipLaunchKernelGGL((k_mesh_taps<P8Fast>), grid, block, 0, st, verts, u, q);
+            else
+                hipLaunchKernelGGL((k_mesh_taps<P8Exact>), grid, block, 0, st, verts, u, q);
+        } else {
+            hipLaunchKernelGGL((k_mesh_taps<TrigExact>), grid, block, 0, st, verts, u, q);
+        }
+        return check_hip(hipGetLastError(), "mesh taps launch");
+    }
+    // Sphere: one thread per tap, then the combine, in batches whose index
+    // space stays below 2^31.
     const uint64_t batch = std::min<uint64_t>(nverts, ((1ull << 31) - 1) / 7);
     float* taps = nullptr;
     rc = check_hip(hipMallocAsync((void**)&taps, batch * 7 * sizeof(float), st), "hipMallocAsync taps");
@@ -795,14 +924,9 @@ int mesh_emit(Ctx& c, const cth_shape& s, const float* spans, const float* dists
         const uint32_t nb = (uint32_t)std::min<uint64_t>(batch, nverts - off);
         float* vb = verts + 7 * off;
         SrcMeshTaps src{vb};
-        SinkDist sink{taps, nullptr};
-        if (family == FAM_SPHERE) {
-            hipLaunchKernelGGL((k_sphere<SrcMeshTaps>), dim3(cdiv_u32((uint64_t)nb * 7, 256)), dim3(256), 0, st, src,
-                               nb * 7, s.center[0], s.center[1], s.center[2], s.radius, taps, nullptr);
-            rc = check_hip(hipGetLastError(), "mesh taps launch");
-        } else {
-            rc = run_refill(c, family, u, src, sink, nb * 7, st, c.shape_mesh);
-        }
+        hipLaunchKernelGGL((k_sphere<SrcMeshTaps>), dim3(cdiv_u32((uint64_t)nb * 7, 256)), dim3(256), 0, st, src,
+                           nb * 7, s.center[0], s.center[1], s.center[2], s.radius, taps, nullptr);
+        rc = check_hip(hipGetLastError(), "mesh taps launch");
         if (rc) break;
         hipLaunchKernelGGL(k_mesh_fd_combine, dim3(cdiv_u32(nb, 256)), dim3(256), 0, st, taps, nb, vb);
         rc = check_hip(hipGetLastError(), "mesh normals launch");
