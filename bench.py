@@ -176,7 +176,8 @@ def main() -> None:
                "sample": f"{m2} of {nleaves} depth-8 surface leaves (seeded): 9^3 corners + 7 per vertex, "
                          f"whole surface nets, oracle restatement, {threads} threads, {dt:.1f} s",
                "value_1thread": (len(one) * per + 7 * int(vo1[-1])) / dt1,
-               "sample_1thread": f"the first {len(one)} leaves of that sample on one thread, {dt1:.2f} s"}
+               "sample_1thread": f"the first {len(one)} leaves of that sample on one thread, {dt1:.2f} s",
+               "host": _host_cpu()}
 
     extra = {}
     for name, fn in (("c1_grid", bench_c1), ("c3_trace", bench_c3), ("c4_normals", bench_c4), ("c5_jobs", bench_c5),
@@ -221,6 +222,19 @@ def main() -> None:
         "extra": extra,
     }
     print(json.dumps(line))
+
+
+def _host_cpu() -> str:
+    """CPU model and logical CPU count of the host the baseline ran on."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return f"{model}, {os.cpu_count()} logical CPUs (threads used: see cores)"
 
 
 def _phases(dev, torch, shape, c2, hd, leaves_buf, dists, cross, res):
