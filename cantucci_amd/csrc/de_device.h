@@ -34,6 +34,11 @@ namespace cth {
 // Bit-level helpers
 // ---------------------------------------------------------------------------
 
+// Lanes of the wave (active ones only) where `p` holds.  The builtin keeps
+// the predicate in its scalar lane mask; HIP's __ballot first widens it to a
+// vector register and compares it again, two VALU instructions per call.
+__device__ __forceinline__ uint64_t wave_mask(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
 // `Vec3::is_on_z_axis` (src/shape/mandelbulb.rs:421-426): x and y both +-0.
 __device__ __forceinline__ bool on_z_axis(float x, float y) {
     return ((__float_as_uint(x) | __float_as_uint(y)) & 0x7fffffffu) == 0u;
@@ -55,7 +60,7 @@ __device__ __forceinline__ float div_rn(float a, float b) { return a / b; }
 // passes 0 and inf through.  NaN, zero, negative and tiny arguments take the
 // full sequence.
 __device__ __forceinline__ float sqrt_rn_wave(float x) {
-    if (__builtin_expect(__ballot(!(x >= 0x1p-96f)) == 0, 1)) {
+    if (__builtin_expect(wave_mask(!(x >= 0x1p-96f)) == 0, 1)) {
         const float s = __builtin_amdgcn_sqrtf(x);
         const float sd = __uint_as_float(__float_as_uint(s) - 1u);
         const float su = __uint_as_float(__float_as_uint(s) + 1u);
@@ -199,9 +204,13 @@ struct P8Exact {
         float nx = a * ((((x8 - (28.0f * x6) * y2) + (70.0f * x4) * y4) - (28.0f * x2) * y6) - y8);
         float ny = (((8.0f * a) * x) * y) * (((x6 - (7.0f * x4) * y2) + (7.0f * x2) * y4) - y6);
         float nz = (((8.0f * z) * sqrt_rn_wave(w2)) * (z2 - w2)) * ((z4 - (6.0f * z2) * w2) + w4);
-        bool axis = live && on_z_axis(x, y);
-        if (__ballot(axis)) {
-            if (axis) {
+        // The wave test looks at every lane (a compare's lane mask is free;
+        // folding `live` in first would cost two VALU instructions); a lane
+        // without a live point that happens to sit on the axis only sends the
+        // wave through the branch, where it does nothing.
+        const bool axis = on_z_axis(x, y);
+        if (wave_mask(axis)) {
+            if (live && axis) {
                 // rotate_on_z_axis::<8> (:114-126)
                 float c = div_rn(z, r);
                 if (__builtin_isinf(c)) pos_nan = true;

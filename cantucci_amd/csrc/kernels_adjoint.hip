@@ -49,7 +49,7 @@ __global__ __launch_bounds__(256) void k_adj_forward(SrcPoints src, MandelUnifor
     bool exhausted = false;
     const size_t stride = (size_t)u.max_iters + 1;
     for (;;) {
-        if ((uint32_t)__popcll(__ballot(!live)) >= q.refill_min) {
+        if ((uint32_t)__popcll(wave_mask(!live)) >= q.refill_min) {
             if (hold) {
                 float d = ev.finish(r);
                 if (dout) dout[idx] = d;
@@ -59,7 +59,7 @@ __global__ __launch_bounds__(256) void k_adj_forward(SrcPoints src, MandelUnifor
                 hold = false;
             }
             while (!exhausted) {
-                uint64_t need = __ballot(!live);
+                uint64_t need = wave_mask(!live);
                 if (need == 0) break;
                 if (c_next >= c_end) {
                     uint32_t b = next_chunk(q, round);
@@ -92,7 +92,7 @@ __global__ __launch_bounds__(256) void k_adj_forward(SrcPoints src, MandelUnifor
                 }
                 c_next += (nneed < avail) ? nneed : avail;
             }
-            if (__ballot(live) == 0) break;
+            if (wave_mask(live) == 0) break;
         }
         if (live) orbit[(size_t)idx * stride + k] = make_float4(ev.x, ev.y, ev.z, ev.dr);
         ev.body(r, u, live);
@@ -278,9 +278,9 @@ __global__ __launch_bounds__(256) void k_adj_reverse(SrcPoints src, MandelUnifor
     bool exhausted = false;
     const size_t stride = (size_t)u.max_iters + 1;
     for (;;) {
-        if ((uint32_t)__popcll(__ballot(!live)) >= q.refill_min) {
+        if ((uint32_t)__popcll(wave_mask(!live)) >= q.refill_min) {
             while (!exhausted) {
-                uint64_t need = __ballot(!live);
+                uint64_t need = wave_mask(!live);
                 if (need == 0) break;
                 if (c_next >= c_end) {
                     uint32_t b = next_chunk(q, round);
@@ -331,7 +331,7 @@ __global__ __launch_bounds__(256) void k_adj_reverse(SrcPoints src, MandelUnifor
                 }
                 c_next += (nneed < avail) ? nneed : avail;
             }
-            if (__ballot(live) == 0) break;
+            if (wave_mask(live) == 0) break;
         }
         if (live) {
             float4 s = orbit[(size_t)idx * stride + j];

@@ -41,7 +41,7 @@ __global__ __launch_bounds__(256) void k_cross_bits(const float* __restrict__ di
         }
         crosses = acc != 0;
     }
-    uint64_t bits = __ballot(crosses);
+    uint64_t bits = wave_mask(crosses);
     if (lane == 0) cross[wave] = bits;
 }
 

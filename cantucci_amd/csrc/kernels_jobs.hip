@@ -60,26 +60,26 @@ __global__ __launch_bounds__(256) void k_jobs(const JobDev* __restrict__ jobs, u
     SinkGrad sink{dists, normals, nullptr, nullptr, nullptr, 0u};
 
     for (;;) {
-        if ((uint32_t)__popcll(__ballot(!live)) >= q.refill_min) {
+        if ((uint32_t)__popcll(wave_mask(!live)) >= q.refill_min) {
             // write back held points, then stamp their jobs (one atomic per job per wave)
             if (hold) {
                 sink.max_iters = kcap;
                 sink.put(idx, ev.finish(r), k, r, ev.dr, ev);
             }
-            uint64_t pend = __ballot(hold);
+            uint64_t pend = wave_mask(hold);
             if (pend) {
                 unsigned long long now = wall_clock64();
                 while (pend) {
                     int lead = __ffsll((unsigned long long)pend) - 1;
                     uint32_t jl = (uint32_t)__builtin_amdgcn_readlane((int)job, lead);
-                    uint64_t same = __ballot(hold && job == jl);
+                    uint64_t same = wave_mask(hold && job == jl);
                     if ((int)__lane_id() == lead) atomicMax(done + jl, now);
                     pend &= ~same;
                 }
             }
             hold = false;
             while (!exhausted) {
-                uint64_t need = __ballot(!live);
+                uint64_t need = wave_mask(!live);
                 if (need == 0) break;
                 if (c_next >= c_end) {
                     uint32_t b = next_chunk(q, round);
@@ -130,7 +130,7 @@ __global__ __launch_bounds__(256) void k_jobs(const JobDev* __restrict__ jobs, u
                 }
                 c_next += (nneed < avail) ? nneed : avail;
             }
-            if (__ballot(live) == 0) break;
+            if (wave_mask(live) == 0) break;
         }
         ev.body(r, u, live);
         k += live ? 1u : 0u;

@@ -164,7 +164,7 @@ __global__ __launch_bounds__(kMeshBlock) void k_mesh_count_p(const float* __rest
         const bool cr = crossing(mk);
         nv += cr ? 1u : 0u;
         nf += (uint32_t)__popc(face_bits(mk, x, y, z));
-        const uint64_t bits = __ballot(cr);
+        const uint64_t bits = wave_mask(cr);
         if (cross && (threadIdx.x & 63u) == 0) cross[(size_t)leaf * nwords + ((x * pc + j) >> 6)] = bits;
     }
     nv = block_sum(nv, wsum);
@@ -314,7 +314,7 @@ __global__ __launch_bounds__(kMeshBlock) void k_mesh_emit_p(
         const uint32_t j = (ww << 6) | lane;
         float d[8];
         const bool cr = crossing(load_cell(D, m.n1, px, j >> m.lg, j & (m.res - 1u), d));
-        const uint64_t bits = __ballot(cr);
+        const uint64_t bits = wave_mask(cr);
         if (lane == 0) ((w < pw) ? wc : wp)[ww] = bits;
     }
     __syncthreads();
@@ -630,7 +630,12 @@ __device__ __forceinline__ void put_fd_normal(float* v, const float t[7]) {
 template <class Ev>
 __global__ __launch_bounds__(256) void k_mesh_taps(float* __restrict__ verts, MandelUniform u, WorkQueue q) {
     Ev ev;
-    bool live = false, hold = false, has_v = false;
+    // Lane state as in refill_loop (work_loop.h): live / holding lanes are
+    // wave-uniform lane masks; `has_v` (the lane owns a vertex whose taps are
+    // not all evaluated) is only read at the batched refill.
+    const uint64_t wave = wave_mask(true);
+    uint64_t live_m = 0, hold_m = 0;
+    bool has_v = false;
     uint32_t vid = 0, t = 0, k = 0;
     float r = 0.0f;
     float p[3] = {0.0f, 0.0f, 0.0f}, dl[3] = {0.0f, 0.0f, 0.0f};
@@ -639,20 +644,21 @@ __global__ __launch_bounds__(256) void k_mesh_taps(float* __restrict__ verts, Ma
     bool exhausted = false;
 
     for (;;) {
-        if ((uint32_t)__popcll(__ballot(!live)) >= q.refill_min) {
-            if (hold) {
+        if ((uint32_t)__popcll(wave & ~live_m) >= q.refill_min) {
+            if (__builtin_amdgcn_inverse_ballot_w64(hold_m)) {
                 const float d = ev.finish(r);
 #pragma unroll
                 for (int j = 0; j < 7; ++j) acc[j] = (t == (uint32_t)j) ? d : acc[j];
                 ++t;
-                hold = false;
             }
+            hold_m = 0;
             for (;;) {
-                if (!live && has_v && t == 7u) {
+                // every lane here is idle; a lane with all 7 taps writes its vertex
+                if (has_v && t == 7u && !__builtin_amdgcn_inverse_ballot_w64(live_m)) {
                     put_fd_normal(verts + 7 * (size_t)vid, acc);
                     has_v = false;
                 }
-                const uint64_t need = __ballot(!live && !has_v);
+                const uint64_t need = wave_mask(!has_v) & ~live_m;
                 if (need != 0 && !exhausted) {
                     if (c_next >= c_end) {
                         const uint32_t b = next_chunk(q, round);
@@ -666,30 +672,28 @@ __global__ __launch_bounds__(256) void k_mesh_taps(float* __restrict__ verts, Ma
                     if (!exhausted) {
                         const uint32_t avail = c_end - c_next;
                         const uint32_t nneed = (uint32_t)__popcll(need);
-                        if (!live && !has_v) {
-                            const uint32_t rank = lane_rank(need);
-                            if (rank < avail) {
-                                vid = c_next + rank;
-                                const float* rec = verts + 7 * (size_t)vid;
-                                p[0] = rec[0];
-                                p[1] = rec[1];
-                                p[2] = rec[2];
-                                dl[0] = rec[3];
-                                dl[1] = rec[4];
-                                dl[2] = rec[5];
-                                t = 0;
-                                has_v = true;
-                            }
+                        const uint32_t rank = lane_rank(need);
+                        if (__builtin_amdgcn_inverse_ballot_w64(need) && rank < avail) {
+                            vid = c_next + rank;
+                            const float* rec = verts + 7 * (size_t)vid;
+                            p[0] = rec[0];
+                            p[1] = rec[1];
+                            p[2] = rec[2];
+                            dl[0] = rec[3];
+                            dl[1] = rec[4];
+                            dl[2] = rec[5];
+                            t = 0;
+                            has_v = true;
                         }
                         c_next += (nneed < avail) ? nneed : avail;
                     }
                 }
-                const bool start = !live && has_v;
-                if (__ballot(start) == 0) {
-                    if (exhausted || __ballot(!live && !has_v) == 0) break;
+                const uint64_t start_m = wave_mask(has_v) & ~live_m;
+                if (start_m == 0) {
+                    if (exhausted || (wave_mask(!has_v) & ~live_m) == 0) break;
                     continue;
                 }
-                if (start) {
+                if (__builtin_amdgcn_inverse_ballot_w64(start_m)) {
                     // tap t: p, or p + unit_a * (+-delta_a) (src/mesh/buffer.rs:259-264)
                     float c[3] = {p[0], p[1], p[2]};
                     if (t > 0) {
@@ -707,24 +711,22 @@ __global__ __launch_bounds__(256) void k_mesh_taps(float* __restrict__ verts, Ma
 #pragma unroll
                         for (int j = 0; j < 7; ++j) acc[j] = (t == (uint32_t)j) ? d : acc[j];
                         ++t;
-                    } else {
-                        live = true;
                     }
                 }
+                live_m |= start_m & ~wave_mask(r > u.bailout);
             }
-            if (__ballot(live) == 0) break;
+            if (live_m == 0) break;
         }
+        const bool live = __builtin_amdgcn_inverse_ballot_w64(live_m);
         ev.body(r, u, live);
         k += live ? 1u : 0u;
-        const bool cap = live && (k == u.max_iters);
+        const uint64_t cap_m = wave_mask(k == u.max_iters) & live_m;
         const float rn = ev.mag();
-        const bool bail = live && !cap && (rn > u.bailout);
+        const uint64_t bail_m = wave_mask(rn > u.bailout) & live_m & ~cap_m;
         // at the cap r stays the magnitude before the last body (src/shape/mandelbulb.rs:65-78)
-        if (live && !cap) r = rn;
-        if (cap || bail) {
-            live = false;
-            hold = true;
-        }
+        if (__builtin_amdgcn_inverse_ballot_w64(live_m & ~cap_m)) r = rn;
+        live_m &= ~(cap_m | bail_m);
+        hold_m |= cap_m | bail_m;
     }
     if (q.dynamic) wave_retire(q);
 }

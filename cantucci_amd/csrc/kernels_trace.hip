@@ -92,7 +92,7 @@ __global__ __launch_bounds__(256) void k_trace(Rays rays, MandelUniform u, Trace
     bool exhausted = false;
 
     for (;;) {
-        if ((uint32_t)__popcll(__ballot(!live)) >= q.refill_min) {
+        if ((uint32_t)__popcll(wave_mask(!live)) >= q.refill_min) {
             for (;;) {
                 // Apply finished evaluations: one sphere-tracing step.
                 if (pending) {
@@ -117,7 +117,7 @@ __global__ __launch_bounds__(256) void k_trace(Rays rays, MandelUniform u, Trace
                 }
                 // New rays for lanes without one.
                 if (!exhausted) {
-                    uint64_t need = __ballot(!has_ray);
+                    uint64_t need = wave_mask(!has_ray);
                     if (need) {
                         if (c_next >= c_end) {
                             uint32_t b = next_chunk(q, round);
@@ -155,10 +155,10 @@ __global__ __launch_bounds__(256) void k_trace(Rays rays, MandelUniform u, Trace
                     else
                         live = true;
                 }
-                bool more = __ballot(pending) != 0 || (!exhausted && __ballot(!has_ray) != 0);
+                bool more = wave_mask(pending) != 0 || (!exhausted && wave_mask(!has_ray) != 0);
                 if (!more) break;
             }
-            if (__ballot(has_ray) == 0) break;
+            if (wave_mask(has_ray) == 0) break;
         }
         ev.body(r, u, live);
         k += live ? 1u : 0u;

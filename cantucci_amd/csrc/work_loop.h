@@ -34,15 +34,18 @@ __device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
 
 // One atomic per wave; the value is broadcast from the first active lane.
 __device__ __forceinline__ uint32_t wave_claim(uint32_t* ctr, uint32_t amount) {
-    uint64_t active = __ballot(1);
+    uint64_t active = wave_mask(1);
     int leader = __ffsll((unsigned long long)active) - 1;
     uint32_t v = 0;
     if ((int)__lane_id() == leader) v = atomicAdd(ctr, amount);
     return (uint32_t)__builtin_amdgcn_readlane((int)v, leader);
 }
 
-// Wave index within the launch.
-__device__ __forceinline__ uint32_t wave_id() { return blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); }
+// Wave index within the launch (read from the first lane so that the compiler
+// knows it is wave-uniform and keeps the chunk window in scalar registers).
+__device__ __forceinline__ uint32_t wave_id() {
+    return blockIdx.x * (blockDim.x >> 6) + (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+}
 
 // Next chunk of the wave: either the next unclaimed chunk of the global
 // counter (dynamic) or, statically, chunks w, w + W, w + 2W, ... of the
@@ -58,7 +61,7 @@ __device__ __forceinline__ uint32_t next_chunk(const WorkQueue& q, uint32_t& rou
 // Called once per wave after its last claim; the final wave zeroes both
 // counters.  Every other wave has already made its final claim by then.
 __device__ __forceinline__ void wave_retire(const WorkQueue& q) {
-    uint64_t active = __ballot(1);
+    uint64_t active = wave_mask(1);
     int leader = __ffsll((unsigned long long)active) - 1;
     if ((int)__lane_id() == leader) {
         uint32_t prev = atomicAdd(q.done, 1u);
@@ -79,12 +82,19 @@ __device__ __forceinline__ void wave_retire(const WorkQueue& q) {
 // least `q.refill_min` lanes of the wave are idle; then the held points are
 // written and the idle lanes take new indices.  Batching the refill keeps the
 // per-tick control flow to a few scalar instructions.
+//
+// Which lanes are live or hold a finished point is kept in wave-uniform lane
+// masks (scalar registers).  A compare's lane mask is free to combine with
+// them, and the per-lane view is the same mask
+// (__builtin_amdgcn_inverse_ballot_w64), so the tick spends no vector
+// instructions on bookkeeping beyond the iteration counter.
 template <class Ev, class Src, class Sink>
 __device__ __forceinline__ void refill_loop(const Src& src, const MandelUniform& u, const Sink& sink,
                                             const WorkQueue& q) {
     Ev ev;
-    bool live = false;  // the lane's point needs another loop body
-    bool hold = false;  // the lane holds a finished point not yet written
+    const uint64_t wave = wave_mask(true);  // the wave's lanes
+    uint64_t live_m = 0;  // lanes whose point needs another loop body
+    uint64_t hold_m = 0;  // lanes holding a finished point not yet written
     uint32_t idx = 0, k = 0;
     float r = 0.0f;
     uint32_t c_next = 0, c_end = 0;  // wave-uniform window of claimed indices
@@ -92,17 +102,14 @@ __device__ __forceinline__ void refill_loop(const Src& src, const MandelUniform&
     bool exhausted = false;
 
     for (;;) {
-        uint64_t idle = __ballot(!live);
-        if ((uint32_t)__popcll(idle) >= q.refill_min) {
-            if (hold) {
-                sink.put(idx, ev.finish(r), k, r, ev.dr, ev);
-                hold = false;
-            }
+        if ((uint32_t)__popcll(wave & ~live_m) >= q.refill_min) {
+            if (__builtin_amdgcn_inverse_ballot_w64(hold_m)) sink.put(idx, ev.finish(r), k, r, ev.dr, ev);
+            hold_m = 0;
             // Hand new points to idle lanes.  A point that bails before its
             // first body (|p| > bailout) is written on the spot and the lane
             // refilled again.
             while (!exhausted) {
-                uint64_t need = __ballot(!live);
+                const uint64_t need = wave & ~live_m;
                 if (need == 0) break;
                 if (c_next >= c_end) {
                     uint32_t b = next_chunk(q, round);
@@ -113,40 +120,37 @@ __device__ __forceinline__ void refill_loop(const Src& src, const MandelUniform&
                     c_next = b;
                     c_end = (q.total - b < q.chunk) ? q.total : b + q.chunk;
                 }
-                uint32_t avail = c_end - c_next;
-                uint32_t nneed = (uint32_t)__popcll(need);
-                if (!live) {
-                    uint32_t rank = lane_rank(need);
-                    if (rank < avail) {
-                        idx = c_next + rank;
-                        float px, py, pz, pw;
-                        src.get(idx, px, py, pz, pw);
-                        ev.start(px, py, pz, pw, u);
-                        k = 0;
-                        r = ev.mag();
-                        if (r > u.bailout) {
-                            sink.put(idx, ev.finish(r), 0u, r, ev.dr, ev);
-                        } else {
-                            live = true;
-                        }
-                    }
+                const uint32_t avail = c_end - c_next;
+                const uint32_t nneed = (uint32_t)__popcll(need);
+                const uint32_t rank = lane_rank(need);
+                const bool take = __builtin_amdgcn_inverse_ballot_w64(need) && rank < avail;
+                if (take) {
+                    idx = c_next + rank;
+                    float px, py, pz, pw;
+                    src.get(idx, px, py, pz, pw);
+                    ev.start(px, py, pz, pw, u);
+                    k = 0;
+                    r = ev.mag();
                 }
+                const uint64_t took = wave_mask(take);
+                const uint64_t bailed = wave_mask(r > u.bailout) & took;
+                if (__builtin_amdgcn_inverse_ballot_w64(bailed)) sink.put(idx, ev.finish(r), 0u, r, ev.dr, ev);
+                live_m |= took & ~bailed;
                 c_next += (nneed < avail) ? nneed : avail;
             }
-            if (__ballot(live) == 0) break;
+            if (live_m == 0) break;
         }
+        const bool live = __builtin_amdgcn_inverse_ballot_w64(live_m);
         ev.body(r, u, live);
         k += live ? 1u : 0u;
-        bool cap = live && (k == u.max_iters);
-        float rn = ev.mag();
-        bool bail = live && !cap && (rn > u.bailout);
+        const uint64_t cap_m = wave_mask(k == u.max_iters) & live_m;
+        const float rn = ev.mag();
+        const uint64_t bail_m = wave_mask(rn > u.bailout) & live_m & ~cap_m;
         // At the iteration cap, r stays the magnitude computed before the last
         // body (the reference's off-by-one, src/shape/mandelbulb.rs:65-78).
-        if (live && !cap) r = rn;
-        if (cap || bail) {
-            live = false;
-            hold = true;
-        }
+        if (__builtin_amdgcn_inverse_ballot_w64(live_m & ~cap_m)) r = rn;
+        live_m &= ~(cap_m | bail_m);
+        hold_m |= cap_m | bail_m;
     }
     if (q.dynamic) wave_retire(q);
 }
