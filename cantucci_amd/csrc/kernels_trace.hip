@@ -82,8 +82,11 @@ template <class Ev, class Rays>
 __global__ __launch_bounds__(256) void k_trace(Rays rays, MandelUniform u, TraceParams tp, TraceOut out,
                                                WorkQueue q) {
     Ev ev;
-    bool live = false;     // an evaluation of this lane's ray is in progress
-    bool pending = false;  // an evaluation finished and its step is not applied yet
+    // Wave-uniform lane masks, as in refill_loop (work_loop.h): lanes with an
+    // evaluation in progress, and lanes whose evaluation finished and whose
+    // step is not applied yet.  `has_ray` is only read at the batched refill.
+    const uint64_t wave = wave_mask(true);
+    uint64_t live_m = 0, pend_m = 0;
     bool has_ray = false;
     uint32_t idx = 0, k = 0, nsteps = 0;
     float r = 0.0f, t = 0.0f;
@@ -92,11 +95,10 @@ __global__ __launch_bounds__(256) void k_trace(Rays rays, MandelUniform u, Trace
     bool exhausted = false;
 
     for (;;) {
-        if ((uint32_t)__popcll(wave_mask(!live)) >= q.refill_min) {
+        if ((uint32_t)__popcll(wave & ~live_m) >= q.refill_min) {
             for (;;) {
                 // Apply finished evaluations: one sphere-tracing step.
-                if (pending) {
-                    pending = false;
+                if (__builtin_amdgcn_inverse_ballot_w64(pend_m)) {
                     float d = ev.finish(r);
                     pos[0] = pos[0] + dir[0] * d;
                     pos[1] = pos[1] + dir[1] * d;
@@ -115,6 +117,7 @@ __global__ __launch_bounds__(256) void k_trace(Rays rays, MandelUniform u, Trace
                         has_ray = false;
                     }
                 }
+                pend_m = 0;
                 // New rays for lanes without one.
                 if (!exhausted) {
                     uint64_t need = wave_mask(!has_ray);
@@ -131,45 +134,42 @@ __global__ __launch_bounds__(256) void k_trace(Rays rays, MandelUniform u, Trace
                         if (!exhausted) {
                             uint32_t avail = c_end - c_next;
                             uint32_t nneed = (uint32_t)__popcll(need);
-                            if (!has_ray) {
-                                uint32_t rank = lane_rank(need);
-                                if (rank < avail) {
-                                    idx = c_next + rank;
-                                    rays.get(idx, pos, dir);
-                                    t = 0.0f;
-                                    nsteps = 0;
-                                    has_ray = true;
-                                }
+                            uint32_t rank = lane_rank(need);
+                            if (!has_ray && rank < avail) {
+                                idx = c_next + rank;
+                                rays.get(idx, pos, dir);
+                                t = 0.0f;
+                                nsteps = 0;
+                                has_ray = true;
                             }
                             c_next += (nneed < avail) ? nneed : avail;
                         }
                     }
                 }
                 // Start the next evaluation of every ray that has none running.
-                if (has_ray && !live) {
+                const uint64_t start_m = wave_mask(has_ray) & ~live_m;
+                if (__builtin_amdgcn_inverse_ballot_w64(start_m)) {
                     ev.start(pos[0], pos[1], pos[2], u.power, u);
                     k = 0;
                     r = ev.mag();
-                    if (r > u.bailout)
-                        pending = true;  // zero loop bodies: finished already
-                    else
-                        live = true;
                 }
-                bool more = wave_mask(pending) != 0 || (!exhausted && wave_mask(!has_ray) != 0);
+                // zero loop bodies (|p| > bailout): finished already
+                pend_m = wave_mask(r > u.bailout) & start_m;
+                live_m |= start_m & ~pend_m;
+                const bool more = pend_m != 0 || (!exhausted && wave_mask(!has_ray) != 0);
                 if (!more) break;
             }
-            if (wave_mask(has_ray) == 0) break;
+            if (live_m == 0) break;
         }
+        const bool live = __builtin_amdgcn_inverse_ballot_w64(live_m);
         ev.body(r, u, live);
         k += live ? 1u : 0u;
-        bool cap = live && (k == u.max_iters);
-        float rn = ev.mag();
-        bool bail = live && !cap && (rn > u.bailout);
-        if (live && !cap) r = rn;
-        if (cap || bail) {
-            live = false;
-            pending = true;
-        }
+        const uint64_t cap_m = wave_mask(k == u.max_iters) & live_m;
+        const float rn = ev.mag();
+        const uint64_t bail_m = wave_mask(rn > u.bailout) & live_m & ~cap_m;
+        if (__builtin_amdgcn_inverse_ballot_w64(live_m & ~cap_m)) r = rn;
+        live_m &= ~(cap_m | bail_m);
+        pend_m |= cap_m | bail_m;
     }
     if (q.dynamic) wave_retire(q);
 }
