@@ -77,17 +77,18 @@ __device__ __forceinline__ float div_rn_unscaled(float a, float b) {
 // below 2^-96 it first scales x by 2^32 (and the result by 2^-16), and it
 // passes 0 and inf through.  NaN, zero, negative and tiny arguments take the
 // full sequence.
+__device__ __forceinline__ float sqrt_rn_unscaled(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sd = __uint_as_float(__float_as_uint(s) - 1u);
+    const float su = __uint_as_float(__float_as_uint(s) + 1u);
+    const float rd = __builtin_fmaf(-sd, s, x);
+    const float ru = __builtin_fmaf(-su, s, x);
+    float r = (rd <= 0.0f) ? sd : s;
+    r = (ru > 0.0f) ? su : r;
+    return r;
+}
 __device__ __forceinline__ float sqrt_rn_wave(float x) {
-    if (__builtin_expect(wave_mask(!(x >= 0x1p-96f)) == 0, 1)) {
-        const float s = __builtin_amdgcn_sqrtf(x);
-        const float sd = __uint_as_float(__float_as_uint(s) - 1u);
-        const float su = __uint_as_float(__float_as_uint(s) + 1u);
-        const float rd = __builtin_fmaf(-sd, s, x);
-        const float ru = __builtin_fmaf(-su, s, x);
-        float r = (rd <= 0.0f) ? sd : s;
-        r = (ru > 0.0f) ? su : r;
-        return r;
-    }
+    if (__builtin_expect(wave_mask(!(x >= 0x1p-96f)) == 0, 1)) return sqrt_rn_unscaled(x);
     return __builtin_sqrtf(x);
 }
 
@@ -220,19 +221,23 @@ struct P8Exact {
         float num = ((z8 - (28.0f * z6) * w2) + (70.0f * z4) * w4) - (28.0f * z2) * w6;
         // A live lane has r <= bailout (the loop stops at r > bailout), so
         // with bailout <= 16: w8 <= 2^32 and |num| < 127 * 2^32.  If every
-        // lane also has w8 >= 2^-60 (false for NaN), no step of the unscaled
-        // division over- or underflows unless |num / w8| < 2^-25, where
-        // 1 + num / w8 rounds to 1 either way; a is then the same float as
-        // with the full sequence.
-        float q;
-        if (u.bailout <= 16.0f && wave_mask(!(w8 >= 0x1p-60f)) == 0)
+        // lane also has w2 >= 2^-15 (false for NaN), then w8 >= 2^-60 and no
+        // step of the unscaled division over- or underflows unless
+        // |num / w8| < 2^-25, where 1 + num / w8 rounds to 1 either way; a is
+        // then the same float as with the full sequence.  The same test
+        // covers the square root's w2 >= 2^-96.
+        float q, sw2;
+        if (u.bailout <= 16.0f && wave_mask(!(w2 >= 0x1p-15f)) == 0) {
             q = div_rn_unscaled(num, w8);
-        else
+            sw2 = sqrt_rn_unscaled(w2);
+        } else {
             q = div_rn(num, w8);
+            sw2 = sqrt_rn_wave(w2);
+        }
         float a = 1.0f + q;
         float nx = a * ((((x8 - (28.0f * x6) * y2) + (70.0f * x4) * y4) - (28.0f * x2) * y6) - y8);
         float ny = (((8.0f * a) * x) * y) * (((x6 - (7.0f * x4) * y2) + (7.0f * x2) * y4) - y6);
-        float nz = (((8.0f * z) * sqrt_rn_wave(w2)) * (z2 - w2)) * ((z4 - (6.0f * z2) * w2) + w4);
+        float nz = (((8.0f * z) * sw2) * (z2 - w2)) * ((z4 - (6.0f * z2) * w2) + w4);
         // The wave test looks at every lane (a compare's lane mask is free;
         // folding `live` in first would cost two VALU instructions); a lane
         // without a live point that happens to sit on the axis only sends the
