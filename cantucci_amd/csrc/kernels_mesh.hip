@@ -407,7 +407,28 @@ struct WaveLeaf {
     static constexpr uint32_t NWORDS = (CELLS + 63) / 64;
     static constexpr uint32_t RPW = 64 / RES;  // rows per crossing word
     static constexpr size_t CORNERS = (size_t)N1 * N1 * N1;
+    // Resolutions up to 8 stage the leaf's corner distances in the wave's LDS
+    // (729 floats at res 8): one round of coalesced loads per leaf, after
+    // which the row pass and the vertex pass read LDS instead of re-reading
+    // global memory with strided and dependent loads.
+    static constexpr bool STAGE = RES <= 8;
+    static constexpr uint32_t STAGED = STAGE ? (uint32_t)((CORNERS + 63) / 64 * 64) : 1;
 };
+
+// Coalesced copy of a leaf's corner grid into the wave's LDS slice.
+template <int RES>
+__device__ __forceinline__ void stage_leaf(const float* __restrict__ D, float* S, uint32_t lane) {
+    using L = WaveLeaf<RES>;
+    float v[L::STAGED / 64];
+#pragma unroll
+    for (uint32_t j = 0; j < L::STAGED / 64; ++j) {
+        const uint32_t i = j * 64 + lane;
+        v[j] = i < L::CORNERS ? D[i] : 0.0f;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < L::STAGED / 64; ++j) S[j * 64 + lane] = v[j];
+    wave_sync();
+}
 
 template <int RES>
 __global__ __launch_bounds__(kMeshBlock) void k_mesh_count_w(const float* __restrict__ dists, uint32_t nleaves,
@@ -415,8 +436,10 @@ __global__ __launch_bounds__(kMeshBlock) void k_mesh_count_w(const float* __rest
                                                              uint64_t* __restrict__ cross) {
     using L = WaveLeaf<RES>;
     __shared__ uint32_t rowbits_all[kMeshBlock / 64][L::ROWS];
+    __shared__ float staged_all[kMeshBlock / 64][L::STAGED];
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     uint32_t* rowbits = rowbits_all[wave];
+    float* staged = staged_all[wave];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         vcnt[nleaves] = 0;
         icnt[nleaves] = 0;
@@ -424,6 +447,10 @@ __global__ __launch_bounds__(kMeshBlock) void k_mesh_count_w(const float* __rest
     const uint32_t wstride = gridDim.x * (blockDim.x >> 6);
     for (uint32_t leaf = blockIdx.x * (blockDim.x >> 6) + wave; leaf < nleaves; leaf += wstride) {
         const float* D = dists + (size_t)leaf * L::CORNERS;
+        if constexpr (L::STAGE) {
+            stage_leaf<RES>(D, staged, lane);
+            D = staged;
+        }
         uint32_t nv = 0, nf = 0;
         for (uint32_t k = 0; k < L::RPL; ++k) {
             const uint32_t row = lane * L::RPL + k;
@@ -477,6 +504,7 @@ __global__ __launch_bounds__(kMeshBlock) void k_mesh_emit_w(const float* __restr
         uint64_t rf[L::ROWS];     // face bits of cell z at bits 3z..3z+2
         uint16_t vlist[L::CELLS];  // cell of each vertex, in order
         uint32_t flist[L::CELLS];  // cell | face bits << 12 | first quad << 16
+        float staged[L::STAGED];   // the leaf's corner distances (res <= 8)
     };
     __shared__ Lists lists_all[kMeshBlock / 64];
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
@@ -484,6 +512,10 @@ __global__ __launch_bounds__(kMeshBlock) void k_mesh_emit_w(const float* __restr
     const uint32_t wstride = gridDim.x * (blockDim.x >> 6);
     for (uint32_t leaf = blockIdx.x * (blockDim.x >> 6) + wave; leaf < nleaves; leaf += wstride) {
         const float* D = dists + (size_t)leaf * L::CORNERS;
+        if constexpr (L::STAGE) {
+            stage_leaf<RES>(D, W.staged, lane);
+            D = W.staged;
+        }
         // Pass over the lane's rows: crossing, sign and face bits.
         uint32_t lv = 0, lf = 0, lc = 0;
         for (uint32_t k = 0; k < L::RPL; ++k) {
@@ -752,19 +784,26 @@ int mesh_dims(uint32_t res, MeshDims& m) {
     return CTH_OK;
 }
 
-// Persistent grid of wave-per-leaf kernels: at most 8 workgroups per CU.
-inline uint32_t wave_grid(const Ctx& c, uint32_t nleaves) {
-    const uint64_t want = (uint64_t)c.num_cus * 8u;
+// Persistent grid of wave-per-leaf kernels: as many workgroups per CU as
+// can be resident at once (at most 8).  Leaves are dealt to waves with a
+// fixed stride, so a workgroup that is not resident from the start would run
+// its whole share after the others: the grid must not exceed residency.
+template <class K>
+inline uint32_t wave_grid(const Ctx& c, uint32_t nleaves, K kernel) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kMeshBlock, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    const uint64_t want = (uint64_t)c.num_cus * (uint64_t)std::min(per_cu, 8);
     const uint64_t need = ((uint64_t)nleaves + 3u) / 4u;
     return (uint32_t)std::max<uint64_t>(1, std::min(want, need));
 }
 
-#define LAUNCH_COUNT_W(R)                                                                                  \
-    hipLaunchKernelGGL(k_mesh_count_w<R>, dim3(blocks), dim3(kMeshBlock), 0, st, dists, nleaves, counts, \
-                       counts + cnt, cross)
-#define LAUNCH_EMIT_W(R)                                                                                        \
-    hipLaunchKernelGGL(k_mesh_emit_w<R>, dim3(blocks), dim3(kMeshBlock), 0, st, spans, dists, nleaves, voff, ioff, \
-                       verts, idx)
+#define LAUNCH_COUNT_W(R)                                                                                    \
+    hipLaunchKernelGGL(k_mesh_count_w<R>, dim3(wave_grid(c, nleaves, k_mesh_count_w<R>)), dim3(kMeshBlock), 0, \
+                       st, dists, nleaves, counts, counts + cnt, cross)
+#define LAUNCH_EMIT_W(R)                                                                                    \
+    hipLaunchKernelGGL(k_mesh_emit_w<R>, dim3(wave_grid(c, nleaves, k_mesh_emit_w<R>)), dim3(kMeshBlock), 0, \
+                       st, spans, dists, nleaves, voff, ioff, verts, idx)
 
 }  // namespace
 
@@ -818,7 +857,6 @@ int mesh_count(Ctx& c, const float* dists, uint32_t nleaves, uint32_t res, uint6
         uint64_t* counts = nullptr;
         if (c.scratch(WS_COUNTS, 2 * cnt * sizeof(uint64_t), (void**)&counts))
             return fail(CTH_E_NOMEM, "mesh count scratch allocation failed");
-        const uint32_t blocks = wave_grid(c, nleaves);
         switch (res) {
             case 1: LAUNCH_COUNT_W(1); break;
             case 2: LAUNCH_COUNT_W(2); break;
@@ -859,7 +897,6 @@ int mesh_emit(Ctx& c, const cth_shape& s, const float* spans, const float* dists
         return fail(CTH_E_UNSUPPORTED, "tangent normals need a Mandelbulb kind");
     if (nleaves == 0) return CTH_OK;
     if (res <= 16) {
-        const uint32_t blocks = wave_grid(c, nleaves);
         switch (res) {
             case 1: LAUNCH_EMIT_W(1); break;
             case 2: LAUNCH_EMIT_W(2); break;
