@@ -150,7 +150,13 @@ int refine_levels(Ctx& c, const cth_shape& s, const float* root, uint32_t min_de
         } else {
             SrcCenters src{cur, s.power};
             SinkSplit sink{flags, half_diag[depth]};
-            rc = run_refill(c, family, u, src, sink, (uint32_t)n, st, c.shape_spans);
+            // A level has at most a few million centres: 64-point chunks
+            // dealt statically spread it over the whole chip without the
+            // claim atomics that a dynamic queue would serialise on.
+            LaunchShape ls = c.shape_spans;
+            ls.chunk = 64;
+            ls.dynamic = 0;
+            rc = run_refill(c, family, u, src, sink, (uint32_t)n, st, ls);
         }
         if (rc) break;
         size_t temp_bytes = 0;
