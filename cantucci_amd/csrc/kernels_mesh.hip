@@ -659,8 +659,18 @@ __device__ __forceinline__ void put_fd_normal(float* v, const float t[7]) {
 // loop (work_loop.h) -- keeps the tap distances and writes the normal and
 // distance_from_surface itself.  The arithmetic is that of SrcMeshTaps +
 // k_mesh_fd_combine; no tap buffer.  Index space: vertices.
+//
+// A lane's vertex record (position, half-steps) and its tap distances are
+// only touched at the batched refill, so they live in the lane's own LDS
+// column rather than in 13 vector registers.  The kernel then fits in 72
+// VGPRs without spilling, and 7 waves share each SIMD (5 with the record in
+// registers); asking for 8 forces spills and is slower.
 template <class Ev>
-__global__ __launch_bounds__(256) void k_mesh_taps(float* __restrict__ verts, MandelUniform u, WorkQueue q) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_mesh_taps(float* __restrict__ verts,
+                                                                                          MandelUniform u, WorkQueue q) {
+    // slots 0..2: vertex position, 3..5: half-steps, 6 + t: distance of tap t
+    __shared__ float lane_slots[13][256];
+    float* const slot = &lane_slots[0][threadIdx.x];  // slot j at slot[256 * j]
     Ev ev;
     // Lane state as in refill_loop (work_loop.h): live / holding lanes are
     // wave-uniform lane masks; `has_v` (the lane owns a vertex whose taps are
@@ -670,24 +680,23 @@ __global__ __launch_bounds__(256) void k_mesh_taps(float* __restrict__ verts, Ma
     bool has_v = false;
     uint32_t vid = 0, t = 0, k = 0;
     float r = 0.0f;
-    float p[3] = {0.0f, 0.0f, 0.0f}, dl[3] = {0.0f, 0.0f, 0.0f};
-    float acc[7] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     uint32_t c_next = 0, c_end = 0, round = 0;
     bool exhausted = false;
 
     for (;;) {
         if ((uint32_t)__popcll(wave & ~live_m) >= q.refill_min) {
             if (__builtin_amdgcn_inverse_ballot_w64(hold_m)) {
-                const float d = ev.finish(r);
-#pragma unroll
-                for (int j = 0; j < 7; ++j) acc[j] = (t == (uint32_t)j) ? d : acc[j];
+                slot[256 * (6 + t)] = ev.finish(r);
                 ++t;
             }
             hold_m = 0;
             for (;;) {
                 // every lane here is idle; a lane with all 7 taps writes its vertex
                 if (has_v && t == 7u && !__builtin_amdgcn_inverse_ballot_w64(live_m)) {
-                    put_fd_normal(verts + 7 * (size_t)vid, acc);
+                    float tv[7];
+#pragma unroll
+                    for (int j = 0; j < 7; ++j) tv[j] = slot[256 * (6 + j)];
+                    put_fd_normal(verts + 7 * (size_t)vid, tv);
                     has_v = false;
                 }
                 const uint64_t need = wave_mask(!has_v) & ~live_m;
@@ -708,12 +717,8 @@ __global__ __launch_bounds__(256) void k_mesh_taps(float* __restrict__ verts, Ma
                         if (__builtin_amdgcn_inverse_ballot_w64(need) && rank < avail) {
                             vid = c_next + rank;
                             const float* rec = verts + 7 * (size_t)vid;
-                            p[0] = rec[0];
-                            p[1] = rec[1];
-                            p[2] = rec[2];
-                            dl[0] = rec[3];
-                            dl[1] = rec[4];
-                            dl[2] = rec[5];
+#pragma unroll
+                            for (int j = 0; j < 6; ++j) slot[256 * j] = rec[j];
                             t = 0;
                             has_v = true;
                         }
@@ -727,10 +732,10 @@ __global__ __launch_bounds__(256) void k_mesh_taps(float* __restrict__ verts, Ma
                 }
                 if (__builtin_amdgcn_inverse_ballot_w64(start_m)) {
                     // tap t: p, or p + unit_a * (+-delta_a) (src/mesh/buffer.rs:259-264)
-                    float c[3] = {p[0], p[1], p[2]};
+                    float c[3] = {slot[0], slot[256], slot[512]};
                     if (t > 0) {
                         const int a = (int)(t - 1) >> 1;
-                        const float da = (a == 0) ? dl[0] : ((a == 1) ? dl[1] : dl[2]);
+                        const float da = slot[256 * (3 + a)];
                         const float h = (t & 1u) ? da : -da;
 #pragma unroll
                         for (int b = 0; b < 3; ++b) c[b] = c[b] + ((a == b) ? 1.0f : 0.0f) * h;
@@ -739,9 +744,7 @@ __global__ __launch_bounds__(256) void k_mesh_taps(float* __restrict__ verts, Ma
                     k = 0;
                     r = ev.mag();
                     if (r > u.bailout) {
-                        const float d = ev.finish(r);
-#pragma unroll
-                        for (int j = 0; j < 7; ++j) acc[j] = (t == (uint32_t)j) ? d : acc[j];
+                        slot[256 * (6 + t)] = ev.finish(r);
                         ++t;
                     }
                 }
