@@ -52,7 +52,9 @@ __global__ __launch_bounds__(256) void k_jobs(const JobDev* __restrict__ jobs, u
     if ((threadIdx.x & 63u) == 0) atomicMin(start, (unsigned long long)wall_clock64());
 
     Ev ev;
-    bool live = false, hold = false;
+    // live / holding lanes as wave-uniform lane masks (see refill_loop)
+    const uint64_t wave = wave_mask(true);
+    uint64_t live_m = 0, hold_m = 0;
     uint32_t idx = 0, k = 0, job = 0, kcap = 1;
     float r = 0.0f;
     uint32_t c_next = 0, c_end = 0, round = 0;
@@ -60,26 +62,26 @@ __global__ __launch_bounds__(256) void k_jobs(const JobDev* __restrict__ jobs, u
     SinkGrad sink{dists, normals, nullptr, nullptr, nullptr, 0u};
 
     for (;;) {
-        if ((uint32_t)__popcll(wave_mask(!live)) >= q.refill_min) {
+        if ((uint32_t)__popcll(wave & ~live_m) >= q.refill_min) {
             // write back held points, then stamp their jobs (one atomic per job per wave)
-            if (hold) {
+            if (__builtin_amdgcn_inverse_ballot_w64(hold_m)) {
                 sink.max_iters = kcap;
                 sink.put(idx, ev.finish(r), k, r, ev.dr, ev);
             }
-            uint64_t pend = wave_mask(hold);
+            uint64_t pend = hold_m;
             if (pend) {
                 unsigned long long now = wall_clock64();
                 while (pend) {
                     int lead = __ffsll((unsigned long long)pend) - 1;
                     uint32_t jl = (uint32_t)__builtin_amdgcn_readlane((int)job, lead);
-                    uint64_t same = wave_mask(hold && job == jl);
+                    uint64_t same = wave_mask(job == jl) & hold_m;
                     if ((int)__lane_id() == lead) atomicMax(done + jl, now);
                     pend &= ~same;
                 }
             }
-            hold = false;
+            hold_m = 0;
             while (!exhausted) {
-                uint64_t need = wave_mask(!live);
+                const uint64_t need = wave & ~live_m;
                 if (need == 0) break;
                 if (c_next >= c_end) {
                     uint32_t b = next_chunk(q, round);
@@ -92,9 +94,10 @@ __global__ __launch_bounds__(256) void k_jobs(const JobDev* __restrict__ jobs, u
                 }
                 uint32_t avail = c_end - c_next;
                 uint32_t nneed = (uint32_t)__popcll(need);
-                if (!live) {
-                    uint32_t rank = lane_rank(need);
-                    if (rank < avail) {
+                const uint32_t rank = lane_rank(need);
+                const bool take = __builtin_amdgcn_inverse_ballot_w64(need) && rank < avail;
+                {
+                    if (take) {
                         idx = c_next + rank;
                         // job of idx: the last offset <= idx
                         uint32_t lo = 0, hi = njobs;
@@ -123,25 +126,23 @@ __global__ __launch_bounds__(256) void k_jobs(const JobDev* __restrict__ jobs, u
                         if (r > u.bailout) {
                             sink.max_iters = kcap;
                             sink.put(idx, ev.finish(r), 0u, r, ev.dr, ev);
-                        } else {
-                            live = true;
                         }
                     }
                 }
+                live_m |= wave_mask(take) & ~wave_mask(r > u.bailout);
                 c_next += (nneed < avail) ? nneed : avail;
             }
-            if (wave_mask(live) == 0) break;
+            if (live_m == 0) break;
         }
+        const bool live = __builtin_amdgcn_inverse_ballot_w64(live_m);
         ev.body(r, u, live);
         k += live ? 1u : 0u;
-        bool cap = live && (k == kcap);
-        float rn = ev.mag();
-        bool bail = live && !cap && (rn > u.bailout);
-        if (live && !cap) r = rn;
-        if (cap || bail) {
-            live = false;
-            hold = true;
-        }
+        const uint64_t cap_m = wave_mask(k == kcap) & live_m;
+        const float rn = ev.mag();
+        const uint64_t bail_m = wave_mask(rn > u.bailout) & live_m & ~cap_m;
+        if (__builtin_amdgcn_inverse_ballot_w64(live_m & ~cap_m)) r = rn;
+        live_m &= ~(cap_m | bail_m);
+        hold_m |= cap_m | bail_m;
     }
     if (q.dynamic) wave_retire(q);
 }
