@@ -24,6 +24,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "internal.h"
 #include "launch.h"
@@ -495,15 +496,16 @@ __global__ __launch_bounds__(kMeshBlock) void k_mesh_emit_w(const float* __restr
                                                             const uint64_t* __restrict__ ioff,
                                                             float* __restrict__ verts, uint32_t* __restrict__ idx) {
     using L = WaveLeaf<RES>;
-    static_assert(RES <= 16, "cell numbers and face offsets are packed into LDS lists");
+    static_assert(RES <= 16, "cell numbers and face bits are packed into 16-bit LDS lists");
+    using RowFaces = std::conditional_t<(3 * RES <= 32), uint32_t, uint64_t>;
     // Per wave: row bookkeeping and the leaf's compacted vertex and quad lists.
     struct Lists {
         uint32_t rb[L::ROWS];     // crossing bits of the row's cells
         uint32_t rp[L::ROWS];     // vertices before the row
         uint32_t rn[L::ROWS];     // bit z: dists[(x, y, z)] < 0.0
-        uint64_t rf[L::ROWS];     // face bits of cell z at bits 3z..3z+2
+        RowFaces rf[L::ROWS];     // face bits of cell z at bits 3z..3z+2
         uint16_t vlist[L::CELLS];  // cell of each vertex, in order
-        uint32_t flist[L::CELLS];  // cell | face bits << 12 | first quad << 16
+        uint16_t flist[L::CELLS];  // cell | face bits << 12, cells with faces in order
         float staged[L::STAGED];   // the leaf's corner distances (res <= 8)
     };
     __shared__ Lists lists_all[kMeshBlock / 64];
@@ -523,12 +525,12 @@ __global__ __launch_bounds__(kMeshBlock) void k_mesh_emit_w(const float* __restr
             if (row < L::ROWS) {
                 const uint32_t x = row / RES, y = row % RES;
                 uint32_t bits = 0, neg = 0;
-                uint64_t fw = 0;
+                RowFaces fw = 0;
                 row_cells<RES>(D, x, y, [&](uint32_t z, const float* d, uint32_t mk) {
                     bits |= (crossing(mk) ? 1u : 0u) << z;
                     neg |= (d[0] < 0.0f ? 1u : 0u) << z;
                     const uint32_t fb = face_bits(mk, x, y, z);
-                    fw |= (uint64_t)fb << (3u * z);
+                    fw |= (RowFaces)fb << (3u * z);
                     lf += (uint32_t)__popc(fb);
                     lc += fb ? 1u : 0u;
                 });
@@ -540,7 +542,7 @@ __global__ __launch_bounds__(kMeshBlock) void k_mesh_emit_w(const float* __restr
         }
         uint32_t tv, tf, tc;
         uint32_t vex = wave_excl_scan(lv, tv);
-        uint32_t fo = wave_excl_scan(lf, tf);
+        (void)wave_excl_scan(lf, tf);
         uint32_t cex = wave_excl_scan(lc, tc);
         // Compacted lists, in cube() cell order.
         for (uint32_t k = 0; k < L::RPL; ++k) {
@@ -549,13 +551,10 @@ __global__ __launch_bounds__(kMeshBlock) void k_mesh_emit_w(const float* __restr
                 W.rp[row] = vex;
                 for (uint32_t bits = W.rb[row]; bits; bits &= bits - 1u)
                     W.vlist[vex++] = (uint16_t)(row * RES + (uint32_t)__builtin_ctz(bits));
-                const uint64_t fw = W.rf[row];
+                const RowFaces fw = W.rf[row];
                 for (uint32_t z = 0; z < (uint32_t)RES; ++z) {
                     const uint32_t fb = (uint32_t)(fw >> (3u * z)) & 7u;
-                    if (fb) {
-                        W.flist[cex++] = (row * RES + z) | (fb << 12) | (fo << 16);
-                        fo += (uint32_t)__popc(fb);
-                    }
+                    if (fb) W.flist[cex++] = (uint16_t)((row * RES + z) | (fb << 12));
                 }
             }
         }
@@ -573,18 +572,26 @@ __global__ __launch_bounds__(kMeshBlock) void k_mesh_emit_w(const float* __restr
             place_vertex(d, x, y, z, s1, step, p);
             put_vertex(verts + 7 * (vbase + j), p, delta);
         }
-        // Quads, one corner per lane per round.
+        // Quads, one corner per lane per round; a round's first quad numbers
+        // come from a wave scan of its corners' face counts.
         uint32_t* iout = idx + ioff[leaf];
         auto pt = [&](uint32_t px, uint32_t py, uint32_t pz) {
             const uint32_t r = px * RES + py;
             return W.rp[r] + (uint32_t)__popc(W.rb[r] & ((1u << pz) - 1u));
         };
-        for (uint32_t j = lane; j < tc; j += 64) {
-            const uint32_t e = W.flist[j];
-            const uint32_t cell = e & 0xFFFu, fb = (e >> 12) & 7u, q = e >> 16;
-            const uint32_t x = cell / (RES * RES), y = (cell / RES) % RES, z = cell % RES;
-            const bool neg = (W.rn[x * RES + y] >> z) & 1u;
-            emit_quads(iout + 6ull * q, fb, neg, x, y, z, pt);
+        uint32_t qbase = 0;
+        for (uint32_t j0 = 0; j0 < tc; j0 += 64) {
+            const uint32_t j = j0 + lane;
+            const uint32_t e = j < tc ? (uint32_t)W.flist[j] : 0u;
+            const uint32_t cell = e & 0xFFFu, fb = (e >> 12) & 7u;
+            uint32_t round_quads;
+            const uint32_t q = qbase + wave_excl_scan((uint32_t)__popc(fb), round_quads);
+            if (j < tc) {
+                const uint32_t x = cell / (RES * RES), y = (cell / RES) % RES, z = cell % RES;
+                const bool neg = (W.rn[x * RES + y] >> z) & 1u;
+                emit_quads(iout + 6ull * q, fb, neg, x, y, z, pt);
+            }
+            qbase += round_quads;
         }
         wave_sync();  // the lists are rewritten for the next leaf
     }
