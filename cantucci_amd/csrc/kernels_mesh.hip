@@ -416,20 +416,89 @@ struct WaveLeaf {
     static constexpr uint32_t STAGED = STAGE ? (uint32_t)((CORNERS + 63) / 64 * 64) : 1;
 };
 
-// Coalesced copy of a leaf's corner grid into the wave's LDS slice.
+// Bit fields of a leaf's corner grid (res <= 8), built with one wave ballot
+// per 64 corners: bit i of `sgn` = IEEE sign bit of corner i (the
+// reference's !is_sign_positive), bit i of `neg` = corner i < 0.0.  Words
+// are 32-bit with two zero words of padding, so any 32-bit window can be read
+// with a funnel shift.  With these, one lane classifies its whole row of
+// cells with a handful of bit operations instead of per-cell compares.
 template <int RES>
-__device__ __forceinline__ void stage_leaf(const float* __restrict__ D, float* S, uint32_t lane) {
-    using L = WaveLeaf<RES>;
-    float v[L::STAGED / 64];
+struct LeafFields {
+    static constexpr uint32_t WORDS = WaveLeaf<RES>::STAGED / 32 + 2;
+    uint32_t sgn[WORDS];
+    uint32_t neg[WORDS];
+};
+
+// A leaf's corner distances, one coalesced round of loads (lane-strided).
+template <int RES>
+struct LeafRegs {
+    static constexpr uint32_t NJ = WaveLeaf<RES>::STAGED / 64;
+    float v[NJ];
+    __device__ __forceinline__ void load(const float* __restrict__ D, uint32_t lane) {
 #pragma unroll
-    for (uint32_t j = 0; j < L::STAGED / 64; ++j) {
-        const uint32_t i = j * 64 + lane;
-        v[j] = i < L::CORNERS ? D[i] : 0.0f;
+        for (uint32_t j = 0; j < NJ; ++j) {
+            const uint32_t i = j * 64 + lane;
+            v[j] = i < WaveLeaf<RES>::CORNERS ? D[i] : 0.0f;
+        }
+    }
+};
+
+// Optionally stages the floats in S, and writes the sign fields (the `< 0`
+// field only when NEG).
+template <int RES, bool NEG>
+__device__ __forceinline__ void leaf_fields(const LeafRegs<RES>& R, float* S, LeafFields<RES>& F, uint32_t lane) {
+    using L = WaveLeaf<RES>;
+    constexpr uint32_t NJ = L::STAGED / 64;
+    const float* v = R.v;
+    if (S) {
+#pragma unroll
+        for (uint32_t j = 0; j < NJ; ++j) S[j * 64 + lane] = v[j];
     }
 #pragma unroll
-    for (uint32_t j = 0; j < L::STAGED / 64; ++j) S[j * 64 + lane] = v[j];
+    for (uint32_t j = 0; j < NJ; ++j) {
+        const uint64_t ms = wave_mask((int)__float_as_uint(v[j]) < 0);
+        // lane 0 stores the low word, lane 1 the high word
+        if (lane < 2) F.sgn[2 * j + lane] = (uint32_t)(ms >> (32 * lane));
+        if constexpr (NEG) {
+            const uint64_t mn = wave_mask(v[j] < 0.0f);
+            if (lane < 2) F.neg[2 * j + lane] = (uint32_t)(mn >> (32 * lane));
+        }
+    }
+    if (lane < 2) {
+        F.sgn[2 * NJ + lane] = 0u;
+        if constexpr (NEG) F.neg[2 * NJ + lane] = 0u;
+    }
     wave_sync();
 }
+
+// Bits [o, o + 32) of a field.
+__device__ __forceinline__ uint32_t field_window(const uint32_t* f, uint32_t o) {
+    const uint32_t w = o >> 5;
+    return __builtin_amdgcn_alignbit(f[w + 1], f[w], o & 31u);
+}
+
+// Row (x, y) of a leaf from its sign field: crossing bits, and the +x / +y / +z
+// quad bits of its cells (src/mesh/buffer.rs:113-147, 299-350).  A, B, C, D are
+// the signs of the corner rows (x, y), (x, y + 1), (x + 1, y), (x + 1, y + 1);
+// bit z of `a & a >> 1` is "corners z and z + 1 both negative".
+template <int RES>
+struct RowBits {
+    uint32_t cross, fx, fy, fz;
+    __device__ __forceinline__ RowBits(const uint32_t* sgn, uint32_t x, uint32_t y) {
+        constexpr uint32_t N1 = RES + 1;
+        constexpr uint32_t cm = (1u << RES) - 1u;
+        const uint32_t o = (x * N1 + y) * N1;
+        const uint32_t a = field_window(sgn, o), b = field_window(sgn, o + N1);
+        const uint32_t c = field_window(sgn, o + N1 * N1), d = field_window(sgn, o + N1 * N1 + N1);
+        const uint32_t all = a & (a >> 1) & b & (b >> 1) & c & (c >> 1) & d & (d >> 1);
+        const uint32_t any = a | (a >> 1) | b | (b >> 1) | c | (c >> 1) | d | (d >> 1);
+        cross = any & ~all & cm;                      // not all 8 signs equal
+        const uint32_t zpos = cm & ~1u;               // z > 0
+        fx = (y > 0) ? ((a ^ c) & zpos) : 0u;         // corner 0 vs 4, needs y, z > 0
+        fy = (x > 0) ? ((a ^ b) & zpos) : 0u;         // corner 0 vs 2, needs x, z > 0
+        fz = (x > 0 && y > 0) ? ((a ^ (a >> 1)) & cm) : 0u;  // corner 0 vs 1, needs x, y > 0
+    }
+};
 
 template <int RES>
 __global__ __launch_bounds__(kMeshBlock) void k_mesh_count_w(const float* __restrict__ dists, uint32_t nleaves,
@@ -437,20 +506,27 @@ __global__ __launch_bounds__(kMeshBlock) void k_mesh_count_w(const float* __rest
                                                              uint64_t* __restrict__ cross) {
     using L = WaveLeaf<RES>;
     __shared__ uint32_t rowbits_all[kMeshBlock / 64][L::ROWS];
-    __shared__ float staged_all[kMeshBlock / 64][L::STAGED];
+    __shared__ LeafFields<RES> fields_all[kMeshBlock / 64];
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     uint32_t* rowbits = rowbits_all[wave];
-    float* staged = staged_all[wave];
+    LeafFields<RES>& F = fields_all[wave];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         vcnt[nleaves] = 0;
         icnt[nleaves] = 0;
     }
     const uint32_t wstride = gridDim.x * (blockDim.x >> 6);
-    for (uint32_t leaf = blockIdx.x * (blockDim.x >> 6) + wave; leaf < nleaves; leaf += wstride) {
+    // The next leaf's corners are loaded while this one is classified.
+    LeafRegs<RES> next;
+    const uint32_t first = blockIdx.x * (blockDim.x >> 6) + wave;
+    if constexpr (L::STAGE) {
+        if (first < nleaves) next.load(dists + (size_t)first * L::CORNERS, lane);
+    }
+    for (uint32_t leaf = first; leaf < nleaves; leaf += wstride) {
         const float* D = dists + (size_t)leaf * L::CORNERS;
         if constexpr (L::STAGE) {
-            stage_leaf<RES>(D, staged, lane);
-            D = staged;
+            const LeafRegs<RES> cur = next;
+            if (leaf + wstride < nleaves) next.load(dists + (size_t)(leaf + wstride) * L::CORNERS, lane);
+            leaf_fields<RES, false>(cur, nullptr, F, lane);
         }
         uint32_t nv = 0, nf = 0;
         for (uint32_t k = 0; k < L::RPL; ++k) {
@@ -458,11 +534,17 @@ __global__ __launch_bounds__(kMeshBlock) void k_mesh_count_w(const float* __rest
             if (row < L::ROWS) {
                 const uint32_t x = row / RES, y = row % RES;
                 uint32_t bits = 0;
-                row_cells<RES>(D, x, y, [&](uint32_t z, const float*, uint32_t mk) {
-                    const bool cr = crossing(mk);
-                    bits |= (cr ? 1u : 0u) << z;
-                    nf += (uint32_t)__popc(face_bits(mk, x, y, z));
-                });
+                if constexpr (L::STAGE) {
+                    const RowBits<RES> rb(F.sgn, x, y);
+                    bits = rb.cross;
+                    nf += (uint32_t)(__popc(rb.fx) + __popc(rb.fy) + __popc(rb.fz));
+                } else {
+                    row_cells<RES>(D, x, y, [&](uint32_t z, const float*, uint32_t mk) {
+                        const bool cr = crossing(mk);
+                        bits |= (cr ? 1u : 0u) << z;
+                        nf += (uint32_t)__popc(face_bits(mk, x, y, z));
+                    });
+                }
                 nv += (uint32_t)__popc(bits);
                 rowbits[row] = bits;
             }
@@ -503,19 +585,30 @@ __global__ __launch_bounds__(kMeshBlock) void k_mesh_emit_w(const float* __restr
         uint32_t rb[L::ROWS];     // crossing bits of the row's cells
         uint32_t rp[L::ROWS];     // vertices before the row
         uint32_t rn[L::ROWS];     // bit z: dists[(x, y, z)] < 0.0
-        RowFaces rf[L::ROWS];     // face bits of cell z at bits 3z..3z+2
+        // face bits of cell z: at bits 3z..3z+2, or (res <= 8) +x at bit z,
+        // +y at bit 8 + z, +z at bit 16 + z
+        RowFaces rf[L::ROWS];
         uint16_t vlist[L::CELLS];  // cell of each vertex, in order
         uint16_t flist[L::CELLS];  // cell | face bits << 12, cells with faces in order
         float staged[L::STAGED];   // the leaf's corner distances (res <= 8)
+        LeafFields<RES> fields;    // their sign fields (res <= 8)
     };
     __shared__ Lists lists_all[kMeshBlock / 64];
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     Lists& W = lists_all[wave];
     const uint32_t wstride = gridDim.x * (blockDim.x >> 6);
-    for (uint32_t leaf = blockIdx.x * (blockDim.x >> 6) + wave; leaf < nleaves; leaf += wstride) {
+    // The next leaf's corners are loaded while this one is processed.
+    LeafRegs<RES> next;
+    const uint32_t first = blockIdx.x * (blockDim.x >> 6) + wave;
+    if constexpr (L::STAGE) {
+        if (first < nleaves) next.load(dists + (size_t)first * L::CORNERS, lane);
+    }
+    for (uint32_t leaf = first; leaf < nleaves; leaf += wstride) {
         const float* D = dists + (size_t)leaf * L::CORNERS;
         if constexpr (L::STAGE) {
-            stage_leaf<RES>(D, W.staged, lane);
+            const LeafRegs<RES> cur = next;
+            if (leaf + wstride < nleaves) next.load(dists + (size_t)(leaf + wstride) * L::CORNERS, lane);
+            leaf_fields<RES, true>(cur, W.staged, W.fields, lane);
             D = W.staged;
         }
         // Pass over the lane's rows: crossing, sign and face bits.
@@ -526,14 +619,23 @@ __global__ __launch_bounds__(kMeshBlock) void k_mesh_emit_w(const float* __restr
                 const uint32_t x = row / RES, y = row % RES;
                 uint32_t bits = 0, neg = 0;
                 RowFaces fw = 0;
-                row_cells<RES>(D, x, y, [&](uint32_t z, const float* d, uint32_t mk) {
-                    bits |= (crossing(mk) ? 1u : 0u) << z;
-                    neg |= (d[0] < 0.0f ? 1u : 0u) << z;
-                    const uint32_t fb = face_bits(mk, x, y, z);
-                    fw |= (RowFaces)fb << (3u * z);
-                    lf += (uint32_t)__popc(fb);
-                    lc += fb ? 1u : 0u;
-                });
+                if constexpr (L::STAGE) {
+                    const RowBits<RES> rb(W.fields.sgn, x, y);
+                    bits = rb.cross;
+                    neg = field_window(W.fields.neg, (x * L::N1 + y) * L::N1) & ((1u << RES) - 1u);
+                    fw = rb.fx | (rb.fy << 8) | (rb.fz << 16);
+                    lf += (uint32_t)(__popc(rb.fx) + __popc(rb.fy) + __popc(rb.fz));
+                    lc += (uint32_t)__popc(rb.fx | rb.fy | rb.fz);
+                } else {
+                    row_cells<RES>(D, x, y, [&](uint32_t z, const float* d, uint32_t mk) {
+                        bits |= (crossing(mk) ? 1u : 0u) << z;
+                        neg |= (d[0] < 0.0f ? 1u : 0u) << z;
+                        const uint32_t fb = face_bits(mk, x, y, z);
+                        fw |= (RowFaces)fb << (3u * z);
+                        lf += (uint32_t)__popc(fb);
+                        lc += fb ? 1u : 0u;
+                    });
+                }
                 W.rb[row] = bits;
                 W.rn[row] = neg;
                 W.rf[row] = fw;
@@ -553,7 +655,12 @@ __global__ __launch_bounds__(kMeshBlock) void k_mesh_emit_w(const float* __restr
                     W.vlist[vex++] = (uint16_t)(row * RES + (uint32_t)__builtin_ctz(bits));
                 const RowFaces fw = W.rf[row];
                 for (uint32_t z = 0; z < (uint32_t)RES; ++z) {
-                    const uint32_t fb = (uint32_t)(fw >> (3u * z)) & 7u;
+                    uint32_t fb;
+                    if constexpr (L::STAGE)
+                        fb = ((uint32_t)(fw >> z) & 1u) | ((uint32_t)(fw >> (z + 7u)) & 2u) |
+                             ((uint32_t)(fw >> (z + 14u)) & 4u);
+                    else
+                        fb = (uint32_t)(fw >> (3u * z)) & 7u;
                     if (fb) W.flist[cex++] = (uint16_t)((row * RES + z) | (fb << 12));
                 }
             }
