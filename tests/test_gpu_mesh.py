@@ -59,7 +59,7 @@ def assert_topology(voff, ioff, ix, voff_o, ioff_o, ix_o):
     assert np.array_equal(ix, ix_o), "indices differ"
 
 
-@pytest.mark.parametrize("res", [8, 1, 2, 16, 32, 64])
+@pytest.mark.parametrize("res", [8, 1, 2, 4, 16, 32, 64])
 def test_sphere_mesh_bitexact(ctx, torch_cuda, res):
     """Sphere: exact closed-form distances, so the whole mesh -- positions,
     6-tap normals and distances -- is bit-identical."""
@@ -78,6 +78,54 @@ def test_sphere_mesh_bitexact(ctx, torch_cuda, res):
     voff_e, ioff_e, v_e, ix_e = O.mesh_batch(O.sphere(center, radius), leaves, res)
     assert_topology(voff, ioff, ix, voff_e, ioff_e, ix_e)
     assert same_bits(v, v_e).all()
+
+
+def crossing_words(d, res):
+    """Crossing words of a batch of corner fields: bit c of a leaf's words is
+    cell c = (x * res + y) * res + z, set when its 8 corner sign bits differ."""
+    n1 = res + 1
+    s = np.signbit(d.reshape(-1, n1, n1, n1))
+    c = [s[:, x:x + res, y:y + res, z:z + res] for x in (0, 1) for y in (0, 1) for z in (0, 1)]
+    allneg = np.logical_and.reduce(c)
+    anyneg = np.logical_or.reduce(c)
+    bits = (anyneg & ~allneg).reshape(len(d), -1)
+    nwords = (res ** 3 + 63) // 64
+    pad = np.zeros((len(d), nwords * 64), bool)
+    pad[:, :bits.shape[1]] = bits
+    w = np.packbits(pad.reshape(len(d), nwords, 64)[:, :, ::-1], axis=2).view(">u8")[..., 0]
+    return w.astype(np.uint64)
+
+
+@pytest.mark.parametrize("res", [1, 2, 4, 8, 16])
+def test_mesh_signed_zero_and_nan_corners(ctx, torch_cuda, res):
+    """Crafted corner fields holding -0.0, +0.0 and NaNs of both signs: the
+    crossing words (sign bit, the reference's !is_sign_positive), counts,
+    every index (winding by `< 0.0`, false for -0.0 and NaN) and the vertex
+    positions match the oracle's; the words also match a numpy restatement."""
+    import cantucci_amd as ca
+    from cantucci_amd import device as dev
+    torch = torch_cuda
+    rng = np.random.default_rng(50 + res)
+    spans = c2_leaves(96, seed=7)
+    n1 = res + 1
+    vals = np.array([-1.0, -0.25, -0.0, 0.0, 0.25, 1.0, np.nan, -np.nan], F32)
+    assert np.signbit(vals[2]) and np.signbit(vals[7]) and not np.signbit(vals[6])
+    d = vals[rng.integers(0, len(vals), (len(spans), n1 ** 3))]
+    shape = ca.Sphere((0.0, 0.0, 0.0), 1.0, ctx=ctx)
+    nwords = (res ** 3 + 63) // 64
+    cb = torch.full((len(spans), nwords), -1, dtype=torch.int64, device="cuda")
+    voff, ioff, v, ix = dev.mesh(shape, gpu(torch, spans), gpu(torch, d), res, "fd6", cross=cb)
+    torch.cuda.synchronize()
+    voff = voff.cpu().numpy().astype(np.uint64)
+    ioff = ioff.cpu().numpy().astype(np.uint64)
+    v = v.cpu().numpy()
+    ix = ix.cpu().numpy().view(np.uint32)
+    assert (cb.cpu().numpy().view(np.uint64) == crossing_words(d, res)).all()
+    voff_o, ioff_o, v_o, ix_o = O.mesh_batch(O.sphere((0.0, 0.0, 0.0), 1.0), spans, res, dists=d)
+    assert_topology(voff, ioff, ix, voff_o, ioff_o, ix_o)
+    assert int(voff[-1]) > 0
+    pos_same = same_bits(v[:, :3], v_o[:, :3]) | (np.isnan(v[:, :3]) & np.isnan(v_o[:, :3]))
+    assert pos_same.all()
 
 
 def test_c2_leaf_mesh_parity(ctx, torch_cuda):
