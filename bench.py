@@ -65,27 +65,27 @@ def main() -> None:
 
     from cantucci_amd import autotune
 
-    c2 = wl.C2
+    oc = wl.OCTREE
     ctx = ca.Context(0, args.mode)
     tuned = autotune.apply(ctx, "spans")
     tuned_mesh = autotune.apply(ctx, "mesh")
-    shape = ca.Mandelbulb.classic(c2["max_iters"], c2["bailout"], ctx=ctx)
-    hd = wl.half_diagonals(3.0, c2["max_depth"])
-    res = c2["res"]
+    shape = ca.Mandelbulb.classic(oc["max_iters"], oc["bailout"], ctx=ctx)
+    hd = wl.half_diagonals(3.0, oc["max_depth"])
+    res = oc["res"]
     per = (res + 1) ** 3
 
     # Size the buffers once (the refinement is deterministic).
-    leaves, lc = dev.refine(shape, c2["root"], c2["min_depth"], c2["max_depth"], hd)
+    leaves, lc = dev.refine(shape, oc["root"], oc["min_depth"], oc["max_depth"], hd)
     nleaves = leaves.shape[0]
     leaves_buf = torch.empty((nleaves, 6), dtype=torch.float32, device="cuda")
     dists = torch.empty((nleaves, per), dtype=torch.float32, device="cuda")
     cross = torch.empty((nleaves, (res ** 3 + 63) // 64), dtype=torch.int64, device="cuda")
-    center_evals = int(sum(int(lc[d]) for d in range(c2["min_depth"], c2["max_depth"] + 1)))
+    center_evals = int(sum(int(lc[d]) for d in range(oc["min_depth"], oc["max_depth"] + 1)))
     corner_evals = nleaves * per
     mesh_out = {}
 
     def step():
-        lv, _ = dev.refine(shape, c2["root"], c2["min_depth"], c2["max_depth"], hd, leaves=leaves_buf)
+        lv, _ = dev.refine(shape, oc["root"], oc["min_depth"], oc["max_depth"], hd, leaves=leaves_buf)
         dev.sample_spans(shape, lv, res, with_cross=False, out=dists)
         # the mesh's count pass classifies the cells and writes the crossing words
         mesh_out["m"] = dev.mesh(shape, lv, dists, res, "fd6", cross=cross)
@@ -106,7 +106,7 @@ def main() -> None:
     evals = center_evals + corner_evals + vertex_evals
     ms_per_step = 1e3 * elapsed / args.steps
     value = evals * args.steps / elapsed
-    phases = _phases(dev, torch, shape, c2, hd, leaves_buf, dists, cross, res)
+    phases = _phases(dev, torch, shape, oc, hd, leaves_buf, dists, cross, res)
 
     # Dominant kernel: the span-sampling refill kernel.  Its average launch
     # time is measured with HIP events on the stream it runs on (torch's
@@ -123,14 +123,14 @@ def main() -> None:
     kernel_s = ev0.elapsed_time(ev1) / 1e3 / reps
     _, _, iters = dev.sample_spans(shape, leaves_buf, res, with_cross=False, with_iters=True)
     torch.cuda.synchronize()
-    flop = algorithmic_flop(iters, c2["max_iters"])
+    flop = algorithmic_flop(iters, oc["max_iters"])
     mean_iters = float(iters.to(torch.float64).mean().item())
     del iters
     achieved = flop / kernel_s
     traffic = None
     # HBM bytes per launch of the span kernel from the committed rocprofv3
     # FETCH_SIZE / WRITE_SIZE summary (copy-kernel calibrated).
-    tf = ROOT / "profiles" / "spans_c2_traffic.md"
+    tf = ROOT / "profiles" / "spans_octree_traffic.md"
     if tf.exists():
         m = re.search(r"^bytes_per_launch:\s*(\d+)", tf.read_text(), re.M)
         traffic = int(m.group(1)) if m else None
@@ -157,7 +157,7 @@ def main() -> None:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         lv_host = leaves_buf.cpu().numpy()
         rng = np.random.default_rng(0)
-        s = O.mandelbulb(c2["max_iters"], c2["bailout"])
+        s = O.mandelbulb(oc["max_iters"], oc["bailout"])
         m = min(256, nleaves)
         t = time.perf_counter()
         O.mesh_batch(s, lv_host[rng.choice(nleaves, m, replace=False)], res, threads=threads)
@@ -180,8 +180,9 @@ def main() -> None:
                "host": _host_cpu()}
 
     extra = {}
-    for name, fn in (("c1_grid", bench_c1), ("c3_trace", bench_c3), ("c4_normals", bench_c4), ("c5_jobs", bench_c5),
-                     ("reference_default_mesh", bench_ref_default), ("point_query", bench_point_query)):
+    for name, fn in (("grid_64", bench_grid), ("trace_1080p", bench_trace), ("normals_shell", bench_normals),
+                     ("mixed_jobs", bench_jobs), ("reference_default_mesh", bench_ref_default),
+                     ("point_query", bench_point_query)):
         try:
             extra[name] = fn(ctx, args)
         except Exception as e:  # a secondary leg must not hide the headline
@@ -237,7 +238,7 @@ def _host_cpu() -> str:
     return f"{model}, {os.cpu_count()} logical CPUs (threads used: see cores)"
 
 
-def _phases(dev, torch, shape, c2, hd, leaves_buf, dists, cross, res):
+def _phases(dev, torch, shape, oc, hd, leaves_buf, dists, cross, res):
     """Per-phase times of one step (HIP events on torch's current stream)."""
     out = {}
     marks = []
@@ -249,7 +250,7 @@ def _phases(dev, torch, shape, c2, hd, leaves_buf, dists, cross, res):
 
     torch.cuda.synchronize()
     mark()
-    lv, _ = dev.refine(shape, c2["root"], c2["min_depth"], c2["max_depth"], hd, leaves=leaves_buf)
+    lv, _ = dev.refine(shape, oc["root"], oc["min_depth"], oc["max_depth"], hd, leaves=leaves_buf)
     mark()
     dev.sample_spans(shape, lv, res, with_cross=False, out=dists)
     mark()
@@ -273,7 +274,7 @@ def _time_ms(fn, reps):
     return 1e3 * (time.perf_counter() - t) / reps
 
 
-def bench_c1(ctx, args):
+def bench_grid(ctx, args):
     """64^3 grid over [-1.5,1.5]^3 (divisor 63), P8 / 15 / 2: one launch is a
     few microseconds of work, so 100 launches are captured in a HIP graph."""
     import ctypes as C
@@ -426,7 +427,7 @@ def _create_spans(span):
     return out
 
 
-def bench_c3(ctx, args):
+def bench_trace(ctx, args):
     """32 seeded orbit poses x 1920x1080 primary rays, 256 steps, hit d < 1e-4 t, escape t > 10."""
     import torch
 
@@ -436,18 +437,18 @@ def bench_c3(ctx, args):
     from cantucci_amd import workloads as wl
 
     autotune.apply(ctx, "trace")
-    c3 = wl.C3
-    shape = ca.Mandelbulb.classic(c3["max_iters"], c3["bailout"], ctx=ctx)
-    cams = torch.from_numpy(wl.orbit_poses(c3["poses"], c3["seed"], c3["W"], c3["H"], c3["fov_deg"])).cuda()
-    depth = torch.empty((c3["poses"], c3["H"], c3["W"]), device="cuda")
-    steps = torch.empty((c3["poses"], c3["H"], c3["W"]), dtype=torch.int32, device="cuda")
+    tc = wl.TRACE_1080P
+    shape = ca.Mandelbulb.classic(tc["max_iters"], tc["bailout"], ctx=ctx)
+    cams = torch.from_numpy(wl.orbit_poses(tc["poses"], tc["seed"], tc["W"], tc["H"], tc["fov_deg"])).cuda()
+    depth = torch.empty((tc["poses"], tc["H"], tc["W"]), device="cuda")
+    steps = torch.empty((tc["poses"], tc["H"], tc["W"]), dtype=torch.int32, device="cuda")
 
     def run():
-        dev.render_depth(shape, cams, c3["W"], c3["H"], c3["max_steps"], 0.0, c3["eps_rel"], c3["t_max"],
+        dev.render_depth(shape, cams, tc["W"], tc["H"], tc["max_steps"], 0.0, tc["eps_rel"], tc["t_max"],
                          depth=depth, steps=steps)
 
     ms = _time_ms(run, 5)
-    rays = c3["poses"] * c3["W"] * c3["H"]
+    rays = tc["poses"] * tc["W"] * tc["H"]
     return {"workload": "32 poses x 1920x1080, P8 15 it, 256 steps, eps 1e-4 t, t_max 10",
             "rays_per_s": rays / (ms / 1e3), "ms": ms,
             "hit_fraction": float(torch.isfinite(depth).float().mean().item()),
@@ -455,7 +456,7 @@ def bench_c3(ctx, args):
             "de_evals_per_s": float(steps.to(torch.float64).sum().item()) / (ms / 1e3)}
 
 
-def bench_c4(ctx, args):
+def bench_normals(ctx, args):
     """2^22 points in the shell 0.9 < |p| < 1.3 (seed 1), power U[6,10], 15 it:
     distance + normal by the fused forward+tangent kernel, by the reverse-sweep
     VJP, and by the 6-tap central difference (h = 1e-3)."""
@@ -467,20 +468,20 @@ def bench_c4(ctx, args):
     from cantucci_amd import device as dev
     from cantucci_amd import workloads as wl
 
-    c4 = wl.C4
-    pts, powers = wl.shell_points(c4["n"], c4["seed"])
+    sc = wl.SHELL_NORMALS
+    pts, powers = wl.shell_points(sc["n"], sc["seed"])
     P = torch.from_numpy(pts).cuda()
     W = torch.from_numpy(powers).cuda()
-    shape = ca.MandelbulbTrig(c4["max_iters"], c4["bailout"], 8.0, ctx=ctx)
-    nrm = torch.empty((c4["n"], 3), device="cuda")
-    d = torch.empty(c4["n"], device="cuda")
+    shape = ca.MandelbulbTrig(sc["max_iters"], sc["bailout"], 8.0, ctx=ctx)
+    nrm = torch.empty((sc["n"], 3), device="cuda")
+    d = torch.empty(sc["n"], device="cuda")
     autotune.apply(ctx, "tangent")
     t_tan = _time_ms(lambda: dev.normals(shape, P, W, out=nrm, d=d), 5)
     n_tan = nrm.clone()
     t_vjp = _time_ms(lambda: dev.de_vjp(shape, P, W), 3)
     autotune.apply(ctx, "fd")
     t_fwd = _time_ms(lambda: dev.de_points(shape, P, W), 5)
-    t_fd = _time_ms(lambda: dev.normals_fd(shape, P, c4["fd_h"], W, out=nrm, d=d), 5)
+    t_fd = _time_ms(lambda: dev.normals_fd(shape, P, sc["fd_h"], W, out=nrm, d=d), 5)
     n_fd = nrm.clone()
     # angular error against the FP64 dual-number oracle on a seeded subsample
     res = {"workload": "2^22 shell points, power U[6,10], 15 it", "ms_forward": t_fwd,
@@ -488,9 +489,9 @@ def bench_c4(ctx, args):
            "speedup_tangent_vs_fd": t_fd / t_tan, "speedup_vjp_vs_fd": t_fd / t_vjp}
     try:
         import oracle as O
-        idx = np.sort(np.random.default_rng(0).choice(c4["n"], 20000, replace=False))
-        _, g64, it64 = O.de_grad_f64(O.mandelbulb_trig(c4["max_iters"], c4["bailout"]), pts[idx], powers[idx])
-        _, it_o, _, _ = O.de_points(O.mandelbulb_trig(c4["max_iters"], c4["bailout"]), pts[idx], powers[idx])
+        idx = np.sort(np.random.default_rng(0).choice(sc["n"], 20000, replace=False))
+        _, g64, it64 = O.de_grad_f64(O.mandelbulb_trig(sc["max_iters"], sc["bailout"]), pts[idx], powers[idx])
+        _, it_o, _, _ = O.de_points(O.mandelbulb_trig(sc["max_iters"], sc["bailout"]), pts[idx], powers[idx])
 
         def ang(a):
             a = a.astype(np.float64)
@@ -508,7 +509,7 @@ def bench_c4(ctx, args):
     return res
 
 
-def bench_c5(ctx, args):
+def bench_jobs(ctx, args):
     """64 jobs: N from {32,48,64,96,128}, power U[2,12], cap U{8..30} (seed 5);
     distances + unit normals in one persistent launch (captured in a HIP graph)."""
     import numpy as np
@@ -519,8 +520,8 @@ def bench_c5(ctx, args):
     from cantucci_amd import workloads as wl
 
     autotune.apply(ctx, "jobs")
-    N, Pw, cap = wl.mixed_jobs(**{k: v for k, v in wl.C5.items() if k not in ("bailout",)})
-    js = dev.JobSet(ctx, _lib.CTH_SHAPE_MANDELBULB_TRIG, wl.C5["bailout"],
+    N, Pw, cap = wl.mixed_jobs(**{k: v for k, v in wl.MIXED_JOBS.items() if k not in ("bailout",)})
+    js = dev.JobSet(ctx, _lib.CTH_SHAPE_MANDELBULB_TRIG, wl.MIXED_JOBS["bailout"],
                     [dict(n=int(n), power=float(p), max_iters=int(c)) for n, p, c in zip(N, Pw, cap)])
     dists = torch.empty(js.total, device="cuda")
     nrm = torch.empty((js.total, 3), device="cuda")

@@ -120,7 +120,7 @@ def _workloads(ctx):
     m8 = Mandelbulb.classic(15, 2.0, ctx=ctx)
     runs["grid"] = lambda: dev.de_grid(m8, (-1.5,) * 3, (1.5,) * 3, 64, 63.0)
     hd = wl.half_diagonals(3.0, 8)
-    leaves, _ = dev.refine(m8, wl.C2["root"], 2, 8, hd)
+    leaves, _ = dev.refine(m8, wl.OCTREE["root"], 2, 8, hd)
     out = torch.empty((leaves.shape[0], 729), device="cuda")
     runs["spans"] = lambda: dev.sample_spans(m8, leaves, 8, with_cross=False, out=out)
     dev.sample_spans(m8, leaves, 8, with_cross=False, out=out)

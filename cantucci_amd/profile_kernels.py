@@ -31,8 +31,8 @@ def main() -> None:
     if "spans" in legs:
         autotune.apply(ctx, "spans")
         autotune.apply(ctx, "mesh")
-        c2 = wl.C2
-        leaves, _ = dev.refine(m8, c2["root"], c2["min_depth"], c2["max_depth"], wl.half_diagonals(3.0, 8))
+        oc = wl.OCTREE
+        leaves, _ = dev.refine(m8, oc["root"], oc["min_depth"], oc["max_depth"], wl.half_diagonals(3.0, 8))
         out = torch.empty((leaves.shape[0], 729), device="cuda")
         for _ in range(args.reps):
             dev.sample_spans(m8, leaves, 8, with_cross=False, out=out)

@@ -28,14 +28,14 @@ def uniform(seed: int, n: int, lo=0.0, hi=1.0, offset: int = 0) -> np.ndarray:
 
 
 # ---------------------------------------------------------------------------
-# C1: plain grid.  64^3 over [-1.5, 1.5]^3, p = s + (e - s) * (i / 63)
+# Plain grid: 64^3 over [-1.5, 1.5]^3, p = s + (e - s) * (i / 63)
 # ---------------------------------------------------------------------------
-C1 = dict(n=64, start=(-1.5, -1.5, -1.5), end=(1.5, 1.5, 1.5), divisor=63.0, power=8, max_iters=15, bailout=2.0)
+GRID64 = dict(n=64, start=(-1.5, -1.5, -1.5), end=(1.5, 1.5, 1.5), divisor=63.0, power=8, max_iters=15, bailout=2.0)
 
 # ---------------------------------------------------------------------------
-# C2: octree refinement to depth 8 then 8^3 spans per surface leaf
+# Octree workload: refinement to depth 8 then 8^3 spans per surface leaf
 # ---------------------------------------------------------------------------
-C2 = dict(root=(-1.5, -1.5, -1.5, 1.5, 1.5, 1.5), min_depth=2, max_depth=8, res=8, power=8, max_iters=15,
+OCTREE = dict(root=(-1.5, -1.5, -1.5, 1.5, 1.5, 1.5), min_depth=2, max_depth=8, res=8, power=8, max_iters=15,
           bailout=2.0)
 
 
@@ -45,9 +45,9 @@ def half_diagonals(width: float = 3.0, max_depth: int = 8) -> np.ndarray:
 
 
 # ---------------------------------------------------------------------------
-# C3: 1920x1080 primary rays for 32 orbit poses
+# Camera tracing: 1920x1080 primary rays for 32 orbit poses
 # ---------------------------------------------------------------------------
-C3 = dict(W=1920, H=1080, poses=32, seed=0, fov_deg=60.0, max_steps=256, eps_rel=1e-4, t_max=10.0, power=8,
+TRACE_1080P = dict(W=1920, H=1080, poses=32, seed=0, fov_deg=60.0, max_steps=256, eps_rel=1e-4, t_max=10.0, power=8,
           max_iters=15, bailout=2.0)
 
 THETA_SAFE = 0.02  # src/camera.rs:10
@@ -89,9 +89,9 @@ def orbit_poses(n=32, seed=0, W=1920, H=1080, fov_deg=60.0) -> np.ndarray:
 
 
 # ---------------------------------------------------------------------------
-# C4: 2^22 points in the shell 0.9 < |p| < 1.3, power U[6, 10]
+# Shell normals: 2^22 points in the shell 0.9 < |p| < 1.3, power U[6, 10]
 # ---------------------------------------------------------------------------
-C4 = dict(n=1 << 22, seed=1, r_lo=0.9, r_hi=1.3, p_lo=6.0, p_hi=10.0, max_iters=15, bailout=2.0, fd_h=1e-3)
+SHELL_NORMALS = dict(n=1 << 22, seed=1, r_lo=0.9, r_hi=1.3, p_lo=6.0, p_hi=10.0, max_iters=15, bailout=2.0, fd_h=1e-3)
 
 
 def shell_points(n: int, seed: int = 1, r_lo=0.9, r_hi=1.3, p_lo=6.0, p_hi=10.0):
@@ -107,9 +107,9 @@ def shell_points(n: int, seed: int = 1, r_lo=0.9, r_hi=1.3, p_lo=6.0, p_hi=10.0)
 
 
 # ---------------------------------------------------------------------------
-# C5: 64 independent grid jobs of mixed size, power and iteration cap
+# Mixed jobs: 64 independent grid jobs of mixed size, power and iteration cap
 # ---------------------------------------------------------------------------
-C5 = dict(jobs=64, seed=5, sizes=(32, 48, 64, 96, 128), p_lo=2.0, p_hi=12.0, cap_lo=8, cap_hi=30, bailout=2.0)
+MIXED_JOBS = dict(jobs=64, seed=5, sizes=(32, 48, 64, 96, 128), p_lo=2.0, p_hi=12.0, cap_lo=8, cap_hi=30, bailout=2.0)
 
 
 def mixed_jobs(jobs=64, seed=5, sizes=(32, 48, 64, 96, 128), p_lo=2.0, p_hi=12.0, cap_lo=8, cap_hi=30):

@@ -715,7 +715,7 @@ void orc_de_grad_f64(const orc_shape* shape, const double* pts, const float* pow
     });
 }
 
-// Regular grid (C1): p = start + (end - start) * (i / divisor), GridTable order
+// Regular grid (the 64^3 grid workload): p = start + (end - start) * (i / divisor), GridTable order
 // (x slowest, z fastest).  The reference's own form of a grid coordinate.
 void orc_de_grid_f32(const orc_shape* shape, const float* start, const float* end, uint32_t n, float divisor,
                      float* d, uint32_t* iters, int threads) {
@@ -733,7 +733,7 @@ void orc_de_grid_f32(const orc_shape* shape, const float* start, const float* en
     });
 }
 
-// Span sampling + sign classification (C2): for each span (start xyz, end xyz),
+// Span sampling + sign classification (the octree workload): for each span (start xyz, end xyz),
 // the (res+1)^3 corner distances of naive surface nets step 1 and one crossing
 // bit per cell (bit = cell linear index (x*res + y)*res + z, 64 bits a word).
 void orc_sample_spans(const orc_shape* shape, const float* spans, size_t nspans, uint32_t res, float* dists,
@@ -858,7 +858,7 @@ void orc_create_spans(const float* span, float* out48) {
     std::memcpy(out48, o, sizeof(o));
 }
 
-// Level-synchronous refinement (the C2 policy; the reference's own policy is
+// Level-synchronous refinement (the octree workload's policy; the reference's own policy is
 // camera driven, src/mesh/mod.rs:91-104): nodes of depth < min_depth always
 // split (the reference pre-splits two levels, src/mesh/mod.rs:52-56); deeper
 // nodes split when |DE(center)| < half_diag[depth] or DE is NaN.  Children are

@@ -21,7 +21,7 @@ def angle(a, b):
 
 
 def test_trig_vjp_matches_dual_oracle(ctx, torch_cuda):
-    """C4 inputs: shell 0.9 < |p| < 1.3 (seed 1), power U[6, 10], 15 iterations."""
+    """Shell-normals inputs: shell 0.9 < |p| < 1.3 (seed 1), power U[6, 10], 15 iterations."""
     import cantucci_amd as ca
     from cantucci_amd import device as dev
     from cantucci_amd.workloads import shell_points

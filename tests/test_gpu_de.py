@@ -12,7 +12,7 @@ import oracle as O
 pytestmark = pytest.mark.gpu
 
 F32 = np.float32
-C1 = dict(start=(-1.5, -1.5, -1.5), end=(1.5, 1.5, 1.5))
+GRID = dict(start=(-1.5, -1.5, -1.5), end=(1.5, 1.5, 1.5))
 
 
 def ulp_dist(a, b):
@@ -70,7 +70,7 @@ def ulp_stable(shape, pts, d_ref, powers=None, tol=1e-6):
     return ok
 
 
-def grid_points(n, divisor, start=C1["start"], end=C1["end"]):
+def grid_points(n, divisor, start=GRID["start"], end=GRID["end"]):
     """The reference's corner formula p = s + (e - s) * (i / divisor) in f32."""
     t = np.arange(n, dtype=F32) / F32(divisor)
     ax = [F32(start[a]) + (F32(end[a]) - F32(start[a])) * t for a in range(3)]
@@ -83,16 +83,16 @@ def gpu(torch, a):
 
 
 @pytest.mark.parametrize("divisor", [63.0, 64.0])
-def test_c1_grid_p8_parity(ctx, torch_cuda, divisor):
+def test_grid64_p8_parity(ctx, torch_cuda, divisor):
     """64^3 grid, P8, 15 iterations, bailout 2 (divisor 64 puts the origin and
     64 z-axis points on the grid)."""
     import cantucci_amd as ca
     from cantucci_amd import device as dev
     torch = torch_cuda
     shape = ca.Mandelbulb.classic(15, 2.0, ctx=ctx)
-    d, it = dev.de_grid(shape, C1["start"], C1["end"], 64, divisor, with_iters=True)
+    d, it = dev.de_grid(shape, GRID["start"], GRID["end"], 64, divisor, with_iters=True)
     torch.cuda.synchronize()
-    dref, itref = O.de_grid(O.mandelbulb(15, 2.0), C1["start"], C1["end"], 64, divisor)
+    dref, itref = O.de_grid(O.mandelbulb(15, 2.0), GRID["start"], GRID["end"], 64, divisor)
     assert (it.cpu().numpy().astype(np.uint32) == itref).all()
     assert_de_parity(d.cpu().numpy(), dref)
     # the loop state from explicit points with the same coordinates is bit-identical
@@ -232,7 +232,7 @@ def test_empty_and_invalid_device_calls(ctx, torch_cuda):
         dev.sample_spans(shape, torch.zeros((1, 6), device="cuda"), 6)  # res not a power of two
 
 
-def c2_leaf_sample(count=400, seed=0):
+def octree_leaf_sample(count=400, seed=0):
     from cantucci_amd.workloads import half_diagonals
     leaves, _ = O.refine(O.mandelbulb(15, 2.0), [-1.5, -1.5, -1.5, 1.5, 1.5, 1.5], 2, 6, half_diagonals(3.0, 6))
     idx = np.random.default_rng(seed).choice(len(leaves), min(count, len(leaves)), replace=False)
@@ -241,12 +241,12 @@ def c2_leaf_sample(count=400, seed=0):
 
 @pytest.mark.parametrize("res", [8, 4, 16])
 def test_span_sampling_and_sign_bits(ctx, torch_cuda, res):
-    """Octree-leaf spans (depth-6 surface leaves of the C2 refinement): corner
+    """Octree-leaf spans (depth-6 surface leaves of the octree workload's refinement): corner
     distances at parity and crossing bits identical to the oracle."""
     import cantucci_amd as ca
     from cantucci_amd import device as dev
     torch = torch_cuda
-    spans = c2_leaf_sample(300 if res <= 8 else 60)
+    spans = octree_leaf_sample(300 if res <= 8 else 60)
     shape = ca.Mandelbulb.classic(15, 2.0, ctx=ctx)
     d, cb = dev.sample_spans(shape, gpu(torch, spans), res)
     torch.cuda.synchronize()
@@ -279,9 +279,9 @@ def test_fast_mode_error_is_reported(torch_cuda):
     torch = torch_cuda
     fctx = ca.Context(0, "fast")
     shape = ca.Mandelbulb.classic(15, 2.0, ctx=fctx)
-    d, it = dev.de_grid(shape, C1["start"], C1["end"], 64, 63.0, with_iters=True)
+    d, it = dev.de_grid(shape, GRID["start"], GRID["end"], 64, 63.0, with_iters=True)
     torch.cuda.synchronize()
-    dref, itref = O.de_grid(O.mandelbulb(15, 2.0), C1["start"], C1["end"], 64, 63.0)
+    dref, itref = O.de_grid(O.mandelbulb(15, 2.0), GRID["start"], GRID["end"], 64, 63.0)
     d = d.cpu().numpy()
     rel = np.abs(d.astype(np.float64) - dref) / np.maximum(np.abs(dref.astype(np.float64)), 1e-30)
     frac = np.mean(rel > 1e-5)

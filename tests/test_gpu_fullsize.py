@@ -37,17 +37,17 @@ def test_octree_workload_full_size(ctx, torch_cuda):
     from cantucci_amd import device as dev
     from cantucci_amd import workloads as wl
     torch = torch_cuda
-    c2 = wl.C2
-    hd = wl.half_diagonals(3.0, c2["max_depth"])
-    shape = ca.Mandelbulb.classic(c2["max_iters"], c2["bailout"], ctx=ctx)
-    leaves, lc = dev.refine(shape, c2["root"], c2["min_depth"], c2["max_depth"], hd)
+    oc = wl.OCTREE
+    hd = wl.half_diagonals(3.0, oc["max_depth"])
+    shape = ca.Mandelbulb.classic(oc["max_iters"], oc["bailout"], ctx=ctx)
+    leaves, lc = dev.refine(shape, oc["root"], oc["min_depth"], oc["max_depth"], hd)
     torch.cuda.synchronize()
-    leaves_o, lc_o = O.refine(O.mandelbulb(15, 2.0), c2["root"], c2["min_depth"], c2["max_depth"], hd)
+    leaves_o, lc_o = O.refine(O.mandelbulb(15, 2.0), oc["root"], oc["min_depth"], oc["max_depth"], hd)
     assert list(lc) == list(lc_o) and int(lc[-1]) == 1252864
     lv = leaves.cpu().numpy()
     assert lv.shape == (921604, 6) and (lv.view(np.uint32) == leaves_o.view(np.uint32)).all()
 
-    res = c2["res"]
+    res = oc["res"]
     dists, cross = dev.sample_spans(shape, leaves, res)
     cross2 = torch.full_like(cross, -1)
     voff, ioff, verts, idx = dev.mesh(shape, leaves, dists, res, "fd6", cross=cross2)
@@ -95,9 +95,9 @@ def test_camera_trace_full_size(ctx, torch_cuda):
     from cantucci_amd import device as dev
     from cantucci_amd import workloads as wl
     torch = torch_cuda
-    c3 = wl.C3
-    W, H = c3["W"], c3["H"]
-    cams = wl.orbit_poses(c3["poses"], c3["seed"], W, H, c3["fov_deg"])
+    tc = wl.TRACE_1080P
+    W, H = tc["W"], tc["H"]
+    cams = wl.orbit_poses(tc["poses"], tc["seed"], W, H, tc["fov_deg"])
     shape = ca.Mandelbulb.classic(15, 2.0, ctx=ctx)
     depth, steps = dev.render_depth(shape, gpu(torch, cams), W, H)
     torch.cuda.synchronize()
@@ -121,12 +121,12 @@ def test_shell_normals_full_size(ctx, torch_cuda):
     from cantucci_amd import device as dev
     from cantucci_amd import workloads as wl
     torch = torch_cuda
-    c4 = wl.C4
-    pts, powers = wl.shell_points(c4["n"], c4["seed"])
+    sc = wl.SHELL_NORMALS
+    pts, powers = wl.shell_points(sc["n"], sc["seed"])
     shape = ca.MandelbulbTrig(15, 2.0, 8.0, ctx=ctx)
     _, nrm = dev.normals(shape, gpu(torch, pts), gpu(torch, powers))
     torch.cuda.synchronize()
-    sel = np.random.default_rng(2).choice(c4["n"], 20000, replace=False)
+    sel = np.random.default_rng(2).choice(sc["n"], 20000, replace=False)
     n = nrm[torch.from_numpy(sel).cuda()].cpu().numpy().astype(np.float64)
     _, g64, _ = O.de_grad_f64(O.mandelbulb_trig(15, 2.0), pts[sel], powers[sel])
     ok = np.isfinite(g64[:, :3]).all(1) & (np.linalg.norm(g64[:, :3], axis=1) > 0)
@@ -143,7 +143,7 @@ def test_mixed_jobs_full_size(ctx, torch_cuda):
     from cantucci_amd import device as dev
     from cantucci_amd import workloads as wl
     torch = torch_cuda
-    N, Pw, cap = wl.mixed_jobs(**{k: v for k, v in wl.C5.items() if k != "bailout"})
+    N, Pw, cap = wl.mixed_jobs(**{k: v for k, v in wl.MIXED_JOBS.items() if k != "bailout"})
     jobs = [dict(n=int(n), power=float(p), max_iters=int(c)) for n, p, c in zip(N, Pw, cap)]
     js = dev.JobSet(ctx, _lib.CTH_SHAPE_MANDELBULB_TRIG, 2.0, jobs)
     d = torch.empty(js.total, device="cuda")

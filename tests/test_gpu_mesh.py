@@ -33,7 +33,7 @@ def angle(a, b):
     return np.arccos(np.clip(c, -1, 1))
 
 
-def c2_leaves(count=2000, seed=0, depth=8):
+def octree_leaves(count=2000, seed=0, depth=8):
     """A seeded sample of the depth-8 surface leaves of the octree workload."""
     from cantucci_amd.workloads import half_diagonals
     leaves, _ = O.refine(O.mandelbulb(15, 2.0), ROOT, 2, depth, half_diagonals(3.0, depth))
@@ -106,7 +106,7 @@ def test_mesh_signed_zero_and_nan_corners(ctx, torch_cuda, res):
     from cantucci_amd import device as dev
     torch = torch_cuda
     rng = np.random.default_rng(50 + res)
-    spans = c2_leaves(96, seed=7)
+    spans = octree_leaves(96, seed=7)
     n1 = res + 1
     vals = np.array([-1.0, -0.25, -0.0, 0.0, 0.25, 1.0, np.nan, -np.nan], F32)
     assert np.signbit(vals[2]) and np.signbit(vals[7]) and not np.signbit(vals[6])
@@ -128,12 +128,12 @@ def test_mesh_signed_zero_and_nan_corners(ctx, torch_cuda, res):
     assert pos_same.all()
 
 
-def test_c2_leaf_mesh_parity(ctx, torch_cuda):
+def test_octree_leaf_mesh_parity(ctx, torch_cuda):
     """2,000 seeded depth-8 surface leaves of the octree workload (P8, 15 it,
     bailout 2, 8^3 cells per leaf)."""
     import cantucci_amd as ca
     torch = torch_cuda
-    spans = c2_leaves(2000)
+    spans = octree_leaves(2000)
     shape = ca.Mandelbulb.classic(15, 2.0, ctx=ctx)
     d, voff, ioff, v, ix = run_gpu_mesh(torch, shape, spans, 8)
     voff_o, ioff_o, v_o, ix_o = O.mesh_batch(O.mandelbulb(15, 2.0), spans, 8, dists=d)
@@ -177,7 +177,7 @@ def test_tangent_normals_on_mesh(ctx, torch_cuda):
     mesh, normals within 1e-3 rad of the FP64 dual-number gradient."""
     import cantucci_amd as ca
     torch = torch_cuda
-    spans = c2_leaves(500, seed=3)
+    spans = octree_leaves(500, seed=3)
     shape = ca.Mandelbulb.classic(15, 2.0, ctx=ctx)
     _, voff, ioff, v_fd, ix_fd = run_gpu_mesh(torch, shape, spans, 8, "fd6")
     _, voff_t, ioff_t, v_t, ix_t = run_gpu_mesh(torch, shape, spans, 8, "tangent")
