@@ -225,9 +225,11 @@ struct P8Exact {
         // step of the unscaled division over- or underflows unless
         // |num / w8| < 2^-25, where 1 + num / w8 rounds to 1 either way; a is
         // then the same float as with the full sequence.  The same test
-        // covers the square root's w2 >= 2^-96.
+        // covers the square root's w2 >= 2^-96, and it rules out the z axis
+        // (x = y = 0 gives w2 = 0).
         float q, sw2;
-        if (u.bailout <= 16.0f && wave_mask(!(w2 >= 0x1p-15f)) == 0) {
+        const bool unscaled = u.bailout <= 16.0f && wave_mask(!(w2 >= 0x1p-15f)) == 0;
+        if (unscaled) {
             q = div_rn_unscaled(num, w8);
             sw2 = sqrt_rn_unscaled(w2);
         } else {
@@ -238,21 +240,24 @@ struct P8Exact {
         float nx = a * ((((x8 - (28.0f * x6) * y2) + (70.0f * x4) * y4) - (28.0f * x2) * y6) - y8);
         float ny = (((8.0f * a) * x) * y) * (((x6 - (7.0f * x4) * y2) + (7.0f * x2) * y4) - y6);
         float nz = (((8.0f * z) * sw2) * (z2 - w2)) * ((z4 - (6.0f * z2) * w2) + w4);
-        // The wave test looks at every lane (a compare's lane mask is free;
-        // folding `live` in first would cost two VALU instructions); a lane
-        // without a live point that happens to sit on the axis only sends the
-        // wave through the branch, where it does nothing.
-        const bool axis = on_z_axis(x, y);
-        if (wave_mask(axis)) {
-            if (live && axis) {
-                // rotate_on_z_axis::<8> (:114-126)
-                float c = div_rn(z, r);
-                if (__builtin_isinf(c)) pos_nan = true;
-                float th = acosf(c);
-                float r8 = r4 * r4;
-                nx = 0.0f;
-                ny = 0.0f;
-                nz = r8 * cosf(th * 8.0f);
+        // On the z axis (only reachable with the scaled sequence).  The wave
+        // test looks at every lane (a compare's lane mask is free; folding
+        // `live` in first would cost two VALU instructions); a lane without a
+        // live point that happens to sit on the axis only sends the wave
+        // through the branch, where it does nothing.
+        if (!unscaled) {
+            const bool axis = on_z_axis(x, y);
+            if (wave_mask(axis)) {
+                if (live && axis) {
+                    // rotate_on_z_axis::<8> (:114-126)
+                    float c = div_rn(z, r);
+                    if (__builtin_isinf(c)) pos_nan = true;
+                    float th = acosf(c);
+                    float r8 = r4 * r4;
+                    nx = 0.0f;
+                    ny = 0.0f;
+                    nz = r8 * cosf(th * 8.0f);
+                }
             }
         }
         if (live) {

@@ -55,7 +55,11 @@ __global__ __launch_bounds__(256) void k_jobs(const JobDev* __restrict__ jobs, u
     // live / holding lanes as wave-uniform lane masks (see refill_loop)
     const uint64_t wave = wave_mask(true);
     uint64_t live_m = 0, hold_m = 0;
-    uint32_t idx = 0, k = 0, job = 0, kcap = 1;
+    uint64_t capd_m = 0;  // held lanes whose point stopped at its job's iteration cap
+    uint32_t idx = 0, job = 0, kcap = 1;
+    // The wave's body count, and per lane the tick of its point's last body
+    // (the job's cap counted from the tick at which the lane took the point).
+    uint32_t tick = 0, tcap = 0;
     float r = 0.0f;
     uint32_t c_next = 0, c_end = 0, round = 0;
     bool exhausted = false;
@@ -64,9 +68,10 @@ __global__ __launch_bounds__(256) void k_jobs(const JobDev* __restrict__ jobs, u
     for (;;) {
         if ((uint32_t)__popcll(wave & ~live_m) >= q.refill_min) {
             // write back held points, then stamp their jobs (one atomic per job per wave)
+            // SinkGrad reads k only as k == max_iters (the point stopped at the cap).
             if (__builtin_amdgcn_inverse_ballot_w64(hold_m)) {
                 sink.max_iters = kcap;
-                sink.put(idx, ev.finish(r), k, r, ev.dr, ev);
+                sink.put(idx, ev.finish(r), __builtin_amdgcn_inverse_ballot_w64(capd_m) ? kcap : 0u, r, ev.dr, ev);
             }
             uint64_t pend = hold_m;
             if (pend) {
@@ -80,6 +85,7 @@ __global__ __launch_bounds__(256) void k_jobs(const JobDev* __restrict__ jobs, u
                 }
             }
             hold_m = 0;
+            capd_m = 0;
             while (!exhausted) {
                 const uint64_t need = wave & ~live_m;
                 if (need == 0) break;
@@ -120,8 +126,8 @@ __global__ __launch_bounds__(256) void k_jobs(const JobDev* __restrict__ jobs, u
                             c[a] = J.start[a] + J.across[a] * v;
                         }
                         kcap = J.max_iters;
+                        tcap = tick + kcap - 1u;
                         ev.start(c[0], c[1], c[2], J.power, u);
-                        k = 0;
                         r = ev.mag();
                         if (r > u.bailout) {
                             sink.max_iters = kcap;
@@ -135,14 +141,15 @@ __global__ __launch_bounds__(256) void k_jobs(const JobDev* __restrict__ jobs, u
             if (live_m == 0) break;
         }
         const bool live = __builtin_amdgcn_inverse_ballot_w64(live_m);
+        const uint64_t cap_m = wave_mask(tcap == tick) & live_m;
         ev.body(r, u, live);
-        k += live ? 1u : 0u;
-        const uint64_t cap_m = wave_mask(k == kcap) & live_m;
+        ++tick;
         const float rn = ev.mag();
         const uint64_t bail_m = wave_mask(rn > u.bailout) & live_m & ~cap_m;
         if (__builtin_amdgcn_inverse_ballot_w64(live_m & ~cap_m)) r = rn;
         live_m &= ~(cap_m | bail_m);
         hold_m |= cap_m | bail_m;
+        capd_m |= cap_m;
     }
     if (q.dynamic) wave_retire(q);
 }

@@ -741,6 +741,7 @@ struct SrcMeshVerts {
 struct SinkMeshVerts {
     float* verts;
     uint32_t max_iters;
+    __device__ __forceinline__ bool wants_k() const { return true; }
     template <class E>
     __device__ __forceinline__ void put(uint32_t i, float dv, uint32_t k, float r, float, const E& ev) const {
         float g[4];
@@ -792,7 +793,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
     const uint64_t wave = wave_mask(true);
     uint64_t live_m = 0, hold_m = 0;
     bool has_v = false;
-    uint32_t vid = 0, t = 0, k = 0;
+    uint32_t vid = 0, t = 0;
+    uint32_t tick = 0, t0 = 0;  // wave's body count; tick at which the lane's tap started
     float r = 0.0f;
     uint32_t c_next = 0, c_end = 0, round = 0;
     bool exhausted = false;
@@ -855,7 +857,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
                         for (int b = 0; b < 3; ++b) c[b] = c[b] + ((a == b) ? 1.0f : 0.0f) * h;
                     }
                     ev.start(c[0], c[1], c[2], 0.0f, u);
-                    k = 0;
+                    t0 = tick;
                     r = ev.mag();
                     if (r > u.bailout) {
                         slot[256 * (6 + t)] = ev.finish(r);
@@ -867,9 +869,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
             if (live_m == 0) break;
         }
         const bool live = __builtin_amdgcn_inverse_ballot_w64(live_m);
+        const uint64_t cap_m = wave_mask(t0 == tick + 1u - u.max_iters) & live_m;  // the max_iters-th body
         ev.body(r, u, live);
-        k += live ? 1u : 0u;
-        const uint64_t cap_m = wave_mask(k == u.max_iters) & live_m;
+        ++tick;
         const float rn = ev.mag();
         const uint64_t bail_m = wave_mask(rn > u.bailout) & live_m & ~cap_m;
         // at the cap r stays the magnitude before the last body (src/shape/mandelbulb.rs:65-78)

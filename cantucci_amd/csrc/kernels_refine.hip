@@ -37,6 +37,7 @@ struct SrcCenters {
 struct SinkSplit {
     uint32_t* flags;
     float half_diag;
+    __device__ __forceinline__ bool wants_k() const { return false; }
     template <class E>
     __device__ __forceinline__ void put(uint32_t i, float d, uint32_t, float, float, const E&) const {
         flags[i] = ((d != d) || (fabsf(d) < half_diag)) ? 1u : 0u;

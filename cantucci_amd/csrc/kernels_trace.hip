@@ -88,7 +88,8 @@ __global__ __launch_bounds__(256) void k_trace(Rays rays, MandelUniform u, Trace
     const uint64_t wave = wave_mask(true);
     uint64_t live_m = 0, pend_m = 0;
     bool has_ray = false;
-    uint32_t idx = 0, k = 0, nsteps = 0;
+    uint32_t idx = 0, nsteps = 0;
+    uint32_t tick = 0, t0 = 0;  // wave's body count; tick at which the lane's evaluation started
     float r = 0.0f, t = 0.0f;
     float pos[3] = {0.0f, 0.0f, 0.0f}, dir[3] = {0.0f, 0.0f, 0.0f};
     uint32_t c_next = 0, c_end = 0, round = 0;
@@ -150,7 +151,7 @@ __global__ __launch_bounds__(256) void k_trace(Rays rays, MandelUniform u, Trace
                 const uint64_t start_m = wave_mask(has_ray) & ~live_m;
                 if (__builtin_amdgcn_inverse_ballot_w64(start_m)) {
                     ev.start(pos[0], pos[1], pos[2], u.power, u);
-                    k = 0;
+                    t0 = tick;
                     r = ev.mag();
                 }
                 // zero loop bodies (|p| > bailout): finished already
@@ -162,9 +163,9 @@ __global__ __launch_bounds__(256) void k_trace(Rays rays, MandelUniform u, Trace
             if (live_m == 0) break;
         }
         const bool live = __builtin_amdgcn_inverse_ballot_w64(live_m);
+        const uint64_t cap_m = wave_mask(t0 == tick + 1u - u.max_iters) & live_m;  // the max_iters-th body
         ev.body(r, u, live);
-        k += live ? 1u : 0u;
-        const uint64_t cap_m = wave_mask(k == u.max_iters) & live_m;
+        ++tick;
         const float rn = ev.mag();
         const uint64_t bail_m = wave_mask(rn > u.bailout) & live_m & ~cap_m;
         if (__builtin_amdgcn_inverse_ballot_w64(live_m & ~cap_m)) r = rn;

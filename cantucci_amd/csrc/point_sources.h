@@ -113,11 +113,13 @@ struct SrcSpans {
 };
 
 // ---------------------------------------------------------------------------
-// Sinks
+// Sinks.  wants_k(): whether put() reads the loop-body count k (the work
+// loop only keeps a finished point's count for sinks that do).
 // ---------------------------------------------------------------------------
 struct SinkDist {
     float* out;
     uint32_t* iters;
+    __device__ __forceinline__ bool wants_k() const { return iters != nullptr; }
     template <class E>
     __device__ __forceinline__ void put(uint32_t i, float d, uint32_t k, float, float, const E&) const {
         out[i] = d;
@@ -129,6 +131,7 @@ struct SinkState {
     float* r;
     float* dr;
     uint32_t* iters;
+    __device__ __forceinline__ bool wants_k() const { return true; }
     template <class E>
     __device__ __forceinline__ void put(uint32_t i, float, uint32_t k, float rr, float drr, const E&) const {
         r[i] = rr;
@@ -146,6 +149,7 @@ struct SinkGrad {
     float* grad_power;
     const float* grad_d;
     uint32_t max_iters;
+    __device__ __forceinline__ bool wants_k() const { return true; }
     template <class E>
     __device__ __forceinline__ void put(uint32_t i, float dv, uint32_t k, float r, float, const E& ev) const {
         float g[4];

@@ -86,8 +86,9 @@ __device__ __forceinline__ void wave_retire(const WorkQueue& q) {
 // Which lanes are live or hold a finished point is kept in wave-uniform lane
 // masks (scalar registers).  A compare's lane mask is free to combine with
 // them, and the per-lane view is the same mask
-// (__builtin_amdgcn_inverse_ballot_w64), so the tick spends no vector
-// instructions on bookkeeping beyond the iteration counter.
+// (__builtin_amdgcn_inverse_ballot_w64).  A point's body count is the wave's
+// tick counter (scalar) minus the tick at which the lane took the point, so
+// the iteration cap costs one compare per tick and no per-lane counter.
 template <class Ev, class Src, class Sink>
 __device__ __forceinline__ void refill_loop(const Src& src, const MandelUniform& u, const Sink& sink,
                                             const WorkQueue& q) {
@@ -95,7 +96,10 @@ __device__ __forceinline__ void refill_loop(const Src& src, const MandelUniform&
     const uint64_t wave = wave_mask(true);  // the wave's lanes
     uint64_t live_m = 0;  // lanes whose point needs another loop body
     uint64_t hold_m = 0;  // lanes holding a finished point not yet written
-    uint32_t idx = 0, k = 0;
+    uint32_t tick = 0;    // loop bodies run by the wave
+    uint32_t idx = 0;
+    uint32_t t0 = 0;      // tick at which the lane took its point
+    uint32_t kh = 0;      // body count of a held point (kept if the sink reads it)
     float r = 0.0f;
     uint32_t c_next = 0, c_end = 0;  // wave-uniform window of claimed indices
     uint32_t round = 0;
@@ -103,7 +107,7 @@ __device__ __forceinline__ void refill_loop(const Src& src, const MandelUniform&
 
     for (;;) {
         if ((uint32_t)__popcll(wave & ~live_m) >= q.refill_min) {
-            if (__builtin_amdgcn_inverse_ballot_w64(hold_m)) sink.put(idx, ev.finish(r), k, r, ev.dr, ev);
+            if (__builtin_amdgcn_inverse_ballot_w64(hold_m)) sink.put(idx, ev.finish(r), kh, r, ev.dr, ev);
             hold_m = 0;
             // Hand new points to idle lanes.  A point that bails before its
             // first body (|p| > bailout) is written on the spot and the lane
@@ -129,7 +133,7 @@ __device__ __forceinline__ void refill_loop(const Src& src, const MandelUniform&
                     float px, py, pz, pw;
                     src.get(idx, px, py, pz, pw);
                     ev.start(px, py, pz, pw, u);
-                    k = 0;
+                    t0 = tick;
                     r = ev.mag();
                 }
                 const uint64_t took = wave_mask(take);
@@ -141,16 +145,21 @@ __device__ __forceinline__ void refill_loop(const Src& src, const MandelUniform&
             if (live_m == 0) break;
         }
         const bool live = __builtin_amdgcn_inverse_ballot_w64(live_m);
+        // Lanes making their max_iters-th body in this tick (max_iters >= 1).
+        const uint64_t cap_m = wave_mask(t0 == tick + 1u - u.max_iters) & live_m;
         ev.body(r, u, live);
-        k += live ? 1u : 0u;
-        const uint64_t cap_m = wave_mask(k == u.max_iters) & live_m;
+        ++tick;
         const float rn = ev.mag();
         const uint64_t bail_m = wave_mask(rn > u.bailout) & live_m & ~cap_m;
         // At the iteration cap, r stays the magnitude computed before the last
         // body (the reference's off-by-one, src/shape/mandelbulb.rs:65-78).
         if (__builtin_amdgcn_inverse_ballot_w64(live_m & ~cap_m)) r = rn;
-        live_m &= ~(cap_m | bail_m);
-        hold_m |= cap_m | bail_m;
+        const uint64_t done_m = cap_m | bail_m;
+        if (done_m != 0 && sink.wants_k()) {
+            if (__builtin_amdgcn_inverse_ballot_w64(done_m)) kh = tick - t0;
+        }
+        live_m &= ~done_m;
+        hold_m |= done_m;
     }
     if (q.dynamic) wave_retire(q);
 }
