@@ -183,6 +183,7 @@ struct MandelUniform {
 // Exact P8 polynomial evaluator (reference `Mandelbulb<8>`)
 // ---------------------------------------------------------------------------
 struct P8Exact {
+    static constexpr int kWavesPerSimd = 8;  // fits 64 VGPRs in the refill kernel
     float px, py, pz, x, y, z, dr;
     float x2, y2, w2, z2;  // squares of the last magnitude, reused by rotate
     bool pos_nan;          // the orbit's first NaN came from acosf(+-inf)
@@ -278,6 +279,7 @@ struct P8Exact {
 // extension with powf).  Transcendentals are ocml's.
 // ---------------------------------------------------------------------------
 struct TrigExact {
+    static constexpr int kWavesPerSimd = 1;  // no constraint (64 VGPRs would spill)
     float px, py, pz, x, y, z, dr, P;
     int Pi;  // integer power for the powi ladder, or -1 for powf
     bool pos_nan;
@@ -360,6 +362,7 @@ struct TrigExact {
 // Fast P8 evaluator (perf mode).  Same map; FMAs, reciprocal, hardware sqrt.
 // ---------------------------------------------------------------------------
 struct P8Fast {
+    static constexpr int kWavesPerSimd = 8;
     float px, py, pz, x, y, z, dr;
     float x2, y2, w2, z2;
 
@@ -423,6 +426,7 @@ __device__ __forceinline__ float sphere_de(float px, float py, float pz, const f
 // so mag() reports +inf (an immediate "bailout") and finish() returns the
 // exact distance.  The centre and radius travel in the uniforms' spare fields.
 struct SphereEv {
+    static constexpr int kWavesPerSimd = 8;
     float px, py, pz, dr;
     float c[3], radius;
     __device__ __forceinline__ void start(float ax, float ay, float az, float, const MandelUniform& u) {

@@ -79,8 +79,8 @@ struct TraceOut {
 };
 
 template <class Ev, class Rays>
-__global__ __launch_bounds__(256) void k_trace(Rays rays, MandelUniform u, TraceParams tp, TraceOut out,
-                                               WorkQueue q) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Ev::kWavesPerSimd))) void k_trace(
+    Rays rays, MandelUniform u, TraceParams tp, TraceOut out, WorkQueue q) {
     Ev ev;
     // Wave-uniform lane masks, as in refill_loop (work_loop.h): lanes with an
     // evaluation in progress, and lanes whose evaluation finished and whose

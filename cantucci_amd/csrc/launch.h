@@ -11,8 +11,11 @@
 
 namespace cth {
 
+// Ev::kWavesPerSimd: the occupancy the evaluator's register budget is fitted
+// to (the P8 evaluators fit 64 VGPRs, 8 waves per SIMD, without spilling).
 template <class Ev, class Src, class Sink>
-__global__ __launch_bounds__(256) void k_refill(Src src, MandelUniform u, Sink sink, WorkQueue q) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Ev::kWavesPerSimd))) void k_refill(
+    Src src, MandelUniform u, Sink sink, WorkQueue q) {
     refill_loop<Ev>(src, u, sink, q);
 }
 

@@ -30,6 +30,7 @@ __device__ __forceinline__ float grad_final(float r, float dr, float rq, float d
 // (equal to rotate_inner_p8_scalar, src/shape/mandelbulb.rs:148-200).
 // ---------------------------------------------------------------------------
 struct P8Tangent : P8Exact {
+    static constexpr int kWavesPerSimd = 1;  // the tangent state does not fit 64 VGPRs
     float T[3][3];    // dz/dp
     float tdr[3];     // d dr / dp
     float rq_prev[3]; // dr/dp of the magnitude entering the last body
@@ -136,6 +137,7 @@ struct P8Tangent : P8Exact {
 template <int NQ>
 struct TrigTangentN : TrigExact {
     static_assert(NQ == 3 || NQ == 4, "3 point columns, optionally the power");
+    static constexpr int kWavesPerSimd = 1;
     float T[3][NQ];
     float tdr[NQ];
     float rq_prev[NQ];
