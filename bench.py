@@ -311,8 +311,21 @@ def bench_grid(ctx, args):
     per_launch = (time.perf_counter() - t) / (reps * launches)
     single = _time_ms(lambda: dev.de_grid(shape, (-1.5,) * 3, (1.5,) * 3, n, 63.0), 20)
     g.close()
+    # The same grid scaled up (divisor m - 1), one eager launch each.
+    scaled = {}
+    for m in (256, 512):
+        ms = _time_ms(lambda: dev.de_grid(shape, (-1.5,) * 3, (1.5,) * 3, m, float(m - 1)), 5)
+        scaled[f"{m}^3"] = {"evals_per_s": m ** 3 / (ms / 1e3), "ms": ms}
+    # Other P8 settings on the 64^3 grid: the reference application's
+    # classic(6, 2.5) and classic(8, 5.0) (max_iters, bailout).
+    variants = {}
+    for mi, bo in ((6, 2.5), (8, 5.0)):
+        other = ca.Mandelbulb.classic(mi, bo, ctx=ctx)
+        ms = _time_ms(lambda: dev.de_grid(other, (-1.5,) * 3, (1.5,) * 3, n, 63.0), 20)
+        variants[f"classic({mi}, {bo})"] = {"us_per_launch_eager": ms * 1e3}
     return {"workload": "64^3 grid, P8, 15 it, bailout 2", "evals_per_s": n ** 3 / per_launch,
-            "us_per_launch_graph": per_launch * 1e6, "us_per_launch_eager": single * 1e3}
+            "us_per_launch_graph": per_launch * 1e6, "us_per_launch_eager": single * 1e3,
+            "scaled": scaled, "other_settings_64^3": variants}
 
 
 def bench_point_query(ctx, args):
