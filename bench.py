@@ -238,29 +238,31 @@ def _host_cpu() -> str:
     return f"{model}, {os.cpu_count()} logical CPUs (threads used: see cores)"
 
 
-def _phases(dev, torch, shape, oc, hd, leaves_buf, dists, cross, res):
-    """Per-phase times of one step (HIP events on torch's current stream)."""
-    out = {}
-    marks = []
+def _phases(dev, torch, shape, oc, hd, leaves_buf, dists, cross, res, reps=3):
+    """Per-phase times of a step, the median of `reps` steps (HIP events on
+    torch's current stream)."""
+    runs = {"refine": [], "sample_spans": [], "mesh": []}
+    for _ in range(reps):
+        marks = []
 
-    def mark():
-        e = torch.cuda.Event(enable_timing=True)
-        e.record()
-        marks.append(e)
+        def mark():
+            e = torch.cuda.Event(enable_timing=True)
+            e.record()
+            marks.append(e)
 
-    torch.cuda.synchronize()
-    mark()
-    lv, _ = dev.refine(shape, oc["root"], oc["min_depth"], oc["max_depth"], hd, leaves=leaves_buf)
-    mark()
-    dev.sample_spans(shape, lv, res, with_cross=False, out=dists)
-    mark()
-    m = dev.mesh(shape, lv, dists, res, "fd6", cross=cross)
-    mark()
-    torch.cuda.synchronize()
-    for name, a, b in (("refine", 0, 1), ("sample_spans", 1, 2), ("mesh", 2, 3)):
-        out[name] = marks[a].elapsed_time(marks[b])
-    del m
-    return out
+        torch.cuda.synchronize()
+        mark()
+        lv, _ = dev.refine(shape, oc["root"], oc["min_depth"], oc["max_depth"], hd, leaves=leaves_buf)
+        mark()
+        dev.sample_spans(shape, lv, res, with_cross=False, out=dists)
+        mark()
+        m = dev.mesh(shape, lv, dists, res, "fd6", cross=cross)
+        mark()
+        torch.cuda.synchronize()
+        for name, a, b in (("refine", 0, 1), ("sample_spans", 1, 2), ("mesh", 2, 3)):
+            runs[name].append(marks[a].elapsed_time(marks[b]))
+        del m
+    return {name: sorted(v)[len(v) // 2] for name, v in runs.items()}
 
 
 def _time_ms(fn, reps):
