@@ -79,7 +79,7 @@ struct TraceOut {
 };
 
 template <class Ev, class Rays>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Ev::kWavesPerSimd))) void k_trace(
+__global__ __launch_bounds__(256) void k_trace(
     Rays rays, MandelUniform u, TraceParams tp, TraceOut out, WorkQueue q) {
     Ev ev;
     // Wave-uniform lane masks, as in refill_loop (work_loop.h): lanes with an
@@ -95,71 +95,76 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Ev::kWavesP
     uint32_t c_next = 0, c_end = 0, round = 0;
     bool exhausted = false;
 
+    // One sphere-tracing step of the lane's ray by distance d.  When the ray
+    // ends (hit, escape or the step limit) its outputs are written and the
+    // lane has no ray.
+    auto step = [&](float d) {
+        pos[0] = pos[0] + dir[0] * d;
+        pos[1] = pos[1] + dir[1] * d;
+        pos[2] = pos[2] + dir[2] * d;
+        t = t + d;
+        ++nsteps;
+        bool hit = d < (tp.eps_abs + tp.eps_rel * t);
+        if (hit || t > tp.t_max || nsteps == tp.max_steps) {
+            out.depth[idx] = hit ? t : __builtin_inff();
+            out.steps[idx] = nsteps;
+            if (out.hit_pos) {
+                out.hit_pos[3 * (size_t)idx] = pos[0];
+                out.hit_pos[3 * (size_t)idx + 1] = pos[1];
+                out.hit_pos[3 * (size_t)idx + 2] = pos[2];
+            }
+            has_ray = false;
+        }
+    };
+
     for (;;) {
         if ((uint32_t)__popcll(wave & ~live_m) >= q.refill_min) {
+            // step_m: lanes whose evaluation finished (after loop bodies, or
+            // with none when the point is beyond the bailout radius) and whose
+            // step is not applied yet; new_m: lanes that just took a ray.
+            // Steps are applied and the next evaluations started over and
+            // over while some lane has a zero-body step; new rays are fetched
+            // only when none has, so that rays are generated in batches.
+            uint64_t step_m = pend_m, new_m = 0;
             for (;;) {
-                // Apply finished evaluations: one sphere-tracing step.
-                if (__builtin_amdgcn_inverse_ballot_w64(pend_m)) {
-                    float d = ev.finish(r);
-                    pos[0] = pos[0] + dir[0] * d;
-                    pos[1] = pos[1] + dir[1] * d;
-                    pos[2] = pos[2] + dir[2] * d;
-                    t = t + d;
-                    ++nsteps;
-                    bool hit = d < (tp.eps_abs + tp.eps_rel * t);
-                    if (hit || t > tp.t_max || nsteps == tp.max_steps) {
-                        out.depth[idx] = hit ? t : __builtin_inff();
-                        out.steps[idx] = nsteps;
-                        if (out.hit_pos) {
-                            out.hit_pos[3 * (size_t)idx] = pos[0];
-                            out.hit_pos[3 * (size_t)idx + 1] = pos[1];
-                            out.hit_pos[3 * (size_t)idx + 2] = pos[2];
-                        }
-                        has_ray = false;
-                    }
-                }
-                pend_m = 0;
-                // New rays for lanes without one.
-                if (!exhausted) {
-                    uint64_t need = wave_mask(!has_ray);
-                    if (need) {
-                        if (c_next >= c_end) {
-                            uint32_t b = next_chunk(q, round);
-                            if (b >= q.total) {
-                                exhausted = true;
-                            } else {
-                                c_next = b;
-                                c_end = (q.total - b < q.chunk) ? q.total : b + q.chunk;
-                            }
-                        }
-                        if (!exhausted) {
-                            uint32_t avail = c_end - c_next;
-                            uint32_t nneed = (uint32_t)__popcll(need);
-                            uint32_t rank = lane_rank(need);
-                            if (!has_ray && rank < avail) {
-                                idx = c_next + rank;
-                                rays.get(idx, pos, dir);
-                                t = 0.0f;
-                                nsteps = 0;
-                                has_ray = true;
-                            }
-                            c_next += (nneed < avail) ? nneed : avail;
-                        }
-                    }
-                }
-                // Start the next evaluation of every ray that has none running.
-                const uint64_t start_m = wave_mask(has_ray) & ~live_m;
+                if (__builtin_amdgcn_inverse_ballot_w64(step_m)) step(ev.finish(r));
+                const uint64_t start_m = (step_m | new_m) & wave_mask(has_ray);
                 if (__builtin_amdgcn_inverse_ballot_w64(start_m)) {
                     ev.start(pos[0], pos[1], pos[2], u.power, u);
                     t0 = tick;
                     r = ev.mag();
                 }
-                // zero loop bodies (|p| > bailout): finished already
-                pend_m = wave_mask(r > u.bailout) & start_m;
-                live_m |= start_m & ~pend_m;
-                const bool more = pend_m != 0 || (!exhausted && wave_mask(!has_ray) != 0);
-                if (!more) break;
+                step_m = wave_mask(r > u.bailout) & start_m;  // zero loop bodies: finished already
+                live_m |= start_m & ~step_m;
+                new_m = 0;
+                if (step_m != 0) continue;
+                // New rays for lanes without one.
+                const uint64_t need = wave_mask(!has_ray);
+                if (exhausted || need == 0) break;
+                if (c_next >= c_end) {
+                    const uint32_t b = next_chunk(q, round);
+                    if (b >= q.total) {
+                        exhausted = true;
+                        break;
+                    }
+                    c_next = b;
+                    c_end = (q.total - b < q.chunk) ? q.total : b + q.chunk;
+                }
+                const uint32_t avail = c_end - c_next;
+                const uint32_t nneed = (uint32_t)__popcll(need);
+                const uint32_t rank = lane_rank(need);
+                const bool take = !has_ray && rank < avail;
+                if (take) {
+                    idx = c_next + rank;
+                    rays.get(idx, pos, dir);
+                    t = 0.0f;
+                    nsteps = 0;
+                    has_ray = true;
+                }
+                new_m = wave_mask(take);
+                c_next += (nneed < avail) ? nneed : avail;
             }
+            pend_m = 0;
             if (live_m == 0) break;
         }
         const bool live = __builtin_amdgcn_inverse_ballot_w64(live_m);
