@@ -81,7 +81,7 @@ class CthError(RuntimeError):
 def _load() -> C.CDLL:
     if not LIB_PATH.exists():
         raise ImportError(
-            f"{LIB_PATH} is missing: build it with `python -m cantucci_amd.build` "
+            f"{LIB_PATH} is missing: build it with `python cantucci_amd/build.py` "
             "(there is no CPU fallback)")
     # torch ships its own HIP runtime under the same SONAME (libamdhip64.so.7).
     # Loading torch first makes the dynamic linker bind this library to that

@@ -77,5 +77,7 @@ def build(verbose: bool = False, extra: list[str] | None = None) -> Path:
     return LIB
 
 
+# Run as a script (`python cantucci_amd/build.py`), not as `-m`: the package's
+# __init__ loads the library, so it cannot be imported before the first build.
 if __name__ == "__main__":
     build(verbose=True, extra=sys.argv[1:])
