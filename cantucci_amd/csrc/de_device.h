@@ -52,21 +52,23 @@ __device__ __forceinline__ bool on_z_axis(float x, float y) {
 __device__ __forceinline__ float sqrt_rn(float x) { return __builtin_sqrtf(x); }
 __device__ __forceinline__ float div_rn(float a, float b) { return a / b; }
 
-// a / b as the compiler's correctly rounded division sequence without its
-// operand scaling (v_div_scale) and special-case fix-up (v_div_fixup): the
-// reciprocal estimate, one Newton step on it, then two remainder corrections
-// of the quotient.  Where the scaling is the identity -- no operand,
-// reciprocal, quotient or remainder over- or underflows -- the steps are the
-// same operations on the same values, so the result is `a / b` bit for bit.
-// Callers guarantee that range.  8 instructions instead of 11.
+// a / b without operand scaling or special-case handling: the reciprocal
+// estimate, one Newton step on it, the quotient, and one remainder
+// correction of the quotient (Markstein's final step, "IA-64 and elementary
+// functions", 2000): 6 instructions instead of the 11 of the compiler's
+// correctly rounded sequence.  Where no operand, reciprocal, quotient or
+// remainder over- or underflows, every step scales exactly with powers of
+// two, so a check over all pairs of significands covers every such pair:
+// tests/hip/div_check.hip finds the result equal to `a / b` on all 2^46 of
+// them (and on 2^32 random pairs over the P8 body's operand range).  A second
+// remainder correction, as in the compiler's sequence, changes nothing there;
+// without the Newton step 26,621 pairs differ.  Callers guarantee the range.
 __device__ __forceinline__ float div_rn_unscaled(float a, float b) {
     float y = __builtin_amdgcn_rcpf(b);
     const float e = __builtin_fmaf(-b, y, 1.0f);
     y = __builtin_fmaf(e, y, y);
-    float q = a * y;
-    float r = __builtin_fmaf(-b, q, a);
-    q = __builtin_fmaf(r, y, q);
-    r = __builtin_fmaf(-b, q, a);
+    const float q = a * y;
+    const float r = __builtin_fmaf(-b, q, a);
     return __builtin_fmaf(r, y, q);
 }
 
