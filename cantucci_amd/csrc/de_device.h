@@ -72,25 +72,23 @@ __device__ __forceinline__ float div_rn_unscaled(float a, float b) {
     return __builtin_fmaf(r, y, q);
 }
 
-// The same correctly rounded square root without the small-argument scaling
-// whenever no active lane needs it.  For x >= 2^-96 (and +inf) the compiler's
-// expansion of __builtin_sqrtf is exactly this: the hardware estimate s
-// (within an ulp), then two FMA residual tests choose s - ulp, s or s + ulp;
-// below 2^-96 it first scales x by 2^32 (and the result by 2^-16), and it
-// passes 0 and inf through.  NaN, zero, negative and tiny arguments take the
-// full sequence.
+// The correctly rounded square root of x in [2^-96, FLT_MAX]: the hardware
+// reciprocal square root estimate y, g = x y, h = y / 2, and one Markstein
+// correction g + (x - g^2) h ("IA-64 and elementary functions", 2000).  Over
+// that range no step over- or underflows; tests/hip/sqrt_exhaustive.hip finds
+// the result equal to __builtin_sqrtf on every input (through sqrt_rn_wave).
+// Infinite input gives NaN, so callers keep it out.  4 instructions and the
+// estimate, where the compiler's sequence has 8 and the estimate.
 __device__ __forceinline__ float sqrt_rn_unscaled(float x) {
-    const float s = __builtin_amdgcn_sqrtf(x);
-    const float sd = __uint_as_float(__float_as_uint(s) - 1u);
-    const float su = __uint_as_float(__float_as_uint(s) + 1u);
-    const float rd = __builtin_fmaf(-sd, s, x);
-    const float ru = __builtin_fmaf(-su, s, x);
-    float r = (rd <= 0.0f) ? sd : s;
-    r = (ru > 0.0f) ? su : r;
-    return r;
+    const float y = __builtin_amdgcn_rsqf(x);
+    const float g = x * y, h = 0.5f * y;
+    const float r = __builtin_fmaf(-g, g, x);
+    return __builtin_fmaf(r, h, g);
 }
+// sqrt_rn_unscaled when every lane of the wave is in its range (false for
+// NaN), else the full sequence.
 __device__ __forceinline__ float sqrt_rn_wave(float x) {
-    if (__builtin_expect(wave_mask(!(x >= 0x1p-96f)) == 0, 1)) return sqrt_rn_unscaled(x);
+    if (__builtin_expect(wave_mask(!(x >= 0x1p-96f && x <= 0x1.fffffep127f)) == 0, 1)) return sqrt_rn_unscaled(x);
     return __builtin_sqrtf(x);
 }
 
@@ -228,8 +226,9 @@ struct P8Exact {
         // step of the unscaled division over- or underflows unless
         // |num / w8| < 2^-25, where 1 + num / w8 rounds to 1 either way; a is
         // then the same float as with the full sequence.  The same test
-        // covers the square root's w2 >= 2^-96, and it rules out the z axis
-        // (x = y = 0 gives w2 = 0).
+        // keeps a live lane's w2 in the square root's range [2^-96, 256]
+        // (lanes without a live point may be outside: their results are
+        // dropped), and it rules out the z axis (x = y = 0 gives w2 = 0).
         float q, sw2;
         const bool unscaled = u.bailout <= 16.0f && wave_mask(!(w2 >= 0x1p-15f)) == 0;
         if (unscaled) {
