@@ -26,7 +26,7 @@ SPACE = {
     "block": [256],
     "blocks_per_cu": [4, 8, 16],
     "chunk": [64, 256, 1024],
-    "refill_min": [16, 32, 48, 56, 62, 64],
+    "refill_min": [16, 32, 40, 44, 48, 56, 62, 64],
     "dynamic": [0, 1],
 }
 
