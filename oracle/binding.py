@@ -21,7 +21,7 @@ K_SPHERE = 2
 
 __all__ = [
     "OrcShape", "mandelbulb", "mandelbulb_trig", "sphere", "de_points", "de_points_f64", "de_grad_f64",
-    "de_grid", "sample_spans", "surface_nets", "mesh_batch", "focus_split", "create_spans", "refine", "ray_dirs", "sphere_trace", "fd_diff",
+    "de_grid", "sample_spans", "surface_nets", "mesh_batch", "focus_split", "create_spans", "refine", "ray_dirs", "focus_dirs", "get_focii", "sphere_trace", "fd_diff",
     "powi", "build", "K_MANDELBULB", "K_MANDELBULB_TRIG", "K_SPHERE", "default_threads",
 ]
 
@@ -68,6 +68,7 @@ for _name, _res, _args in [
     ("orc_focus_split", None, [_P, _S, _P, _P, _S, _P, _P]),
     ("orc_refine", _S, [_SH, _P, _U, _U, _P, _P, _S, _P, _I]),
     ("orc_ray_dirs", None, [_P, _U, _U, _U, _U, _P]),
+    ("orc_focus_dirs", None, [_P, _U, _P]),
     ("orc_sphere_trace", None, [_SH, _P, _P, _S, _I, _U, _F, _F, _F, _P, _P, _P, _I]),
     ("orc_fd_diff", None, [_SH, _P, _P, _S, _F, _P, _I]),
     ("orc_powi_f32", _F, [_F, _I]),
@@ -253,6 +254,27 @@ def ray_dirs(pose, W: int, H: int, row0: int = 0, rows: int | None = None) -> np
     out = np.empty((rows * W, 3), np.float32)
     _lib.orc_ray_dirs(_ptr(p), W, H, row0, rows, _ptr(out))
     return out
+
+
+def focus_dirs(cam, focus_points: int = 5) -> np.ndarray:
+    """The focus-ray directions of `ShapeMesh::get_focii` (src/mesh/mod.rs:205-231)
+    for cam = (position xyz, unit direction xyz, fov, aspect, near, far)
+    (src/camera.rs:16-20,114-127); ray x * fp + y of iter::square."""
+    c = _f32(cam).reshape(10)
+    out = np.empty((focus_points * focus_points, 3), np.float32)
+    _lib.orc_focus_dirs(_ptr(c), focus_points, _ptr(out))
+    return out
+
+
+def get_focii(shape: OrcShape, cam, focus_points: int = 5, max_steps: int = 100, eps: float = 1e-6):
+    """`ShapeMesh::get_focii` (src/mesh/mod.rs:205-243) without the filter:
+    (positions after the last step (fp^2, 3), depth (+inf on a miss), steps,
+    directions).  The reference returns positions[np.isfinite(depth)]."""
+    c = _f32(cam).reshape(10)
+    dirs = focus_dirs(c, focus_points)
+    depth, steps, pos = sphere_trace(shape, c[:3][None], dirs, max_steps, eps_rel=0.0, t_max=float("inf"),
+                                     eps_abs=eps, with_pos=True)
+    return pos, depth, steps, dirs
 
 
 def sphere_trace(shape: OrcShape, origins, dirs, max_steps=256, eps_rel=1e-4, t_max=10.0, eps_abs=0.0,

@@ -25,12 +25,22 @@
 //   * Sphere                 -- closed forms.
 //   * span sampling / sign   -- sphere sign field; dyadic grid coordinates exact.
 //   * surface nets           -- sphere: watertight, every edge shared by 2 quads.
+//   * vertex weights         -- linear corner fields: each vertex is the mean of
+//                               the zero crossings mirrored about their edge
+//                               midpoints; the +0 / -0 special cases.
+//   * quad winding           -- sphere: every directed edge once, every
+//                               triangle's right-hand normal outward.
+//   * 6-tap stencil          -- sphere and zero-iteration closed forms.
+//   * pinhole rays           -- image orientation and field of view.
+//   * get_focii rays         -- exact-arithmetic near-plane geometry within the
+//                               f32 cancellation bound; sphere hits analytic.
 //   * sphere tracing         -- sphere: analytic ray/sphere hit.
 //   * dual-number gradients  -- FP64 central differences; sphere closed form;
 //                               0-iteration closed form.
 // Parity unpinned (no reference test or closed form fixes them; stated):
 //   * LLVM's lowering order of `powi` (the ladder below) -- at most 1 ulp/op.
 //   * cgmath's `normalize`, `lerp` operation order (vertex normals/positions).
+//   * cgmath's general 4x4 `invert` (get_focii uses the closed-form inverse).
 
 #include <algorithm>
 #include <atomic>
@@ -928,6 +938,72 @@ void orc_ray_dirs(const float* pose, uint32_t W, uint32_t H, uint32_t row0, uint
             float inv = 1.0f / m;
             size_t o = 3 * (size_t(yy) * W + x);
             for (int a = 0; a < 3; ++a) dirs[o + a] = d[a] * inv;
+        }
+}
+
+// The focus rays of `ShapeMesh::get_focii` (src/mesh/mod.rs:205-231) for the
+// reference camera cam = {position[3], direction[3], fov, aspect, near, far}
+// (src/camera.rs:16-20,114-127; `direction` unit length, as Camera::new
+// stores it).  Ray (x, y) of iter::square (x outer, src/util/iter.rs:55-87)
+// has index x * fp + y.  Every step is the f32 operation of the reference:
+//   view = look_at_rh(pos, pos + dir, (0, 0, 1))      (src/camera.rs:69-75)
+//     f = normalize((pos + dir) - pos); s = normalize(cross(f, up));
+//     u = cross(s, f)  (cgmath's look_to_rh)
+//   height = (2 near) tan(fov 0.5); width = height aspect  (:186-189)
+//   top_left = (-width / 2, -height / 2, -near), bottom_right likewise (:90-103)
+//   size = (width / fp, height / fp); center_diff = (br - tl) / (2 fp)
+//   center = (top_left + (x size_h, y size_v, 0)) + center_diff
+//   p = inv_view * (center, 1);  dir = normalize(p - pos)
+// The inverse of the view matrix is taken in closed form: columns s, u, -f
+// and pos, since the view transform is a rotation and a translation;
+// cgmath's `invert` is the general cofactor inverse, whose entries can
+// differ in the last bits (parity unpinned: cgmath is not vendored).  The
+// product follows cgmath's column order ((c0 x + c1 y) + c2 z) + c3 w, and
+// its w is 1, so from_homogeneous leaves p unchanged.  Writes fp^2 unit
+// directions (3 floats each) to `dirs`.
+void orc_focus_dirs(const float* cam, uint32_t fp, float* dirs) {
+    const float* pos = cam;
+    const float* dir = cam + 3;
+    const float fov = cam[6], aspect = cam[7], near_plane = cam[8];
+    auto normalize = [](float v[3]) {
+        // cgmath InnerSpace::normalize: v * (1 / |v|), |v| = sqrt((x x + y y) + z z)
+        float m = std::sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+        float inv = 1.0f / m;
+        for (int a = 0; a < 3; ++a) v[a] = v[a] * inv;
+    };
+    auto cross = [](const float a[3], const float b[3], float o[3]) {
+        o[0] = a[1] * b[2] - a[2] * b[1];
+        o[1] = a[2] * b[0] - a[0] * b[2];
+        o[2] = a[0] * b[1] - a[1] * b[0];
+    };
+    const float up[3] = {0.0f, 0.0f, 1.0f};
+    float f[3], s[3], u[3];
+    for (int a = 0; a < 3; ++a) f[a] = (pos[a] + dir[a]) - pos[a];
+    normalize(f);
+    cross(f, up, s);
+    normalize(s);
+    cross(s, f, u);
+    const float height = (2.0f * near_plane) * std::tan(fov * 0.5f);
+    const float width = height * aspect;
+    const float tl[3] = {-width / 2.0f, -height / 2.0f, -near_plane};
+    const float br[3] = {width / 2.0f, height / 2.0f, -near_plane};
+    const float ffp = float(fp);
+    const float size_h = width / ffp, size_v = height / ffp;
+    float cd[3];
+    for (int a = 0; a < 3; ++a) cd[a] = (br[a] - tl[a]) / (2.0f * ffp);
+    for (uint32_t x = 0; x < fp; ++x)
+        for (uint32_t y = 0; y < fp; ++y) {
+            const float off[3] = {float(x) * size_h, float(y) * size_v, 0.0f};
+            float c[3];
+            for (int a = 0; a < 3; ++a) c[a] = (tl[a] + off[a]) + cd[a];
+            float d[3];
+            for (int a = 0; a < 3; ++a) {
+                float p = ((s[a] * c[0] + u[a] * c[1]) + (-f[a]) * c[2]) + pos[a] * 1.0f;
+                d[a] = p - pos[a];
+            }
+            normalize(d);
+            float* o = dirs + 3 * (size_t(x) * fp + y);
+            for (int a = 0; a < 3; ++a) o[a] = d[a];
         }
 }
 

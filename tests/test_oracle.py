@@ -411,3 +411,341 @@ def test_focus_split_rule_by_hand():
     # outside the octree (half-open at +1) and a missed ray
     assert not O.focus_split(leaves, [[1.0, 0.0, 0.0]], [1.0, 0.0, 0.1]).any()
     assert not O.focus_split(leaves, [[0.1, 0.1, 0.1]], [0.1, 0.1, 0.2], depth=[np.inf]).any()
+
+
+# ---------------------------------------------------------------------------
+# Independent pins of the surface-net vertex weights, the quad winding, the
+# 6-tap stencil and the camera rays.  Each check is also run on a copy of the
+# oracle's output with a plausible mistake planted in it (on the CPU), which
+# it must reject.
+# ---------------------------------------------------------------------------
+
+def _corner_grid(span, res):
+    """Corner coordinates of a span at `res` (adjusted span, GridTable order)
+    in float64; dyadic spans make them exact in f32."""
+    s = np.asarray(span[:3], np.float64)
+    e = np.asarray(span[3:], np.float64)
+    of = (e - s) / res
+    s2, e2 = s - of, e + of
+    t = np.arange(res + 1) / res
+    ax = [s2[a] + (e2[a] - s2[a]) * t for a in range(3)]
+    X, Y, Z = np.meshgrid(ax[0], ax[1], ax[2], indexing="ij")
+    return np.stack([X, Y, Z], -1), (e2 - s2) / res
+
+
+def _mirrored_crossing_centroids(P, D, h):
+    """Closed form of the reference's vertex for a corner field D at corner
+    positions P: on every crossing edge the point mirrored from the zero
+    crossing about the edge's midpoint, averaged over the cell's crossing
+    edges; keyed by cell (x, y, z) in cube() order."""
+    res = D.shape[0] - 1
+    sp = ~np.signbit(D)
+    out = {}
+    for x in range(res):
+        for y in range(res):
+            for z in range(res):
+                corners = [(x + (c >> 2 & 1), y + (c >> 1 & 1), z + (c & 1)) for c in range(8)]
+                signs = [sp[c] for c in corners]
+                if all(s == signs[0] for s in signs):
+                    continue
+                pts = []
+                for i in range(8):
+                    for j in range(i + 1, 8):
+                        if bin(i ^ j).count("1") != 1 or signs[i] == signs[j]:
+                            continue
+                        a, b = corners[i], corners[j]
+                        da, db = float(D[a]), float(D[b])
+                        t = da / (da - db)  # zero crossing from a to b
+                        mid = 0.5 * (P[a] + P[b])
+                        pts.append(2.0 * mid - (P[a] + t * (P[b] - P[a])))
+                out[(x, y, z)] = np.mean(pts, axis=0)
+    return out
+
+
+def _check_vertices_against_closed_form(verts, expect, h):
+    pos = verts[:, :3].astype(np.float64)
+    ref = np.array(list(expect.values()))
+    assert pos.shape == ref.shape
+    return np.abs(pos - ref).max() <= 1e-5 * h.max()
+
+
+@pytest.mark.parametrize("field", ["x", "-y", "z", "oblique"])
+def test_vertex_weights_are_the_mirrored_crossing(field):
+    """Vertex placement (src/mesh/buffer.rs:209-250) on linear corner fields.
+    On an edge whose ends have distances a < 0 < b the reference's weight
+    (d_from + delta) / delta = b / (b - a) puts the point at the zero crossing
+    mirrored about the edge midpoint; the vertex is the mean over the cell's
+    crossing edges.  Textbook weights (the zero crossing itself) fail."""
+    res = 4
+    span = np.array([0.0, 0.0, 0.0, 1.0, 1.0, 1.0], F32)
+    P, h = _corner_grid(span, res)
+    a0 = P[1, 1, 1, 0] + 0.3 * h[0]  # a plane 0.3 of a cell past a corner layer (cubic span)
+    if field == "x":
+        D = P[..., 0] - a0
+    elif field == "-y":
+        D = a0 - P[..., 1]
+    elif field == "z":
+        D = P[..., 2] - a0
+    else:
+        D = 0.6 * P[..., 0] - 0.48 * P[..., 1] + 0.64 * P[..., 2] - 0.41
+    D = D.astype(F32)
+    voff, ioff, v, ix = O.mesh_batch(O.sphere((0, 0, 0), 1.0), span[None], res, dists=D.reshape(1, -1))
+    expect = _mirrored_crossing_centroids(P, D, h)
+    assert len(v) == len(expect) > 0
+    assert _check_vertices_against_closed_form(v, expect, h)
+    if field in ("x", "-y", "z"):
+        # exactly one crossing per edge: the vertex sits at 1 - 0.3 of the
+        # cell along the field's axis, the other two coordinates at the centre
+        ax = {"x": 0, "-y": 1, "z": 2}[field]
+        frac = (v[:, ax].astype(np.float64) - P[0, 0, 0, ax]) / h[ax] % 1.0
+        assert np.allclose(frac, 0.7, atol=1e-4)
+        other = [b for b in range(3) if b != ax]
+        fo = (v[:, other].astype(np.float64) - P[0, 0, 0, other]) / h[other] % 1.0
+        assert np.allclose(fo, 0.5, atol=1e-5)
+    # planted mistake: textbook weights put each crossing point at the zero
+    # crossing, i.e. every edge point mirrored back about its edge midpoint
+    tb = v.copy()
+    for i, key in enumerate(expect):
+        # for one crossing per edge the mirrored mean is the mean mirrored
+        # about the mean of the crossing edges' midpoints
+        x, y, z = key
+        corners = [(x + (c >> 2 & 1), y + (c >> 1 & 1), z + (c & 1)) for c in range(8)]
+        mids = []
+        for p in range(8):
+            for q in range(p + 1, 8):
+                if bin(p ^ q).count("1") == 1 and np.signbit(D[corners[p]]) != np.signbit(D[corners[q]]):
+                    mids.append(0.5 * (P[corners[p]] + P[corners[q]]))
+        tb[i, :3] = (2.0 * np.mean(mids, axis=0) - v[i, :3].astype(np.float64)).astype(F32)
+    assert not _check_vertices_against_closed_form(tb, expect, h)
+
+
+def test_vertex_weight_special_cases():
+    """A plane through a corner layer (d = +0 there) gives weight 0 on the
+    edges that end on it: the vertex takes the inside end (the mirrored
+    crossing), and equal signed zeros at both ends (-0 / +0: delta == 0)
+    give the edge midpoint (src/mesh/buffer.rs:217-220)."""
+    res = 2
+    span = np.array([0.0, 0.0, 0.0, 1.0, 1.0, 1.0], F32)
+    P, h = _corner_grid(span, res)
+    D = (P[..., 0] - P[1, 0, 0, 0]).astype(F32)  # zero on the x-layer 1
+    _, _, v, _ = O.mesh_batch(O.sphere((0, 0, 0), 1.0), span[None], res, dists=D.reshape(1, -1))
+    # cells x = 0 cross (negative to +0); cells x = 1 do not (+0 to positive)
+    assert len(v) == res * res
+    assert np.allclose(v[:, 0], P[0, 0, 0, 0], atol=1e-7)
+    D2 = np.full((res + 1,) * 3, 1.0, F32)
+    D2[0] = -0.0
+    D2[1] = 0.0
+    _, _, v2, _ = O.mesh_batch(O.sphere((0, 0, 0), 1.0), span[None], res, dists=D2.reshape(1, -1))
+    assert len(v2) == res * res
+    assert np.allclose(v2[:, 0], P[0, 0, 0, 0] + 0.5 * h[0], atol=1e-7)
+
+
+def _orientation_ok(verts, idx, centre):
+    """Every directed edge used exactly once (a consistently oriented closed
+    surface), each triangle's right-hand normal pointing away from the
+    centre, and the signed volume positive."""
+    tri = idx.reshape(-1, 3).astype(np.int64)
+    directed = {}
+    for t in tri:
+        for a, b in ((t[0], t[1]), (t[1], t[2]), (t[2], t[0])):
+            directed[(a, b)] = directed.get((a, b), 0) + 1
+    if set(directed.values()) != {1} or any((b, a) not in directed for (a, b) in directed):
+        return False
+    p = verts[:, :3].astype(np.float64) - centre
+    v0, v1, v2 = p[tri[:, 0]], p[tri[:, 1]], p[tri[:, 2]]
+    n = np.cross(v1 - v0, v2 - v0)
+    if not ((n * (v0 + v1 + v2)).sum(1) > 0).all():
+        return False
+    return np.einsum("ij,ij->i", v0, np.cross(v1, v2)).sum() / 6.0 > 0
+
+
+def test_quad_winding_points_outward():
+    """Face emission (src/mesh/buffer.rs:288-372): on a sphere every directed
+    edge appears once and every triangle's right-hand normal points toward
+    increasing distance.  (From the tables: for the +x edge with a negative
+    base distance, (v0, v2, v1) spans (0,1,0) x (0,0,1) = +x.)  The signed
+    volume is the ball's.  A swapped winding on the +y quads, or on all
+    quads, fails."""
+    c = np.array([0.01, -0.02, 0.03])
+    span = np.array([-1, -1, -1, 1, 1, 1], F32)
+    _, v, ix = O.surface_nets(O.sphere(tuple(c), 0.55), span, 16)
+    assert _orientation_ok(v, ix, c)
+    tri = ix.reshape(-1, 3).astype(np.int64)
+    p = v[:, :3].astype(np.float64)
+    vol = np.einsum("ij,ij->i", p[tri[:, 0]] - c, np.cross(p[tri[:, 1]] - c, p[tri[:, 2]] - c)).sum() / 6.0
+    # positive and near the ball's (the mirrored weights move vertices up to a
+    # cell off the surface; 1/8 of a cell spacing 0.133 bounds the change)
+    ratio = vol / (4.0 / 3.0 * math.pi * 0.55 ** 3)
+    assert 0.8 < ratio < 1.2
+    # planted mistakes: +y quads (triangles whose normal is mostly along y)
+    # wound the other way, and every triangle wound the other way
+    n = np.cross(p[tri[:, 1]] - p[tri[:, 0]], p[tri[:, 2]] - p[tri[:, 0]])
+    ymost = np.abs(n[:, 1]) > np.maximum(np.abs(n[:, 0]), np.abs(n[:, 2]))
+    swapped = tri.copy()
+    swapped[ymost] = swapped[ymost][:, [0, 2, 1]]
+    assert not _orientation_ok(v, swapped.astype(np.uint32).ravel(), c)
+    assert not _orientation_ok(v, tri[:, [0, 2, 1]].astype(np.uint32).ravel(), c)
+
+
+def test_fd_stencil_closed_forms():
+    """The 6-tap stencil (src/mesh/buffer.rs:256-266): DE(p + h e_a) - DE(p - h e_a)
+    on a sphere equals |p + h e_a - c| - |p - h e_a - c|, and on points beyond
+    the bailout radius (no loop body) g(|p + h e_a|) - g(|p - h e_a|) with
+    g(r) = 0.5 r ln r.  A flipped sign or a missing axis step fails."""
+    rng = np.random.default_rng(31)
+    c = np.array([0.2, -0.1, 0.3])
+    p = rng.uniform(-1.5, 1.5, (2000, 3)).astype(F32)
+    h = 0.01
+
+    def analytic(fn):
+        out = np.zeros((len(p), 3))
+        for a in range(3):
+            e = np.zeros(3)
+            e[a] = h
+            out[:, a] = fn(p.astype(np.float64) + e) - fn(p.astype(np.float64) - e)
+        return out
+
+    diff = O.fd_diff(O.sphere(tuple(c), 0.7), p, h)
+    ref = analytic(lambda q: np.linalg.norm(q - c, axis=1))
+    assert np.abs(diff - ref).max() < 2e-6
+    assert np.abs(-diff - ref).max() > 1e-3  # flipped stencil
+    q = rng.uniform(-3, 3, (4000, 3)).astype(F32)
+    q = q[np.linalg.norm(q.astype(np.float64), axis=1) > 2.05][:1500]
+    p = q
+    diff = O.fd_diff(O.mandelbulb(15, 2.0), p, h)
+    g = lambda r: 0.5 * r * np.log(r)
+    ref = analytic(lambda x: g(np.linalg.norm(x, axis=1)))
+    assert np.abs(diff - ref).max() < 4e-6
+    assert np.abs(diff[:, [1, 0, 2]] - ref).max() > 1e-3  # axes mixed up
+
+
+def _pinhole_pose(eye, fwd, up_hint, vfov, aspect):
+    f = np.asarray(fwd, np.float64) / np.linalg.norm(fwd)
+    r = np.cross(f, up_hint)
+    r /= np.linalg.norm(r)
+    u = np.cross(r, f)
+    return np.concatenate([eye, r, u, f, [math.tan(vfov / 2), aspect]]).astype(F32), r, u, f
+
+
+def test_ray_dirs_image_orientation_and_field_of_view():
+    """Pinhole rays: row 0 is the top of the image (positive along `up`),
+    column 0 the left (negative along `right`), and the angles between the
+    outermost pixel centres are the field of view shrunk by one pixel:
+    2 atan((1 - 1/H) tan(vfov/2)) vertically and 2 atan((1 - 1/W) tan(vfov/2)
+    aspect) horizontally.  A flipped row order or a swapped aspect fails."""
+    W, H, vfov = 32, 18, math.radians(60.0)
+    pose, r, u, f = _pinhole_pose([0.3, -3.0, 0.2], [-0.1, 1.0, -0.05], [0, 0, 1], vfov, W / H)
+    d = O.ray_dirs(pose, W, H).reshape(H, W, 3).astype(np.float64)
+
+    def ok(d, W, H, aspect):
+        if not ((d[0, :, :] @ u) > 0).all() or not ((d[-1, :, :] @ u) < 0).all():
+            return False
+        if not ((d[:, 0, :] @ r) < 0).all() or not ((d[:, -1, :] @ r) > 0).all():
+            return False
+        ang = lambda a, b: math.acos(min(1.0, float(np.dot(a, b))))
+        t = math.tan(vfov / 2)
+        # the middle column pair (W even) straddles the axis symmetrically
+        mid = W // 2
+        vert = ang(0.5 * (d[0, mid - 1] + d[0, mid]) / np.linalg.norm(0.5 * (d[0, mid - 1] + d[0, mid])),
+                   0.5 * (d[-1, mid - 1] + d[-1, mid]) / np.linalg.norm(0.5 * (d[-1, mid - 1] + d[-1, mid])))
+        horiz = ang(0.5 * (d[H // 2 - 1, 0] + d[H // 2, 0]) / np.linalg.norm(d[H // 2 - 1, 0] + d[H // 2, 0]) * 2,
+                    0.5 * (d[H // 2 - 1, -1] + d[H // 2, -1]) / np.linalg.norm(d[H // 2 - 1, -1] + d[H // 2, -1]) * 2)
+        ev = 2 * math.atan((1 - 1 / H) * t)
+        eh = 2 * math.atan((1 - 1 / W) * t * aspect)
+        return abs(vert - ev) < 2e-3 and abs(horiz - eh) < 2e-3
+
+    assert ok(d, W, H, W / H)
+    assert not ok(d[::-1], W, H, W / H)  # rows in the other order
+    pose2 = pose.copy()
+    pose2[13] = H / W  # aspect inverted
+    d2 = O.ray_dirs(pose2, W, H).reshape(H, W, 3).astype(np.float64)
+    assert not ok(d2, W, H, W / H)
+
+
+# ---------------------------------------------------------------------------
+# get_focii: the reference camera's focus rays
+# ---------------------------------------------------------------------------
+
+def _ref_camera(pos, direction, fov=1.0, aspect=16 / 9, near=4e-5, far=10.0):
+    d = np.asarray(direction, np.float64)
+    d = (d / np.linalg.norm(d)).astype(F32)
+    return np.concatenate([np.asarray(pos, F32), d, np.array([fov, aspect, near, far], F32)])
+
+
+def _angle(a, b):
+    """Angle between row vectors, robust near zero: atan2(|a x b|, a . b)."""
+    return np.arctan2(np.linalg.norm(np.cross(a, b), axis=-1), (a * b).sum(-1))
+
+
+def _focus_dirs_f64(cam, fp):
+    """The focus-ray geometry in exact arithmetic: cell centres of an fp x fp
+    grid over the near plane of a symmetric frustum (vertical fov, aspect),
+    x along `right` (outer), y along `up` (inner), from the bottom-left cell
+    (the view space's -x, -y corner)."""
+    pos, d = cam[:3].astype(np.float64), cam[3:6].astype(np.float64)
+    fov, aspect, near = float(cam[6]), float(cam[7]), float(cam[8])
+    f = d / np.linalg.norm(d)
+    s = np.cross(f, [0.0, 0.0, 1.0])
+    s /= np.linalg.norm(s)
+    u = np.cross(s, f)
+    hgt = 2 * near * math.tan(fov / 2)
+    wid = hgt * aspect
+    out = []
+    for x in range(fp):
+        for y in range(fp):
+            cx = -wid / 2 + (x + 0.5) * wid / fp
+            cy = -hgt / 2 + (y + 0.5) * hgt / fp
+            v = s * cx + u * cy + f * near
+            out.append(v / np.linalg.norm(v))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("near", [0.1, 4e-5])
+def test_focus_dirs_geometry(near):
+    """The focus rays (src/mesh/mod.rs:205-231) go through the near-plane cell
+    centres: against the exact-arithmetic construction within the f32
+    cancellation of p - position, which at the app's near plane (4e-5,
+    src/app.rs:98) reaches ~ulp(|position|) / near.  The middle ray of an odd
+    grid is the view direction; ray 0 is the bottom-left cell.  Swapping the
+    x and y loops fails."""
+    rng = np.random.default_rng(41)
+    for _ in range(8):
+        pos = rng.uniform(-3, 3, 3)
+        look = -pos + rng.uniform(-0.1, 0.1, 3)
+        cam = _ref_camera(pos, look, fov=1.0, aspect=1.6, near=near)
+        for fp in (5, 4):
+            got = O.focus_dirs(cam, fp).astype(np.float64)
+            exp = _focus_dirs_f64(cam, fp)
+            err = _angle(got, exp)
+            bound = 8 * np.spacing(np.float32(np.abs(cam[:3]).max())) / near + 1e-6
+            assert err.max() < bound, (err.max(), bound)
+            if near == 0.1:
+                assert err.max() < 1e-5
+                swapped = exp.reshape(fp, fp, 3).transpose(1, 0, 2).reshape(-1, 3)
+                assert _angle(got, swapped).max() > 1e-3
+        mid = O.focus_dirs(cam, 5)[12].astype(np.float64)
+        assert _angle(mid, cam[3:6].astype(np.float64)) < bound
+
+
+def test_get_focii_on_a_sphere():
+    """get_focii's marcher (src/mesh/mod.rs:229-241) against a sphere: rays
+    whose line meets the sphere end on its surface (within the 1e-6 stop
+    distance plus rounding) at the analytic entry point; the others miss
+    after 100 steps or converge onto the silhouette."""
+    c = np.array([0.05, -0.02, 0.01])
+    sh = O.sphere(tuple(c), 0.8)
+    cam = _ref_camera([2.5, 0.4, 0.3], [-2.5, -0.4, -0.3], fov=0.5, aspect=1.5, near=0.1)
+    pos, depth, steps, dirs = O.get_focii(sh, cam, 5)
+    o = cam[:3].astype(np.float64)
+    dd = dirs.astype(np.float64)
+    oc = o - c
+    b = (dd * oc).sum(1)
+    disc = b * b - (oc @ oc - 0.8 ** 2)
+    hit = np.isfinite(depth)
+    assert 9 <= hit.sum() < 25 and (disc[hit] > -1e-6).all()
+    clear = disc > 1e-3
+    assert hit[clear].all()
+    t_star = -b[clear] - np.sqrt(disc[clear])
+    assert np.abs(depth[clear] - t_star).max() < 1e-5
+    assert np.abs(np.linalg.norm(pos[hit].astype(np.float64) - c, axis=1) - 0.8).max() < 2e-6
+    assert (steps <= 100).all() and (steps[~hit] == 100).all()
