@@ -33,14 +33,49 @@ PEAK_HBM = 8.0e12     # MI355X HBM3E (B/s)
 FLOP_PER_BODY = 75    # P8 loop body: magnitude 5, compare 1, dr 7, rotation 59, add 3
 FLOP_BAILED = 6       # final magnitude + compare of a point that bails
 FLOP_FINAL = 4        # ln, *r, *0.5, /dr
+FLOP_VERTEX = 19      # 6-tap normal per vertex: 6 stencil adds, 3 differences, normalize 10
+FLOP_STEP = 11        # sphere-tracing step: pos += dir d (6), t += d, eps + eps_rel t (2), 2 compares
+FLOP_RAY = 33         # pinhole ray: u 6, v 5, direction 12, normalize 10
 
 
-def algorithmic_flop(iters, max_iters: int) -> float:
+def de_flop(bodies: int, bails: int, evals: int) -> float:
+    """Algorithmic FLOP of P8 distance evaluations: 75 per loop body, 6 per
+    evaluation that bails (its last magnitude and compare), 4 per final formula."""
+    return FLOP_PER_BODY * bodies + FLOP_BAILED * bails + FLOP_FINAL * evals
+
+
+def count_work(ctx, fn):
+    """(loop bodies, bailed evaluations, evaluations) that the refill launches
+    issued by fn() ran, from the library's work counters."""
     import torch
-    it = iters.to(torch.int64)
-    total = float(it.sum().item())
-    bailed = float((iters < max_iters).sum().item())
-    return FLOP_PER_BODY * total + FLOP_BAILED * bailed + FLOP_FINAL * iters.numel()
+    cnt = torch.zeros(3, dtype=torch.int64, device="cuda")
+    torch.cuda.synchronize()
+    ctx.count_work(cnt)
+    try:
+        fn()
+        torch.cuda.synchronize()
+    finally:
+        ctx.count_work(None)
+    return tuple(int(v) for v in cnt.cpu())
+
+
+def event_ms(fn, reps, before=None):
+    """Average duration of fn() over reps launches, each bracketed by HIP
+    events on torch's current stream (the stream the kernels run on);
+    before() (untimed) runs ahead of each launch."""
+    import torch
+    total = 0.0
+    for _ in range(reps):
+        if before is not None:
+            before()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        torch.cuda.synchronize()
+        total += e0.elapsed_time(e1)
+    return total / reps
 
 
 def main() -> None:
@@ -108,44 +143,66 @@ def main() -> None:
     value = evals * args.steps / elapsed
     phases = _phases(dev, torch, shape, oc, hd, leaves_buf, dists, cross, res)
 
-    # Dominant kernel: the span-sampling refill kernel.  Its average launch
-    # time is measured with HIP events on the stream it runs on (torch's
-    # current stream), launching exactly what a step launches for it.
+    # The step's two large kernels, each timed alone with HIP events on the
+    # stream it runs on (torch's current stream), launching exactly what a
+    # step launches for it; their algorithmic FLOP come from the library's
+    # work counters on one such launch.
+    #   span sampling: k_refill<P8Exact, SrcSpans, SinkDist> (sample_spans)
+    #   vertex taps:   k_mesh_taps<P8Exact> (the vertex pass of the mesh)
     reps = max(3, args.steps)
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
     dev.sample_spans(shape, leaves_buf, res, with_cross=False, out=dists)
-    ev0.record()
-    for _ in range(reps):
-        dev.sample_spans(shape, leaves_buf, res, with_cross=False, out=dists)
-    ev1.record()
-    torch.cuda.synchronize()
-    kernel_s = ev0.elapsed_time(ev1) / 1e3 / reps
-    _, _, iters = dev.sample_spans(shape, leaves_buf, res, with_cross=False, with_iters=True)
-    torch.cuda.synchronize()
-    flop = algorithmic_flop(iters, oc["max_iters"])
-    mean_iters = float(iters.to(torch.float64).mean().item())
-    del iters
-    achieved = flop / kernel_s
-    traffic = None
-    # HBM bytes per launch of the span kernel from the committed rocprofv3
-    # FETCH_SIZE / WRITE_SIZE summary (copy-kernel calibrated).
-    tf = ROOT / "profiles" / "spans_octree_traffic.md"
-    if tf.exists():
-        m = re.search(r"^bytes_per_launch:\s*(\d+)", tf.read_text(), re.M)
-        traffic = int(m.group(1)) if m else None
+    span_ms = event_ms(lambda: dev.sample_spans(shape, leaves_buf, res, with_cross=False, out=dists), reps)
+    span_w = count_work(ctx, lambda: dev.sample_spans(shape, leaves_buf, res, with_cross=False, out=dists))
+    span_flop = de_flop(*span_w)
+    # records as the mesh's emit pass leaves them for the vertex pass
+    _, _, vdef, _ = dev.mesh(shape, leaves_buf, dists, res, "deferred", cross=cross)
+    vdef = vdef.clone()
+    vwork = torch.empty_like(vdef)
+    taps_ms = event_ms(lambda: dev.mesh_vertices(shape, vwork, "fd6"), reps, before=lambda: vwork.copy_(vdef))
+    vwork.copy_(vdef)
+    taps_w = count_work(ctx, lambda: dev.mesh_vertices(shape, vwork, "fd6"))
+    taps_flop = de_flop(*taps_w) + FLOP_VERTEX * nverts
+    del vdef, vwork
+    traffic = {}
+    # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    # summaries (copy-kernel calibrated).
+    for key, fname in (("spans", "spans_octree_traffic.md"), ("taps", "mesh_taps_octree_traffic.md")):
+        tf = ROOT / "profiles" / fname
+        traffic[key] = None
+        if tf.exists():
+            m = re.search(r"^bytes_per_launch:\s*(\d+)", tf.read_text(), re.M)
+            traffic[key] = int(m.group(1)) if m else None
+    suffix = "Exact" if args.mode == "parity" else "Fast"
+    kernels = {
+        "taps": {
+            "kernel": f"k_mesh_taps<P8{suffix}>", "kernel_ms": taps_ms,
+            "algorithmic_flop_per_launch": taps_flop, "loop_bodies": taps_w[0], "bailed_evals": taps_w[1],
+            "evals": taps_w[2], "vertices": nverts,
+            "achieved": taps_flop / (taps_ms / 1e3) / 1e12, "frac": taps_flop / (taps_ms / 1e3) / PEAK_F32,
+            "traffic": traffic["taps"], "algorithmic_bytes": nverts * (24 + 16),
+        },
+        "spans": {
+            "kernel": f"k_refill<P8{suffix}, SrcSpans, SinkDist>", "kernel_ms": span_ms,
+            "algorithmic_flop_per_launch": span_flop, "loop_bodies": span_w[0], "bailed_evals": span_w[1],
+            "evals": span_w[2], "mean_loop_bodies": span_w[0] / max(1, span_w[2]),
+            "achieved": span_flop / (span_ms / 1e3) / 1e12, "frac": span_flop / (span_ms / 1e3) / PEAK_F32,
+            "traffic": traffic["spans"], "algorithmic_bytes": corner_evals * 4,
+            "hbm_write_gbs": corner_evals * 4 / (span_ms / 1e3) / 1e9,
+        },
+    }
+    dom = max(kernels, key=lambda k: kernels[k]["kernel_ms"])
+    kd = kernels[dom]
     roofline = {
         "bound": "valu",
-        "achieved": achieved / 1e12,
+        "achieved": kd["achieved"],
         "peak": PEAK_F32 / 1e12,
         "unit": "TFLOP/s",
-        "frac": achieved / PEAK_F32,
-        "traffic": traffic,
-        "kernel": "k_refill<P8Exact, SrcSpans, SinkDist>" if args.mode == "parity" else "k_refill<P8Fast, SrcSpans, SinkDist>",
-        "kernel_ms": kernel_s * 1e3,
-        "algorithmic_flop_per_launch": flop,
-        "mean_loop_bodies": mean_iters,
-        "hbm_write_gbs": corner_evals * 4 / kernel_s / 1e9,
+        "frac": kd["frac"],
+        "traffic": kd["traffic"],
+        "kernel": kd["kernel"],
+        "kernel_ms": kd["kernel_ms"],
+        "algorithmic_flop_per_launch": kd["algorithmic_flop_per_launch"],
+        "kernels": kernels,
     }
 
     # Reference CPU path (the oracle's op-exact restatement of the whole
@@ -454,7 +511,11 @@ def _create_spans(span):
 
 
 def bench_trace(ctx, args):
-    """32 seeded orbit poses x 1920x1080 primary rays, 256 steps, hit d < 1e-4 t, escape t > 10."""
+    """32 seeded orbit poses x 1920x1080 primary rays, 256 steps, hit d < 1e-4 t,
+    escape t > 10: eager launches, and the same launch captured in a HIP graph
+    and replayed.  The trace kernel's algorithmic FLOP come from the work
+    counters (75 per loop body, 6 per bailout, 4 per final formula, 11 per
+    step, 33 per ray)."""
     import torch
 
     import cantucci_amd as ca
@@ -474,12 +535,34 @@ def bench_trace(ctx, args):
                          depth=depth, steps=steps)
 
     ms = _time_ms(run, 5)
+    kernel_ms = event_ms(run, 5)
+    bodies, bails, evals = count_work(ctx, run)
     rays = tc["poses"] * tc["W"] * tc["H"]
+    flop = de_flop(bodies, bails, evals) + FLOP_STEP * evals + FLOP_RAY * rays
+    # the same launch replayed from a HIP graph on a stream of its own
+    s = torch.cuda.Stream()
+    ref_depth = depth.clone()
+    g = dev.Graph(lambda: run(), s)
+    g.launch()
+    s.synchronize()
+    replay_equal = bool(torch.equal(depth.nan_to_num(-1.0), ref_depth.nan_to_num(-1.0)))
+    reps = 5
+    t = time.perf_counter()
+    for _ in range(reps):
+        g.launch()
+    s.synchronize()
+    graph_ms = 1e3 * (time.perf_counter() - t) / reps
+    g.close()
     return {"workload": "32 poses x 1920x1080, P8 15 it, 256 steps, eps 1e-4 t, t_max 10",
-            "rays_per_s": rays / (ms / 1e3), "ms": ms,
+            "rays_per_s": rays / (ms / 1e3), "ms": ms, "ms_graph_replay": graph_ms,
+            "rays_per_s_graph": rays / (graph_ms / 1e3), "graph_replay_equals_eager": replay_equal,
             "hit_fraction": float(torch.isfinite(depth).float().mean().item()),
             "mean_steps": float(steps.float().mean().item()),
-            "de_evals_per_s": float(steps.to(torch.float64).sum().item()) / (ms / 1e3)}
+            "de_evals_per_s": evals / (ms / 1e3),
+            "roofline": {"kernel": "k_trace<P8Exact, RaysCamera>", "kernel_ms": kernel_ms,
+                         "loop_bodies": bodies, "bailed_evals": bails, "evals": evals,
+                         "algorithmic_flop_per_launch": flop, "achieved": flop / (kernel_ms / 1e3) / 1e12,
+                         "unit": "TFLOP/s", "frac": flop / (kernel_ms / 1e3) / PEAK_F32}}
 
 
 def bench_normals(ctx, args):

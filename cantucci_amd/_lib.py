@@ -26,6 +26,7 @@ CTH_MODE_FAST = 1
 
 CTH_NORMALS_FD6 = 0
 CTH_NORMALS_TANGENT = 1
+CTH_NORMALS_DEFERRED = 2
 
 
 class CthShape(C.Structure):
@@ -51,6 +52,18 @@ class CthCamera(C.Structure):
         ("fwd", C.c_float * 3),
         ("tan_half_fov", C.c_float),
         ("aspect", C.c_float),
+    ]
+
+
+class CthRefCamera(C.Structure):
+    """The reference's `Camera` + `Projection` (src/camera.rs:16-20,114-127)."""
+    _fields_ = [
+        ("position", C.c_float * 3),
+        ("direction", C.c_float * 3),
+        ("fov", C.c_float),
+        ("aspect", C.c_float),
+        ("near_plane", C.c_float),
+        ("far_plane", C.c_float),
     ]
 
 
@@ -114,6 +127,7 @@ _SIGS = {
     "cth_ctx_num_cus": (_I, [_P]),
     "cth_ctx_set_launch": (_I, [_P, _I, C.POINTER(CthLaunch)]),
     "cth_ctx_get_launch": (_I, [_P, _I, C.POINTER(CthLaunch)]),
+    "cth_ctx_count_work": (_I, [_P, _P]),
     "cth_bounding_box": (_I, [_SH, C.POINTER(CthSpan)]),
     "cth_min_distance_batch": (_I, [_P, _SH, _P, _S, _P]),
     "cth_max_distance_batch": (_I, [_P, _SH, _P, _S, _P]),
@@ -142,8 +156,11 @@ _SIGS = {
     "cth_graph_launch": (_I, [_P, _P]),
     "cth_graph_destroy": (None, [_P]),
     "cth_render_depth_dev": (_I, [_P, _SH, _P, _U, _U, _U, _U, _F, _F, _F, _P, _P, _P]),
+    "cth_get_focii_dev": (_I, [_P, _SH, _P, _U, _U, _U, _F, _P, _P, _P, _P, _P]),
+    "cth_get_focii": (_I, [_P, _SH, C.POINTER(CthRefCamera), _U, _P, C.POINTER(_S)]),
     "cth_mesh_count_dev": (_I, [_P, _P, _S, _U, _P, _P, _P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), _P]),
     "cth_mesh_emit_dev": (_I, [_P, _SH, _P, _P, _S, _U, _I, _P, _P, C.c_uint64, _P, _P, _P]),
+    "cth_mesh_vertices_dev": (_I, [_P, _SH, _P, C.c_uint64, _I, _P]),
     "cth_mesh_span": (_I, [_P, _SH, C.POINTER(CthSpan), _U, _I, _P, _S, _P, _S, C.POINTER(_S), C.POINTER(_S)]),
     "cth_focus_split_dev": (_I, [_P, _P, _S, _P, _P, _S, _P, _P, _P]),
     "cth_split_leaves_dev": (_I, [_P, _P, _S, _P, _P, _S, C.POINTER(_S), _P]),

@@ -5,12 +5,16 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "internal.h"
 
 namespace cth {
 
 static thread_local std::string g_last_error;
+static thread_local CapturePool g_capture;
+
+CapturePool& capture_pool() { return g_capture; }
 
 int fail(int code, const std::string& msg) {
     g_last_error = msg;
@@ -167,6 +171,12 @@ int cth_ctx_get_launch(const cth_ctx* c, int kernel, cth_launch* l) {
     return CTH_OK;
 }
 
+int cth_ctx_count_work(cth_ctx* c, uint64_t* counters) {
+    if (!c) return fail(CTH_E_INVALID, "null context");
+    c->stats = reinterpret_cast<unsigned long long*>(counters);
+    return CTH_OK;
+}
+
 void* cth_ctx_stream(cth_ctx* c) { return c ? (void*)c->stream : nullptr; }
 
 int cth_ctx_num_cus(const cth_ctx* c) { return c ? c->num_cus : 0; }
@@ -317,6 +327,52 @@ int cth_sphere_trace(cth_ctx* c, const cth_shape* s, const float* origins, int p
     return rc ? rc : rs;
 }
 
+int cth_get_focii_dev(cth_ctx* c, const cth_shape* s, const cth_ref_camera* cams, uint32_t ncams,
+                      uint32_t focus_points, uint32_t max_steps, float eps, float* focii, float* depth,
+                      uint32_t* steps, float* dirs, void* stream) {
+    if (!c) return fail(CTH_E_INVALID, "null context");
+    if (int rc = check_shape(s)) return rc;
+    if (focus_points == 0 || focus_points > 255) return fail(CTH_E_INVALID, "focus_points must be in [1, 255]");
+    if (max_steps == 0) return fail(CTH_E_INVALID, "max_steps must be >= 1");
+    if (ncams && (!cams || !focii || !depth || !steps)) return fail(CTH_E_INVALID, "null buffer");
+    return launch_trace_focii(*c, *s, cams, ncams, focus_points, max_steps, eps, focii, depth, steps, dirs,
+                              pick_stream(c, stream));
+}
+
+int cth_get_focii(cth_ctx* c, const cth_shape* s, const cth_ref_camera* cam, uint32_t focus_points, float* focii,
+                  size_t* nhits) {
+    if (!c) return fail(CTH_E_INVALID, "null context");
+    if (int rc = check_shape(s)) return rc;
+    if (!cam || !focii || !nhits) return fail(CTH_E_INVALID, "null buffer");
+    if (focus_points == 0 || focus_points > 255) return fail(CTH_E_INVALID, "focus_points must be in [1, 255]");
+    *nhits = 0;
+    const size_t n = size_t(focus_points) * focus_points;
+    void* b[4];
+    int rc = staging(*c, WS_HOST_IN, {sizeof(cth_ref_camera), n * 12, n * 4, n * 4}, b);
+    if (rc) return rc;
+    rc = check_hip(hipMemcpyAsync(b[0], cam, sizeof(cth_ref_camera), hipMemcpyHostToDevice, c->stream), "H2D");
+    // get_focii's constants (src/mesh/mod.rs:206-207): EPSILON 1e-6, MAX_ITERS 100
+    if (!rc)
+        rc = cth_get_focii_dev(c, s, (const cth_ref_camera*)b[0], 1, focus_points, 100, 0.000001f, (float*)b[1],
+                               (float*)b[2], (uint32_t*)b[3], nullptr, c->stream);
+    std::vector<float> pos(3 * n), depth(n);
+    if (!rc) rc = check_hip(hipMemcpyAsync(pos.data(), b[1], n * 12, hipMemcpyDeviceToHost, c->stream), "D2H");
+    if (!rc) rc = check_hip(hipMemcpyAsync(depth.data(), b[2], n * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+    int rs = check_hip(hipStreamSynchronize(c->stream), "sync");
+    if (rc || rs) return rc ? rc : rs;
+    // filter_map: the rays that hit, in ray order (src/mesh/mod.rs:229-242)
+    size_t k = 0;
+    for (size_t i = 0; i < n; ++i) {
+        if (depth[i] == __builtin_inff()) continue;
+        focii[3 * k] = pos[3 * i];
+        focii[3 * k + 1] = pos[3 * i + 1];
+        focii[3 * k + 2] = pos[3 * i + 2];
+        ++k;
+    }
+    *nhits = k;
+    return CTH_OK;
+}
+
 int cth_render_depth_dev(cth_ctx* c, const cth_shape* s, const cth_camera* cams, uint32_t ncams, uint32_t width,
                          uint32_t height, uint32_t max_steps, float eps_abs, float eps_rel, float t_max, float* depth,
                          uint32_t* steps, void* stream) {
@@ -376,6 +432,7 @@ int cth_normals_fd_dev(cth_ctx* c, const cth_shape* s, const float* pts, const f
     if (!c) return fail(CTH_E_INVALID, "null context");
     if (int rc = check_shape(s)) return rc;
     if (n && !pts) return fail(CTH_E_INVALID, "null buffer");
+    if (n >= (size_t(1) << 31)) return fail(CTH_E_INVALID, "batch larger than 2^31");
     if (!(h > 0.0f)) return fail(CTH_E_INVALID, "h must be positive");
     return launch_fd_normals(*c, *s, pts, powers, (uint32_t)n, h, d, grad_p, normals, pick_stream(c, stream));
 }
@@ -420,17 +477,39 @@ int cth_wall_clock_khz(const cth_ctx* c) {
 struct cth_graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    uint32_t* pairs = nullptr;  // the graph's own work-counter pairs
 };
 
+// The graph's counter pairs are allocated and zeroed before the capture
+// starts (no allocation may happen inside it) and kept until the graph is
+// destroyed, so its replays never share a pair with other launches.
 int cth_graph_begin(void* stream) {
     if (!stream) return fail(CTH_E_INVALID, "graph capture needs a non-null stream");
-    return check_hip(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal), "begin capture");
+    if (g_capture.pairs) return fail(CTH_E_INVALID, "a capture is already in progress on this thread");
+    uint32_t* pairs = nullptr;
+    const size_t bytes = 2 * GRAPH_PAIRS * sizeof(uint32_t);
+    int rc = check_hip(hipMalloc((void**)&pairs, bytes), "hipMalloc graph counters");
+    if (!rc) rc = check_hip(hipMemsetAsync(pairs, 0, bytes, (hipStream_t)stream), "zero graph counters");
+    if (!rc) rc = check_hip(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+    if (!rc) rc = check_hip(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal), "begin capture");
+    if (rc) {
+        if (pairs) (void)hipFree(pairs);
+        return rc;
+    }
+    g_capture.stream = (hipStream_t)stream;
+    g_capture.pairs = pairs;
+    g_capture.pos = 0;
+    return CTH_OK;
 }
 
 int cth_graph_end(void* stream, cth_graph** out) {
     if (!stream || !out) return fail(CTH_E_INVALID, "null argument");
     *out = nullptr;
     cth_graph* g = new cth_graph();
+    if (g_capture.pairs && g_capture.stream == (hipStream_t)stream) {
+        g->pairs = g_capture.pairs;
+        g_capture = CapturePool{};
+    }
     int rc = check_hip(hipStreamEndCapture((hipStream_t)stream, &g->graph), "end capture");
     if (!rc) rc = check_hip(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0), "graph instantiate");
     if (rc) {
@@ -450,6 +529,7 @@ void cth_graph_destroy(cth_graph* g) {
     if (!g) return;
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     if (g->graph) (void)hipGraphDestroy(g->graph);
+    if (g->pairs) (void)hipFree(g->pairs);
     delete g;
 }
 
@@ -508,6 +588,14 @@ int cth_mesh_emit_dev(cth_ctx* c, const cth_shape* s, const float* spans, const 
                      reinterpret_cast<float*>(vertices), indices, pick_stream(c, stream));
 }
 
+int cth_mesh_vertices_dev(cth_ctx* c, const cth_shape* s, cth_vertex* vertices, uint64_t nverts, int normals,
+                          void* stream) {
+    if (!c) return fail(CTH_E_INVALID, "null context");
+    if (int rc = check_shape(s)) return rc;
+    if (nverts && !vertices) return fail(CTH_E_INVALID, "null buffer");
+    return mesh_vertices(*c, *s, normals, nverts, reinterpret_cast<float*>(vertices), pick_stream(c, stream));
+}
+
 int cth_mesh_span(cth_ctx* c, const cth_shape* s, const cth_span* span, uint32_t res, int normals,
                   cth_vertex* vertices, size_t vert_cap, uint32_t* indices, size_t idx_cap, size_t* nverts,
                   size_t* nidx) {
@@ -518,7 +606,8 @@ int cth_mesh_span(cth_ctx* c, const cth_shape* s, const cth_span* span, uint32_t
         if (!(span->start[a] < span->end[a])) return fail(CTH_E_INVALID, "span start must be below end on every axis");
     if (res == 0 || (res & (res - 1)) != 0 || res > 64)
         return fail(CTH_E_INVALID, "mesh resolution must be a power of two in [1, 64]");
-    if (normals != CTH_NORMALS_FD6 && normals != CTH_NORMALS_TANGENT) return fail(CTH_E_INVALID, "unknown normals mode");
+    if (normals != CTH_NORMALS_FD6 && normals != CTH_NORMALS_TANGENT && normals != CTH_NORMALS_DEFERRED)
+        return fail(CTH_E_INVALID, "unknown normals mode");
     if (normals == CTH_NORMALS_TANGENT && s->kind == CTH_SHAPE_SPHERE)
         return fail(CTH_E_UNSUPPORTED, "tangent normals need a Mandelbulb kind");
     const size_t per = size_t(res + 1) * (res + 1) * (res + 1);

@@ -148,12 +148,13 @@ __device__ __forceinline__ float finish_exact(float r, float dr, bool orbit_pos_
 }
 
 // `f32::powi` with the LLVM ExpandPowI ladder: multiply the running square
-// into the result at each set bit, lowest first.  n >= 0.
+// into the result at each set bit of |n|, lowest first; a negative n takes
+// the reciprocal of the ladder's result.
 __device__ __forceinline__ float powi_ladder(float x, int n) {
     if (n == 0) return 1.0f;
     float cur = x, res = 1.0f;
     bool have = false;
-    unsigned v = (unsigned)n;
+    unsigned v = n < 0 ? 0u - (unsigned)n : (unsigned)n;
     while (v) {
         if (v & 1u) {
             res = have ? res * cur : cur;
@@ -162,7 +163,7 @@ __device__ __forceinline__ float powi_ladder(float x, int n) {
         v >>= 1;
         if (v) cur = cur * cur;
     }
-    return res;
+    return n < 0 ? div_rn(1.0f, res) : res;
 }
 
 // ---------------------------------------------------------------------------
@@ -282,12 +283,14 @@ struct P8Exact {
 struct TrigExact {
     static constexpr int kWavesPerSimd = 1;  // no constraint (64 VGPRs would spill)
     float px, py, pz, x, y, z, dr, P;
-    int Pi;  // integer power for the powi ladder, or -1 for powf
+    int Pi;       // integer power of the powi ladder (when `ladder`)
+    bool ladder;  // the powi ladder (integral power) rather than powf
     bool pos_nan;
 
     // Integral powers use the powi ladder (the reference's const-generic u8
     // power); the real-power kind uses powf unless the point's power happens
-    // to be an integer below 64 (the oracle applies the same rule).
+    // to be an integer of magnitude below 64, zero and negative ones included
+    // (the oracle applies the same rule).
     __device__ __forceinline__ void start(float ax, float ay, float az, float power, const MandelUniform& u) {
         px = x = ax;
         py = y = ay;
@@ -295,10 +298,13 @@ struct TrigExact {
         dr = 1.0f;
         P = power;
         pos_nan = false;
-        if (u.integral)
+        if (u.integral) {
             Pi = u.ipower;
-        else
-            Pi = (floorf(power) == power && fabsf(power) < 64.0f) ? (int)power : -1;
+            ladder = true;
+        } else {
+            ladder = floorf(power) == power && fabsf(power) < 64.0f;
+            Pi = ladder ? (int)power : 0;
+        }
     }
     __device__ __forceinline__ float mag() const { return sqrt_rn_wave((x * x + y * y) + z * z); }
 
@@ -316,11 +322,11 @@ struct TrigExact {
 
     __device__ __forceinline__ void eval(float r, Terms& t) const {
         // dr = r.powi(P - 1) * (P as f32) * dr + 1.0   (:73)
-        t.rp = (Pi >= 0) ? powi_ladder(r, Pi - 1) : powf(r, P - 1.0f);
+        t.rp = ladder ? powi_ladder(r, Pi - 1) : powf(r, P - 1.0f);
         t.ndr = ((t.rp * P) * dr) + 1.0f;
         // rotate_inner_px_generic (:128-146); on the z-axis rotate_on_z_axis
         // (:114-126) keeps only (0, 0, R cos(P theta)), the same z product.
-        t.R = (Pi >= 0) ? powi_ladder(r, Pi) : powf(r, P);
+        t.R = ladder ? powi_ladder(r, Pi) : powf(r, P);
         t.c = div_rn(z, r);
         t.th = acosf(t.c);
         t.ph = atan2f(y, x);

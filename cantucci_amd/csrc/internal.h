@@ -14,8 +14,24 @@ namespace cth {
 // Work counters: a ring of {next, done} pairs, zeroed at context creation.
 // Each launch takes the next pair, and the launch's last wave zeroes it again
 // (work_loop.h), so launches on different streams never share a pair unless
-// more than COUNTER_RING launches are in flight at once.
+// more than COUNTER_RING launches are in flight at once.  Launches recorded
+// into a graph through cth_graph_begin take their pairs from the graph's own
+// pool instead (CapturePool), since a graph replays with the pairs it was
+// captured with for as long as it lives.
 constexpr uint32_t COUNTER_RING = 4096;
+
+// Counter pairs owned by the graph being captured on this thread: launches
+// recorded on `stream` take pairs from `pairs` (zeroed before the capture
+// began), cycling through them.  Launches of one graph on its one stream run
+// one after another, and each leaves its pair at zero, so a pair can serve
+// several of them.
+constexpr uint32_t GRAPH_PAIRS = 64;
+struct CapturePool {
+    hipStream_t stream = nullptr;
+    uint32_t* pairs = nullptr;  // 2 * GRAPH_PAIRS, device
+    uint32_t pos = 0;
+};
+CapturePool& capture_pool();  // this thread's (capi.hip)
 
 struct LaunchShape {
     int block = 256;           // threads per workgroup (64, 128 or 256)
@@ -51,6 +67,7 @@ struct Ctx {
     void* ws[WS_SLOTS] = {};
     size_t ws_bytes[WS_SLOTS] = {};
     uint64_t* pinned = nullptr;  // 8 host-pinned words for count read-backs
+    unsigned long long* stats = nullptr;  // work counters of the refill kernels (cth_ctx_count_work)
     // Slot `s` with at least `bytes` bytes (a larger buffer replaces the old
     // one after a device-wide wait, so no launch can still be using it).
     int scratch(int s, size_t bytes, void** out) {
@@ -73,8 +90,12 @@ struct Ctx {
     }
     LaunchShape shape_de;
     LaunchShape shape_spans{256, 8, 1024, 32, 1};
-    // Next counter pair of the ring.
-    uint32_t* take_counters() {
+    // The counter pair of a launch on stream `st`: the capturing graph's own
+    // pairs while cth_graph_begin captures `st` on this thread, else the next
+    // pair of the ring.
+    uint32_t* take_counters(hipStream_t st) {
+        CapturePool& cp = capture_pool();
+        if (cp.pairs && cp.stream == st) return cp.pairs + 2 * (cp.pos++ % GRAPH_PAIRS);
         uint32_t i = __atomic_fetch_add(&ring_pos, 1u, __ATOMIC_RELAXED) % COUNTER_RING;
         return counters + 2 * i;
     }
@@ -118,6 +139,9 @@ int refine_levels(Ctx& c, const cth_shape& s, const float* root, uint32_t min_de
 int launch_trace_camera(Ctx& c, const cth_shape& s, const cth_camera* cams, uint32_t ncams, uint32_t W, uint32_t H,
                         uint32_t max_steps, float eps_abs, float eps_rel, float t_max, float* depth,
                         uint32_t* steps, hipStream_t st);
+int launch_trace_focii(Ctx& c, const cth_shape& s, const cth_ref_camera* cams, uint32_t ncams, uint32_t fp,
+                       uint32_t max_steps, float eps, float* focii, float* depth, uint32_t* steps, float* dirs,
+                       hipStream_t st);
 int launch_trace_rays(Ctx& c, const cth_shape& s, const float* origins, int per_ray_origin, const float* dirs,
                       uint32_t n, uint32_t max_steps, float eps_abs, float eps_rel, float t_max, float* depth,
                       uint32_t* steps, float* hit_pos, hipStream_t st);
@@ -139,6 +163,7 @@ int mesh_count(Ctx& c, const float* dists, uint32_t nleaves, uint32_t res, uint6
 int mesh_emit(Ctx& c, const cth_shape& s, const float* spans, const float* dists, uint32_t nleaves, uint32_t res,
               int normals, const uint64_t* voff, const uint64_t* ioff, uint64_t nverts, float* verts,
               uint32_t* idx, hipStream_t st);
+int mesh_vertices(Ctx& c, const cth_shape& s, int normals, uint64_t nverts, float* verts, hipStream_t st);
 int launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t st);
 int launch_focus_split(Ctx& c, const float* leaves, uint32_t n, const float* focii, const float* depth, uint32_t nf,
                        const float eye[3], uint32_t* flags, hipStream_t st);

@@ -417,7 +417,7 @@ int launch_adjoint(Ctx& c, const cth_shape& s, const float* pts, const float* po
         SrcPoints src{pts + 3 * off, powers ? powers + off : nullptr, s.power};
         // kernel 1: parity forward with orbit recording
         uint32_t blocks = 0;
-        WorkQueue q = make_queue(c, m, c.shape_adj, blocks);
+        WorkQueue q = make_queue(c, m, c.shape_adj, blocks, st);
         if (family == FAM_P8)
             hipLaunchKernelGGL((k_adj_forward<P8Exact>), dim3(blocks), dim3(c.shape_adj.block), 0, st, src, u, orbit, rec,
                                d ? d + off : nullptr, q);
@@ -429,7 +429,7 @@ int launch_adjoint(Ctx& c, const cth_shape& s, const float* pts, const float* po
         // kernel 2: reverse sweep
         AdjOut out{grad_d ? grad_d + off : nullptr, grad_p ? grad_p + 3 * off : nullptr,
                    grad_power ? grad_power + off : nullptr, normals ? normals + 3 * off : nullptr};
-        WorkQueue q2 = make_queue(c, m, c.shape_adj, blocks);
+        WorkQueue q2 = make_queue(c, m, c.shape_adj, blocks, st);
         if (family == FAM_P8)
             hipLaunchKernelGGL((k_adj_reverse<false>), dim3(blocks), dim3(c.shape_adj.block), 0, st, src, u, orbit, rec,
                                out, q2);
@@ -456,7 +456,7 @@ int launch_tangent(Ctx& c, const cth_shape& s, const float* pts, const float* po
     SrcPoints src{pts, powers, s.power};
     SinkGrad sink{d, normals, grad_p, grad_power, grad_d, u.max_iters};
     uint32_t blocks = 0;
-    WorkQueue q = make_queue(c, n, c.shape_adj, blocks);
+    WorkQueue q = make_queue(c, n, c.shape_adj, blocks, st);
     if (family == FAM_P8)
         hipLaunchKernelGGL((k_refill<P8Tangent, SrcPoints, SinkGrad>), dim3(blocks), dim3(c.shape_adj.block), 0, st, src,
                            u, sink, q);

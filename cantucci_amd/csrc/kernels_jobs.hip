@@ -242,7 +242,7 @@ int jobs_run(Ctx& c, const JobSet& js, float* dists, float* normals, unsigned lo
     int rc = check_hip(hipGetLastError(), "jobs reset launch");
     if (rc) return rc;
     uint32_t blocks = 0;
-    WorkQueue q = make_queue(c, js.total, c.shape_jobs, blocks);
+    WorkQueue q = make_queue(c, js.total, c.shape_jobs, blocks, st);
     if (js.kind == CTH_SHAPE_MANDELBULB)
         hipLaunchKernelGGL((k_jobs<P8Tangent>), dim3(blocks), dim3(c.shape_jobs.block), 0, st, js.dev, js.njobs, u,
                            dists, normals, done, start, q);

@@ -780,7 +780,7 @@ __device__ __forceinline__ void put_fd_normal(float* v, const float t[7]) {
 // column rather than in 13 vector registers.  The kernel then fits in 72
 // VGPRs without spilling, and 7 waves share each SIMD (5 with the record in
 // registers); asking for 8 forces spills and is slower.
-template <class Ev>
+template <class Ev, bool kCount = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_mesh_taps(float* __restrict__ verts,
                                                                                           MandelUniform u, WorkQueue q) {
     // slots 0..2: vertex position, 3..5: half-steps, 6 + t: distance of tap t
@@ -798,6 +798,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
     float r = 0.0f;
     uint32_t c_next = 0, c_end = 0, round = 0;
     bool exhausted = false;
+    WorkTally<kCount> tally;
 
     for (;;) {
         if ((uint32_t)__popcll(wave & ~live_m) >= q.refill_min) {
@@ -864,7 +865,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
                         ++t;
                     }
                 }
-                live_m |= start_m & ~wave_mask(r > u.bailout);
+                const uint64_t zero_m = wave_mask(r > u.bailout) & start_m;
+                live_m |= start_m & ~zero_m;
+                tally.eval(start_m);
+                tally.bail(zero_m);
             }
             if (live_m == 0) break;
         }
@@ -872,13 +876,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
         const uint64_t cap_m = wave_mask(t0 == tick + 1u - u.max_iters) & live_m;  // the max_iters-th body
         ev.body(r, u, live);
         ++tick;
+        tally.body(live_m);
         const float rn = ev.mag();
         const uint64_t bail_m = wave_mask(rn > u.bailout) & live_m & ~cap_m;
+        tally.bail(bail_m);
         // at the cap r stays the magnitude before the last body (src/shape/mandelbulb.rs:65-78)
         if (__builtin_amdgcn_inverse_ballot_w64(live_m & ~cap_m)) r = rn;
         live_m &= ~(cap_m | bail_m);
         hold_m |= cap_m | bail_m;
     }
+    tally.flush(q.stats);
     if (q.dynamic) wave_retire(q);
 }
 
@@ -1011,7 +1018,8 @@ int mesh_emit(Ctx& c, const cth_shape& s, const float* spans, const float* dists
               uint32_t* idx, hipStream_t st) {
     MeshDims m;
     if (int rc = mesh_dims(res, m)) return rc;
-    if (normals != CTH_NORMALS_FD6 && normals != CTH_NORMALS_TANGENT) return fail(CTH_E_INVALID, "unknown normals mode");
+    if (normals != CTH_NORMALS_FD6 && normals != CTH_NORMALS_TANGENT && normals != CTH_NORMALS_DEFERRED)
+        return fail(CTH_E_INVALID, "unknown normals mode");
     if (normals == CTH_NORMALS_TANGENT && s.kind == CTH_SHAPE_SPHERE)
         return fail(CTH_E_UNSUPPORTED, "tangent normals need a Mandelbulb kind");
     if (nleaves == 0) return CTH_OK;
@@ -1033,8 +1041,19 @@ int mesh_emit(Ctx& c, const cth_shape& s, const float* spans, const float* dists
                            uoff + units + 1, voff, ioff, verts, idx);
     }
     int rc = check_hip(hipGetLastError(), "mesh emit launch");
-    if (rc || nverts == 0) return rc;
+    if (rc || nverts == 0 || normals == CTH_NORMALS_DEFERRED) return rc;
+    return mesh_vertices(c, s, normals, nverts, verts, st);
+}
 
+// The vertex pass: distance and normal of records that hold their position
+// and, in the normal slots, their leaf's 6-tap half-steps.
+int mesh_vertices(Ctx& c, const cth_shape& s, int normals, uint64_t nverts, float* verts, hipStream_t st) {
+    if (normals != CTH_NORMALS_FD6 && normals != CTH_NORMALS_TANGENT)
+        return fail(CTH_E_INVALID, "the vertex pass computes FD6 or TANGENT normals");
+    if (normals == CTH_NORMALS_TANGENT && s.kind == CTH_SHAPE_SPHERE)
+        return fail(CTH_E_UNSUPPORTED, "tangent normals need a Mandelbulb kind");
+    if (nverts == 0) return CTH_OK;
+    int rc = CTH_OK;
     MandelUniform u{};
     int family = FAM_SPHERE;
     if (s.kind != CTH_SHAPE_SPHERE) {
@@ -1044,7 +1063,7 @@ int mesh_emit(Ctx& c, const cth_shape& s, const float* spans, const float* dists
     if (normals == CTH_NORMALS_TANGENT) {
         if (nverts >= (1ull << 31)) return fail(CTH_E_INVALID, "more than 2^31 vertices");
         uint32_t blocks = 0;
-        WorkQueue q = make_queue(c, (uint32_t)nverts, c.shape_mesh, blocks);
+        WorkQueue q = make_queue(c, (uint32_t)nverts, c.shape_mesh, blocks, st);
         SrcMeshVerts src{verts, s.power};
         SinkMeshVerts sink{verts, u.max_iters};
         if (family == FAM_P8)
@@ -1061,9 +1080,16 @@ int mesh_emit(Ctx& c, const cth_shape& s, const float* spans, const float* dists
     if (family != FAM_SPHERE) {
         if (nverts >= (1ull << 31)) return fail(CTH_E_INVALID, "more than 2^31 vertices");
         uint32_t blocks = 0;
-        WorkQueue q = make_queue(c, (uint32_t)nverts, c.shape_mesh, blocks);
+        WorkQueue q = make_queue(c, (uint32_t)nverts, c.shape_mesh, blocks, st);
         dim3 grid(blocks), block(c.shape_mesh.block);
-        if (family == FAM_P8) {
+        if (q.stats) {
+            if (family == FAM_P8 && c.mode == CTH_MODE_FAST)
+                hipLaunchKernelGGL((k_mesh_taps<P8Fast, true>), grid, block, 0, st, verts, u, q);
+            else if (family == FAM_P8)
+                hipLaunchKernelGGL((k_mesh_taps<P8Exact, true>), grid, block, 0, st, verts, u, q);
+            else
+                hipLaunchKernelGGL((k_mesh_taps<TrigExact, true>), grid, block, 0, st, verts, u, q);
+        } else if (family == FAM_P8) {
             if (c.mode == CTH_MODE_FAST)
                 hipLaunchKernelGGL((k_mesh_taps<P8Fast>), grid, block, 0, st, verts, u, q);
             else

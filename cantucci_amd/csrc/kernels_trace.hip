@@ -57,6 +57,81 @@ struct RaysCamera {
     }
 };
 
+// The focus rays of ShapeMesh::get_focii (src/mesh/mod.rs:205-231), built
+// per lane from the reference camera (src/camera.rs:69-103,186-189) with the
+// reference's f32 operations (the library never contracts):
+//   f = normalize((pos + dir) - pos), s = normalize(cross(f, up)),
+//   u = cross(s, f)                          look_at_rh, up = (0, 0, 1)
+//   height = (2 near) tan(fov 0.5), width = height aspect
+//   top_left = (-width / 2, -height / 2, -near), bottom_right = -top_left.xy
+//   center = (top_left + (x width / fp, y height / fp, 0)) + (br - tl) / (2 fp)
+//   p = ((s cx + u cy) + (-f) cz) + pos      the inverse view matrix's columns
+//   dir = normalize(p - pos)
+// cgmath's normalize is v * (1 / |v|) with |v| = sqrt((x x + y y) + z z).  The
+// tangent is f64 tan rounded to f32: the correctly rounded value, as glibc's
+// tanf gives for the fields of view tested.
+struct RaysFocii {
+    const cth_ref_camera* cams;
+    uint32_t fp;
+    FastDiv per_cam, side;  // fp * fp, fp
+    __device__ __forceinline__ static void normalize3(float* v) {
+        const float m = sqrt_rn((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+        const float inv = div_rn(1.0f, m);
+        v[0] = v[0] * inv;
+        v[1] = v[1] * inv;
+        v[2] = v[2] * inv;
+    }
+    __device__ __forceinline__ static void cross3(const float* a, const float* b, float* o) {
+        o[0] = a[1] * b[2] - a[2] * b[1];
+        o[1] = a[2] * b[0] - a[0] * b[2];
+        o[2] = a[0] * b[1] - a[1] * b[0];
+    }
+    __device__ __forceinline__ void get(uint32_t idx, float* o, float* d) const {
+        const uint32_t cam = fdiv(idx, per_cam);
+        const uint32_t cell = idx - cam * per_cam.d;
+        const uint32_t x = fdiv(cell, side);
+        const uint32_t y = cell - x * side.d;
+        const cth_ref_camera& c = cams[cam];
+        const float up[3] = {0.0f, 0.0f, 1.0f};
+        float f[3], s[3], u[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) f[a] = (c.position[a] + c.direction[a]) - c.position[a];
+        normalize3(f);
+        cross3(f, up, s);
+        normalize3(s);
+        cross3(s, f, u);
+        const float height = (2.0f * c.near_plane) * (float)tan((double)(c.fov * 0.5f));
+        const float width = height * c.aspect;
+        const float tl[3] = {div_rn(-width, 2.0f), div_rn(-height, 2.0f), -c.near_plane};
+        const float br[3] = {div_rn(width, 2.0f), div_rn(height, 2.0f), -c.near_plane};
+        const float ffp = (float)fp;
+        const float off[3] = {(float)x * div_rn(width, ffp), (float)y * div_rn(height, ffp), 0.0f};
+        float cc[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) cc[a] = (tl[a] + off[a]) + div_rn(br[a] - tl[a], 2.0f * ffp);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float p = ((s[a] * cc[0] + u[a] * cc[1]) + (-f[a]) * cc[2]) + c.position[a] * 1.0f;
+            d[a] = p - c.position[a];
+            o[a] = c.position[a];
+        }
+        normalize3(d);
+    }
+};
+
+// Directions of a ray source, one ray per thread (the focus rays' own
+// output, for checking the construction).
+template <class Rays>
+__global__ __launch_bounds__(256) void k_ray_dirs(Rays rays, uint32_t n, float* dirs) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float o[3], d[3];
+    rays.get(i, o, d);
+    dirs[3 * (size_t)i] = d[0];
+    dirs[3 * (size_t)i + 1] = d[1];
+    dirs[3 * (size_t)i + 2] = d[2];
+}
+
 // Explicit rays: origins (one shared or one per ray) and unit directions.
 struct RaysExplicit {
     const float* origins;
@@ -78,7 +153,7 @@ struct TraceOut {
     float* hit_pos;  // nullable: position after the last step
 };
 
-template <class Ev, class Rays>
+template <class Ev, class Rays, bool kCount = false>
 __global__ __launch_bounds__(256) void k_trace(
     Rays rays, MandelUniform u, TraceParams tp, TraceOut out, WorkQueue q) {
     Ev ev;
@@ -94,6 +169,7 @@ __global__ __launch_bounds__(256) void k_trace(
     float pos[3] = {0.0f, 0.0f, 0.0f}, dir[3] = {0.0f, 0.0f, 0.0f};
     uint32_t c_next = 0, c_end = 0, round = 0;
     bool exhausted = false;
+    WorkTally<kCount> tally;
 
     // One sphere-tracing step of the lane's ray by distance d.  When the ray
     // ends (hit, escape or the step limit) its outputs are written and the
@@ -136,6 +212,8 @@ __global__ __launch_bounds__(256) void k_trace(
                 }
                 step_m = wave_mask(r > u.bailout) & start_m;  // zero loop bodies: finished already
                 live_m |= start_m & ~step_m;
+                tally.eval(start_m);
+                tally.bail(step_m);
                 new_m = 0;
                 if (step_m != 0) continue;
                 // New rays for lanes without one.
@@ -171,12 +249,15 @@ __global__ __launch_bounds__(256) void k_trace(
         const uint64_t cap_m = wave_mask(t0 == tick + 1u - u.max_iters) & live_m;  // the max_iters-th body
         ev.body(r, u, live);
         ++tick;
+        tally.body(live_m);
         const float rn = ev.mag();
         const uint64_t bail_m = wave_mask(rn > u.bailout) & live_m & ~cap_m;
+        tally.bail(bail_m);
         if (__builtin_amdgcn_inverse_ballot_w64(live_m & ~cap_m)) r = rn;
         live_m &= ~(cap_m | bail_m);
         pend_m |= cap_m | bail_m;
     }
+    tally.flush(q.stats);
     if (q.dynamic) wave_retire(q);
 }
 
@@ -196,10 +277,17 @@ static int run_trace(Ctx& c, const cth_shape& s, const Rays& rays, uint32_t tota
         if (rc) return rc;
     }
     uint32_t blocks = 0;
-    WorkQueue q = make_queue(c, total, c.shape_trace, blocks);
+    WorkQueue q = make_queue(c, total, c.shape_trace, blocks, st);
     dim3 grid(blocks), block(c.shape_trace.block);
     if (family == FAM_SPHERE) {
         hipLaunchKernelGGL((k_trace<SphereEv, Rays>), grid, block, 0, st, rays, u, tp, out, q);
+    } else if (q.stats) {
+        if (family == FAM_P8 && c.mode == CTH_MODE_FAST)
+            hipLaunchKernelGGL((k_trace<P8Fast, Rays, true>), grid, block, 0, st, rays, u, tp, out, q);
+        else if (family == FAM_P8)
+            hipLaunchKernelGGL((k_trace<P8Exact, Rays, true>), grid, block, 0, st, rays, u, tp, out, q);
+        else
+            hipLaunchKernelGGL((k_trace<TrigExact, Rays, true>), grid, block, 0, st, rays, u, tp, out, q);
     } else if (family == FAM_P8) {
         if (c.mode == CTH_MODE_FAST)
             hipLaunchKernelGGL((k_trace<P8Fast, Rays>), grid, block, 0, st, rays, u, tp, out, q);
@@ -219,6 +307,22 @@ int launch_trace_camera(Ctx& c, const cth_shape& s, const cth_camera* cams, uint
     RaysCamera rays{cams, W, H, make_fastdiv(W * H), make_fastdiv(W)};
     TraceParams tp{max_steps, eps_abs, eps_rel, t_max};
     return run_trace(c, s, rays, (uint32_t)total, tp, TraceOut{depth, steps, nullptr}, st);
+}
+
+int launch_trace_focii(Ctx& c, const cth_shape& s, const cth_ref_camera* cams, uint32_t ncams, uint32_t fp,
+                       uint32_t max_steps, float eps, float* focii, float* depth, uint32_t* steps, float* dirs,
+                       hipStream_t st) {
+    const uint64_t total = (uint64_t)ncams * fp * fp;
+    if (total >= (1ull << 31)) return fail(CTH_E_INVALID, "more than 2^31 focus rays in one launch");
+    if (total == 0) return CTH_OK;
+    RaysFocii rays{cams, fp, make_fastdiv(fp * fp), make_fastdiv(fp)};
+    if (dirs) {
+        const uint32_t n = (uint32_t)total;
+        hipLaunchKernelGGL((k_ray_dirs<RaysFocii>), dim3((n + 255) / 256), dim3(256), 0, st, rays, n, dirs);
+        if (int rc = check_hip(hipGetLastError(), "focus direction launch")) return rc;
+    }
+    TraceParams tp{max_steps, eps, 0.0f, __builtin_inff()};
+    return run_trace(c, s, rays, (uint32_t)total, tp, TraceOut{depth, steps, focii}, st);
 }
 
 int launch_trace_rays(Ctx& c, const cth_shape& s, const float* origins, int per_ray_origin, const float* dirs,

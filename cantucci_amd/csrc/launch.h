@@ -13,10 +13,11 @@ namespace cth {
 
 // Ev::kWavesPerSimd: the occupancy the evaluator's register budget is fitted
 // to (the P8 evaluators fit 64 VGPRs, 8 waves per SIMD, without spilling).
-template <class Ev, class Src, class Sink>
+// kCount: the counting instantiation (work tallies, see WorkTally).
+template <class Ev, class Src, class Sink, bool kCount = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Ev::kWavesPerSimd))) void k_refill(
     Src src, MandelUniform u, Sink sink, WorkQueue q) {
-    refill_loop<Ev>(src, u, sink, q);
+    refill_loop<Ev, kCount>(src, u, sink, q);
 }
 
 // Sphere: no iteration, one point per thread.
@@ -32,13 +33,13 @@ __global__ __launch_bounds__(256) void k_sphere(Src src, uint32_t n, float cx, f
     if (iters) iters[i] = 0;
 }
 
-inline WorkQueue make_queue(Ctx& c, uint32_t total, const LaunchShape& ls, uint32_t& blocks) {
+inline WorkQueue make_queue(Ctx& c, uint32_t total, const LaunchShape& ls, uint32_t& blocks, hipStream_t st) {
     uint32_t waves_per_block = (uint32_t)ls.block / 64u;
     uint64_t want = (uint64_t)c.num_cus * (uint64_t)ls.blocks_per_cu;
     uint64_t need = ((uint64_t)total + (uint64_t)ls.chunk * waves_per_block - 1) / ((uint64_t)ls.chunk * waves_per_block);
     blocks = (uint32_t)std::max<uint64_t>(1, std::min(want, need));
     WorkQueue q;
-    uint32_t* pair = c.take_counters();
+    uint32_t* pair = c.take_counters(st);
     q.next = pair;
     q.done = pair + 1;
     q.waves = blocks * waves_per_block;
@@ -50,6 +51,7 @@ inline WorkQueue make_queue(Ctx& c, uint32_t total, const LaunchShape& ls, uint3
     q.chunk = (uint32_t)(ch & ~63ull);
     q.refill_min = ls.refill_min < 1 ? 1 : (ls.refill_min > 64 ? 64 : ls.refill_min);
     q.dynamic = ls.dynamic ? 1u : 0u;
+    q.stats = c.stats;
     return q;
 }
 
@@ -57,10 +59,19 @@ template <class Src, class Sink>
 inline int run_refill(Ctx& c, int family, const MandelUniform& u, const Src& src, const Sink& sink,
                       uint32_t total, hipStream_t st, const LaunchShape& ls) {
     uint32_t blocks = 0;
-    WorkQueue q = make_queue(c, total, ls, blocks);
+    WorkQueue q = make_queue(c, total, ls, blocks, st);
     dim3 grid(blocks), block(ls.block);
     bool fast = c.mode == CTH_MODE_FAST;
-    if (family == FAM_P8) {
+    if (q.stats) {
+        if (family == FAM_P8) {
+            if (fast)
+                hipLaunchKernelGGL((k_refill<P8Fast, Src, Sink, true>), grid, block, 0, st, src, u, sink, q);
+            else
+                hipLaunchKernelGGL((k_refill<P8Exact, Src, Sink, true>), grid, block, 0, st, src, u, sink, q);
+        } else {
+            hipLaunchKernelGGL((k_refill<TrigExact, Src, Sink, true>), grid, block, 0, st, src, u, sink, q);
+        }
+    } else if (family == FAM_P8) {
         if (fast)
             hipLaunchKernelGGL((k_refill<P8Fast, Src, Sink>), grid, block, 0, st, src, u, sink, q);
         else

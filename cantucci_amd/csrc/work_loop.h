@@ -26,6 +26,41 @@ struct WorkQueue {
     uint32_t chunk;    // indices claimed per atomic
     uint32_t refill_min;  // idle lanes that trigger a write-back + refill (1..64)
     uint32_t dynamic;     // 1: claim chunks with an atomic; 0: static interleave
+    // Work counters (nullable): the launch adds its loop bodies, its
+    // evaluations that ended by bailout, and its evaluations to
+    // stats[0], stats[1], stats[2] (the counting instantiations only).
+    unsigned long long* stats;
+};
+
+// Per-wave work tally of a refill kernel (wave-uniform, scalar registers):
+// lane masks of the lanes that ran a loop body, that ended an evaluation by
+// bailout, that started an evaluation.  Only the counting instantiation of a
+// kernel (kOn, launched while a context counts work) tallies: even scalar
+// adds in the tick cost measurable time (11% on the span kernel), and the
+// counts depend only on the inputs, not on which instantiation ran.
+template <bool kOn>
+struct WorkTally {
+    uint64_t bodies = 0, bails = 0, evals = 0;
+    __device__ __forceinline__ void body(uint64_t m) {
+        if constexpr (kOn) bodies += (uint64_t)__popcll(m);
+    }
+    __device__ __forceinline__ void bail(uint64_t m) {
+        if constexpr (kOn) bails += (uint64_t)__popcll(m);
+    }
+    __device__ __forceinline__ void eval(uint64_t m) {
+        if constexpr (kOn) evals += (uint64_t)__popcll(m);
+    }
+    __device__ __forceinline__ void flush(unsigned long long* stats) const {
+        if constexpr (kOn) {
+            if (!stats) return;
+            const uint64_t active = wave_mask(true);
+            if ((int)__lane_id() == __ffsll((unsigned long long)active) - 1) {
+                atomicAdd(stats, (unsigned long long)bodies);
+                atomicAdd(stats + 1, (unsigned long long)bails);
+                atomicAdd(stats + 2, (unsigned long long)evals);
+            }
+        }
+    }
 };
 
 __device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
@@ -89,7 +124,7 @@ __device__ __forceinline__ void wave_retire(const WorkQueue& q) {
 // (__builtin_amdgcn_inverse_ballot_w64).  A point's body count is the wave's
 // tick counter (scalar) minus the tick at which the lane took the point, so
 // the iteration cap costs one compare per tick and no per-lane counter.
-template <class Ev, class Src, class Sink>
+template <class Ev, bool kCount, class Src, class Sink>
 __device__ __forceinline__ void refill_loop(const Src& src, const MandelUniform& u, const Sink& sink,
                                             const WorkQueue& q) {
     Ev ev;
@@ -104,6 +139,7 @@ __device__ __forceinline__ void refill_loop(const Src& src, const MandelUniform&
     uint32_t c_next = 0, c_end = 0;  // wave-uniform window of claimed indices
     uint32_t round = 0;
     bool exhausted = false;
+    WorkTally<kCount> tally;
 
     for (;;) {
         if ((uint32_t)__popcll(wave & ~live_m) >= q.refill_min) {
@@ -138,6 +174,8 @@ __device__ __forceinline__ void refill_loop(const Src& src, const MandelUniform&
                 }
                 const uint64_t took = wave_mask(take);
                 const uint64_t bailed = wave_mask(r > u.bailout) & took;
+                tally.eval(took);
+                tally.bail(bailed);
                 if (__builtin_amdgcn_inverse_ballot_w64(bailed)) sink.put(idx, ev.finish(r), 0u, r, ev.dr, ev);
                 live_m |= took & ~bailed;
                 c_next += (nneed < avail) ? nneed : avail;
@@ -149,8 +187,10 @@ __device__ __forceinline__ void refill_loop(const Src& src, const MandelUniform&
         const uint64_t cap_m = wave_mask(t0 == tick + 1u - u.max_iters) & live_m;
         ev.body(r, u, live);
         ++tick;
+        tally.body(live_m);
         const float rn = ev.mag();
         const uint64_t bail_m = wave_mask(rn > u.bailout) & live_m & ~cap_m;
+        tally.bail(bail_m);
         // At the iteration cap, r stays the magnitude computed before the last
         // body (the reference's off-by-one, src/shape/mandelbulb.rs:65-78).
         if (__builtin_amdgcn_inverse_ballot_w64(live_m & ~cap_m)) r = rn;
@@ -161,6 +201,7 @@ __device__ __forceinline__ void refill_loop(const Src& src, const MandelUniform&
         live_m &= ~done_m;
         hold_m |= done_m;
     }
+    tally.flush(q.stats);
     if (q.dynamic) wave_retire(q);
 }
 

@@ -30,12 +30,33 @@ def _f32(t: torch.Tensor) -> torch.Tensor:
     return t
 
 
+def _points(pts: torch.Tensor) -> torch.Tensor:
+    """(n, 3) float32 points, made contiguous.  The kernels read 3 floats per
+    point, so any other shape would send them past the buffer."""
+    if pts.dim() != 2 or pts.shape[1] != 3:
+        raise ValueError(f"points must have shape (n, 3), got {tuple(pts.shape)}")
+    return _f32(pts).contiguous()
+
+
+def _per_point(t: torch.Tensor | None, n: int, name: str) -> torch.Tensor | None:
+    """An optional float32 tensor of one value per point (per-point powers,
+    incoming gradients)."""
+    if t is None:
+        return None
+    if t.dtype != torch.float32:
+        raise TypeError(f"{name} must be float32, got {t.dtype}")
+    if t.numel() != n:
+        raise ValueError(f"{name} must hold {n} values (one per point), got {t.numel()}")
+    return t.contiguous()
+
+
 def de_points(shape: Shape, pts: torch.Tensor, powers: torch.Tensor | None = None, with_iters: bool = False,
               ctx: Context | None = None):
     """Distances of (n, 3) f32 points; optionally the per-point iteration count."""
     ctx = ctx or shape.ctx
-    pts = _f32(pts).contiguous()
+    pts = _points(pts)
     n = pts.shape[0]
+    powers = _per_point(powers, n, "powers")
     out = torch.empty(n, dtype=torch.float32, device=pts.device)
     iters = torch.empty(n, dtype=torch.int32, device=pts.device) if with_iters else None
     check(lib.cth_min_distance_dev(ctx.handle, C.byref(shape.desc), _ptr(pts), _ptr(powers), n, _ptr(out),
@@ -46,8 +67,9 @@ def de_points(shape: Shape, pts: torch.Tensor, powers: torch.Tensor | None = Non
 def de_state(shape: Shape, pts: torch.Tensor, powers: torch.Tensor | None = None, ctx: Context | None = None):
     """(r, dr, iters): the loop state the distance is formed from."""
     ctx = ctx or shape.ctx
-    pts = _f32(pts).contiguous()
+    pts = _points(pts)
     n = pts.shape[0]
+    powers = _per_point(powers, n, "powers")
     r = torch.empty(n, dtype=torch.float32, device=pts.device)
     dr = torch.empty_like(r)
     it = torch.empty(n, dtype=torch.int32, device=pts.device)
@@ -91,7 +113,8 @@ def sample_spans(shape: Shape, spans: torch.Tensor, res: int, with_cross: bool =
     return dists, cross
 
 
-_NORMALS = {"fd6": _lib.CTH_NORMALS_FD6, "tangent": _lib.CTH_NORMALS_TANGENT}
+_NORMALS = {"fd6": _lib.CTH_NORMALS_FD6, "tangent": _lib.CTH_NORMALS_TANGENT,
+            "deferred": _lib.CTH_NORMALS_DEFERRED}
 
 
 def mesh(shape: Shape, spans: torch.Tensor, dists: torch.Tensor, res: int, normals: str = "fd6",
@@ -121,6 +144,19 @@ def mesh(shape: Shape, spans: torch.Tensor, dists: torch.Tensor, res: int, norma
                                 _NORMALS[normals], _ptr(voff), _ptr(ioff), nv.value, _ptr(verts), _ptr(idx),
                                 _stream()))
     return voff, ioff, verts[:nv.value], idx[:ni.value]
+
+
+def mesh_vertices(shape: Shape, verts: torch.Tensor, normals: str = "fd6", ctx: Context | None = None):
+    """The vertex pass of surface nets over (nv, 7) records that `mesh` wrote
+    with normals="deferred": fills each normal and distance_from_surface in
+    place."""
+    ctx = ctx or shape.ctx
+    verts = _f32(verts)
+    if verts.dim() != 2 or verts.shape[1] != 7 or not verts.is_contiguous():
+        raise ValueError("vertices must be a contiguous (n, 7) float32 tensor")
+    check(lib.cth_mesh_vertices_dev(ctx.handle, C.byref(shape.desc), _ptr(verts), verts.shape[0], _NORMALS[normals],
+                                    _stream()))
+    return verts
 
 
 def focus_split(ctx: Context, leaves: torch.Tensor, focii: torch.Tensor, eye, depth: torch.Tensor | None = None):
@@ -200,12 +236,34 @@ def render_depth(shape: Shape, cams: torch.Tensor, width: int, height: int, max_
     return depth, steps
 
 
+def get_focii(shape: Shape, cams: torch.Tensor, focus_points: int = 5, max_steps: int = 100, eps: float = 1e-6,
+              with_dirs: bool = False, ctx: Context | None = None):
+    """ShapeMesh::get_focii for (ncams, 10) f32 reference cameras (position,
+    unit direction, fov, aspect, near, far), rays built and marched on the
+    device.  Returns (focii (ncams, fp^2, 3), depth (ncams, fp^2; +inf on a
+    miss), steps[, dirs (ncams, fp^2, 3)])."""
+    ctx = ctx or shape.ctx
+    cams = _f32(cams).contiguous()
+    if cams.dim() != 2 or cams.shape[1] != 10:
+        raise ValueError(f"cameras must have shape (n, 10), got {tuple(cams.shape)}")
+    m, n = cams.shape[0], focus_points * focus_points
+    focii = torch.empty((m, n, 3), dtype=torch.float32, device=cams.device)
+    depth = torch.empty((m, n), dtype=torch.float32, device=cams.device)
+    steps = torch.empty((m, n), dtype=torch.int32, device=cams.device)
+    dirs = torch.empty((m, n, 3), dtype=torch.float32, device=cams.device) if with_dirs else None
+    check(lib.cth_get_focii_dev(ctx.handle, C.byref(shape.desc), _ptr(cams), m, focus_points, max_steps, eps,
+                                _ptr(focii), _ptr(depth), _ptr(steps), _ptr(dirs), _stream()))
+    return (focii, depth, steps, dirs) if with_dirs else (focii, depth, steps)
+
+
 def de_vjp(shape: Shape, pts: torch.Tensor, powers: torch.Tensor | None = None, grad_d: torch.Tensor | None = None,
            with_power: bool = False, ctx: Context | None = None):
     """(d, grad_p[, grad_power]) from the fused forward + reverse sweep."""
     ctx = ctx or shape.ctx
-    pts = _f32(pts).contiguous()
+    pts = _points(pts)
     n = pts.shape[0]
+    powers = _per_point(powers, n, "powers")
+    grad_d = _per_point(grad_d, n, "grad_d")
     d = torch.empty(n, dtype=torch.float32, device=pts.device)
     gp = torch.empty((n, 3), dtype=torch.float32, device=pts.device)
     gpow = torch.empty(n, dtype=torch.float32, device=pts.device) if with_power else None
@@ -218,8 +276,10 @@ def de_grad(shape: Shape, pts: torch.Tensor, powers: torch.Tensor | None = None,
             with_power: bool = False, ctx: Context | None = None):
     """(d, grad_p[, grad_power]) in forward (tangent-linear) mode."""
     ctx = ctx or shape.ctx
-    pts = _f32(pts).contiguous()
+    pts = _points(pts)
     n = pts.shape[0]
+    powers = _per_point(powers, n, "powers")
+    grad_d = _per_point(grad_d, n, "grad_d")
     d = torch.empty(n, dtype=torch.float32, device=pts.device)
     gp = torch.empty((n, 3), dtype=torch.float32, device=pts.device)
     gpow = torch.empty(n, dtype=torch.float32, device=pts.device) if with_power else None
@@ -233,8 +293,9 @@ def normals(shape: Shape, pts: torch.Tensor, powers: torch.Tensor | None = None,
     """Unit normals (n, 3) from one fused launch (parity forward + tangent
     Jacobian); returns (d, normals)."""
     ctx = ctx or shape.ctx
-    pts = _f32(pts).contiguous()
+    pts = _points(pts)
     n = pts.shape[0]
+    powers = _per_point(powers, n, "powers")
     if out is None:
         out = torch.empty((n, 3), dtype=torch.float32, device=pts.device)
     if d is None:
@@ -249,8 +310,9 @@ def normals_fd(shape: Shape, pts: torch.Tensor, h: float, powers: torch.Tensor |
                grad: torch.Tensor | None = None):
     """6-tap central-difference normals; returns (d, normals)."""
     ctx = ctx or shape.ctx
-    pts = _f32(pts).contiguous()
+    pts = _points(pts)
     n = pts.shape[0]
+    powers = _per_point(powers, n, "powers")
     if out is None:
         out = torch.empty((n, 3), dtype=torch.float32, device=pts.device)
     if d is None:

@@ -1,6 +1,7 @@
 """Mesh extraction mirror: `MeshBuffer::generate_for_box`
 (src/mesh/buffer.rs:23-42), whose three steps run on the device through
-cth_mesh_span (corner sampling, vertex placement + normals, quads)."""
+cth_mesh_span (corner sampling, vertex placement + normals, quads), and
+`ShapeMesh::get_focii` (src/mesh/mod.rs:205-243) through cth_get_focii."""
 from __future__ import annotations
 
 import ctypes as C
@@ -49,3 +50,25 @@ class MeshBuffer:
                                 verts.ctypes.data if nv.value else None, nv.value,
                                 idx.ctypes.data if ni.value else None, ni.value, C.byref(nv), C.byref(ni)))
         return cls(verts, idx), Timings(nv.value, ni.value // 6)
+
+
+def get_focii(shape: Shape, position, direction, fov: float, aspect: float, near_plane: float, far_plane: float,
+              focus_points: int = 5) -> np.ndarray:
+    """`ShapeMesh::get_focii(camera, focus_points)` for the camera
+    `Camera::new(position, direction, Projection { fov, aspect, near, far })`
+    (src/camera.rs:26-34,135-144): the hit points, (k, 3) f32 in ray order.
+    Rays are built and marched on the device."""
+    d = np.asarray(direction, np.float32)
+    if not d.any():
+        raise ValueError("direction must not be zero (Camera::new asserts it)")
+    cam = _lib.CthRefCamera()
+    cam.position[:] = [float(v) for v in position]
+    # Camera::new stores the normalised direction (v * (1 / |v|), in f32)
+    m = np.sqrt((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2])
+    cam.direction[:] = [float(v) for v in d * (np.float32(1.0) / m)]
+    cam.fov, cam.aspect, cam.near_plane, cam.far_plane = fov, aspect, near_plane, far_plane
+    out = np.empty((focus_points * focus_points, 3), np.float32)
+    k = C.c_size_t()
+    check(lib.cth_get_focii(shape.ctx.handle, C.byref(shape.desc), C.byref(cam), focus_points, out.ctypes.data,
+                            C.byref(k)))
+    return out[:k.value]

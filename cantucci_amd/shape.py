@@ -42,6 +42,12 @@ class Context:
         return dict(block=l.block, blocks_per_cu=l.blocks_per_cu, chunk=l.chunk, refill_min=l.refill_min,
                     dynamic=l.dynamic)
 
+    def count_work(self, counters) -> None:
+        """Accumulate the refill launches' work into `counters` (a device
+        tensor of 3 int64: loop bodies, bailed evaluations, evaluations), or
+        stop with None."""
+        check(lib.cth_ctx_count_work(self.handle, None if counters is None else counters.data_ptr()))
+
     @property
     def stream(self) -> int:
         return lib.cth_ctx_stream(self.handle) or 0

@@ -106,6 +106,19 @@ typedef struct cth_camera {
     float aspect;
 } cth_camera;
 
+/* The reference's camera (src/camera.rs:16-20 `Camera`, :114-127
+ * `Projection`): position, unit view direction (Camera::new normalises it),
+ * vertical field of view in radians, aspect ratio width / height, near and
+ * far plane.  The up vector is always (0, 0, 1) (src/camera.rs:69-75). */
+typedef struct cth_ref_camera {
+    float position[3];
+    float direction[3];
+    float fov;
+    float aspect;
+    float near_plane;
+    float far_plane;
+} cth_ref_camera;
+
 typedef struct cth_ctx cth_ctx;
 
 /* ---- context ------------------------------------------------------------ */
@@ -149,6 +162,18 @@ enum {
 
 int cth_ctx_set_launch(cth_ctx* ctx, int kernel, const cth_launch* launch);
 int cth_ctx_get_launch(const cth_ctx* ctx, int kernel, cth_launch* launch);
+
+/* Work counters for measurement: while `counters` (device, 3 x uint64) is
+ * set, every lane-refill launch of the context (distances, grids, span
+ * sampling, tracing, the mesh's vertex taps) adds to counters[0] the loop
+ * bodies it ran, to counters[1] its distance evaluations that ended by
+ * bailout (the final magnitude test passed), to counters[2] its distance
+ * evaluations.  With the per-evaluation operation count of the map they give
+ * a launch's algorithmic FLOP.  NULL stops the counting.  The caller zeroes
+ * the counters.  While counting, the launches run counting instantiations of
+ * their kernels, which are slower (about 11% for span sampling): time launches
+ * with counting off.  No counterpart in the reference. */
+int cth_ctx_count_work(cth_ctx* ctx, uint64_t* counters);
 
 /* ---- Shape trait mirror (host buffers, synchronous) ------------------------ */
 
@@ -255,6 +280,37 @@ int cth_sphere_trace(cth_ctx* ctx, const cth_shape* shape, const float* origins,
                      const float* dirs, size_t n, uint32_t max_steps, float eps_abs, float eps_rel, float t_max,
                      float* depth, uint32_t* steps, float* hit_pos);
 
+/* ShapeMesh::get_focii (src/mesh/mod.rs:205-243) for ncams cameras (device
+ * array): focus_points^2 rays per camera, built on the device from the
+ * reference camera exactly as get_focii builds them -- the near-plane
+ * bounding box and cell centres (src/camera.rs:90-103,186-189), mapped to
+ * world space by the inverse of look_at_rh (src/camera.rs:69-80; taken in
+ * closed form, see below), direction normalize(p - position) -- and marched
+ * from the camera position: d = DE(pos); pos += dir * d; a hit when d < eps,
+ * at most max_steps evaluations (the reference: eps 1e-6, 100 steps, no
+ * escape).  Ray r = (cam * fp + x) * fp + y for the cell (x, y) of
+ * iter::square (x outer, src/util/iter.rs:55-87).  focii (3 floats per ray)
+ * receives the position after the last step (get_focii returns it for hits),
+ * depth the distance marched on a hit and +inf on a miss (get_focii drops
+ * those rays), steps the evaluations; dirs (nullable, 3 floats per ray) the
+ * ray directions.  focus_points in [1, 255] (the reference's u8).
+ * The view matrix is a rotation plus a translation, so its inverse is taken
+ * as the matrix with columns s, u, -f and position; cgmath inverts the
+ * general 4x4 matrix by cofactors, which can differ in the entries' last
+ * bits.  Because p - position cancels at the reference's near plane (4e-5,
+ * src/app.rs:98), such a difference can turn a direction by up to about
+ * ulp(|position|) / near_plane radians. */
+int cth_get_focii_dev(cth_ctx* ctx, const cth_shape* shape, const cth_ref_camera* cams, uint32_t ncams,
+                      uint32_t focus_points, uint32_t max_steps, float eps, float* focii, float* depth,
+                      uint32_t* steps, float* dirs, void* stream);
+
+/* ShapeMesh::get_focii for one camera (host buffers, synchronous), with the
+ * reference's constants (eps 1e-6, 100 steps): the hit positions in ray
+ * order, as the reference's filter_map collects them.  focii must hold
+ * focus_points^2 points (3 floats each); *nhits receives their number. */
+int cth_get_focii(cth_ctx* ctx, const cth_shape* shape, const cth_ref_camera* cam, uint32_t focus_points,
+                  float* focii, size_t* nhits);
+
 /* Primary-ray depth images: ncams cameras (device array) x height x width
  * pixel-centre rays, generated on the device in f32 as
  *   u = ((2 (x + 0.5)) / W - 1) * tan_half_fov * aspect,  v = (1 - (2 (y + 0.5)) / H) * tan_half_fov,
@@ -336,8 +392,12 @@ int cth_wall_clock_khz(const cth_ctx* ctx);
 
 /* Stream capture into a HIP graph.  Between begin and end on a non-null
  * stream, the capture-safe `_dev` entry points (distance, grid, span
- * sampling, tracing, tangent gradients/normals, jobs) only record their
- * launches; the graph then replays them with one launch call. */
+ * sampling, tracing, focus rays, tangent gradients/normals, jobs) only record
+ * their launches; the graph then replays them with one launch call.  The
+ * graph owns the work counters of the launches recorded on `stream` by this
+ * thread (allocated by cth_graph_begin, before the capture starts, and freed
+ * by cth_graph_destroy), so replays never share counters with eager launches
+ * on other streams.  One capture per thread at a time. */
 typedef struct cth_graph cth_graph;
 int cth_graph_begin(void* stream);
 int cth_graph_end(void* stream, cth_graph** out);
@@ -353,8 +413,12 @@ void cth_graph_destroy(cth_graph* graph);
  *                        per vertex, distance_from_surface = DE(p) (:254).
  *   CTH_NORMALS_TANGENT  the forward-mode gradient of DE at the vertex,
  *                        normalised (one tangent evaluation per vertex; the
- *                        distance is the parity DE(p)).  Mandelbulb kinds only. */
-enum { CTH_NORMALS_FD6 = 0, CTH_NORMALS_TANGENT = 1 };
+ *                        distance is the parity DE(p)).  Mandelbulb kinds only.
+ *   CTH_NORMALS_DEFERRED no vertex evaluation yet: each record holds its
+ *                        position, in the normal slots its leaf's 6-tap
+ *                        half-steps, and a distance of 0, for
+ *                        cth_mesh_vertices_dev. */
+enum { CTH_NORMALS_FD6 = 0, CTH_NORMALS_TANGENT = 1, CTH_NORMALS_DEFERRED = 2 };
 
 /* The reference's `Vertex` (src/mesh/mod.rs:255-261, #[repr(C)]), 28 bytes. */
 typedef struct cth_vertex {
@@ -387,6 +451,14 @@ int cth_mesh_count_dev(cth_ctx* ctx, const float* dists, size_t nspans, uint32_t
 int cth_mesh_emit_dev(cth_ctx* ctx, const cth_shape* shape, const float* spans, const float* dists, size_t nspans,
                       uint32_t res, int normals, const uint64_t* vert_offsets, const uint64_t* idx_offsets,
                       uint64_t nverts, cth_vertex* vertices, uint32_t* indices, void* stream);
+
+/* Surface nets, the vertex pass on its own: distance_from_surface and normal
+ * (CTH_NORMALS_FD6 or CTH_NORMALS_TANGENT, src/mesh/buffer.rs:254-272) of
+ * nverts records that cth_mesh_emit_dev wrote with CTH_NORMALS_DEFERRED.
+ * cth_mesh_emit_dev with FD6 or TANGENT is the deferred emit followed by this
+ * pass on the same stream; split, the pass can be timed or scheduled alone. */
+int cth_mesh_vertices_dev(cth_ctx* ctx, const cth_shape* shape, cth_vertex* vertices, uint64_t nverts, int normals,
+                          void* stream);
 
 /* MeshBuffer::generate_for_box for one span (host buffers, synchronous):
  * steps 1-3 on the device.  *nverts / *nidx receive the mesh size; vertices /

@@ -105,3 +105,24 @@ def test_bench_refuses_several_gpus():
     res = subprocess.run([sys.executable, str(root / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
                          timeout=120)
     assert res.returncode == 2 and "one GPU" in res.stderr, res.stderr
+
+
+def test_device_wrapper_argument_checks():
+    """The torch wrappers refuse point arrays that are not (n, 3) float32 and
+    per-point arrays (powers, incoming gradients) of the wrong size or
+    dtype, on the host, before any pointer reaches a kernel."""
+    import torch
+    from cantucci_amd import device as dev
+    with pytest.raises(ValueError):
+        dev._points(torch.zeros((4, 2)))
+    with pytest.raises(ValueError):
+        dev._points(torch.zeros(12))
+    with pytest.raises(TypeError):
+        dev._points(torch.zeros((4, 3), dtype=torch.float64))
+    assert dev._points(torch.zeros((4, 3)).t().contiguous().t()).is_contiguous()
+    with pytest.raises(ValueError):
+        dev._per_point(torch.zeros(3), 4, "powers")
+    with pytest.raises(TypeError):
+        dev._per_point(torch.zeros(4, dtype=torch.float64), 4, "grad_d")
+    assert dev._per_point(None, 4, "powers") is None
+    assert dev._per_point(torch.zeros((2, 2)), 4, "powers").numel() == 4

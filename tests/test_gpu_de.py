@@ -230,6 +230,26 @@ def test_empty_and_invalid_device_calls(ctx, torch_cuda):
     assert dev.de_points(shape, torch.zeros((0, 3), device="cuda")).numel() == 0
     with pytest.raises(ca.CthError):
         dev.sample_spans(shape, torch.zeros((1, 6), device="cuda"), 6)  # res not a power of two
+    # batches of 2^31 points or more are refused before any launch (the
+    # buffers below hold 16 points, so nothing is read even if one ran)
+    import ctypes as C
+    from cantucci_amd import _lib
+    buf = torch.zeros((16, 3), device="cuda")
+    for n in (2 ** 31, 2 ** 32 + 5):
+        rc = _lib.lib.cth_normals_fd_dev(ctx.handle, C.byref(shape.desc), buf.data_ptr(), None, n, 1e-3,
+                                         buf.data_ptr(), None, None, None)
+        assert rc == _lib.CTH_E_INVALID and b"2^31" in _lib.lib.cth_last_error()
+    # the torch wrappers check shapes and dtypes before launching
+    with pytest.raises(ValueError):
+        dev.de_points(shape, torch.zeros((8, 2), device="cuda"))
+    with pytest.raises(ValueError):
+        dev.de_points(ca.MandelbulbTrig(15, 2.0, 8.0, ctx=ctx), buf, torch.ones(15, device="cuda"))
+    with pytest.raises(TypeError):
+        dev.de_points(ca.MandelbulbTrig(15, 2.0, 8.0, ctx=ctx), buf, torch.ones(16, dtype=torch.float64,
+                                                                              device="cuda"))
+    with pytest.raises(ValueError):
+        dev.de_vjp(shape, buf, grad_d=torch.ones(4, device="cuda"))
+    torch.cuda.synchronize()
 
 
 def octree_leaf_sample(count=400, seed=0):
@@ -345,3 +365,43 @@ def test_final_ln_is_correctly_rounded(ctx, torch_cuda, bailout, lo, hi):
     agree = same_bits(d.cpu().numpy(), expect)
     print(f"bailout {bailout}: {len(pts)} points, correctly rounded ln on {agree.mean():.7f}")
     assert agree.mean() > 0.99999
+
+
+def test_trig_integral_powers_zero_and_negative(ctx, torch_cuda):
+    """Per-point integral powers 0, -0.0, 1, -2, -3 of the real-power kind take
+    the powi ladder like the oracle (a negative exponent as the reciprocal of
+    the ladder on its magnitude).  At power +-0 the map is (+-0, +-0, 1) and
+    dr stays 1 whatever the transcendentals, so the loop state is
+    bit-identical there; the other powers go through ocml's acos/atan2/sin/cos
+    and are held to the ulp-stable bar of the other trig tests."""
+    import cantucci_amd as ca
+    from cantucci_amd import device as dev
+    torch = torch_cuda
+    rng = np.random.default_rng(77)
+    pts = rng.uniform(-1.5, 1.5, (50000, 3)).astype(F32)
+    choices = np.array([0.0, -0.0, 1.0, -2.0, -3.0], F32)
+    powers = choices[np.arange(len(pts)) % len(choices)]
+    shape = ca.MandelbulbTrig(15, 2.0, 8.0, ctx=ctx)
+    gp, gw = gpu(torch, pts), gpu(torch, powers)
+    d = dev.de_points(shape, gp, gw).cpu().numpy()
+    r, dr, it = (t.cpu().numpy() for t in dev.de_state(shape, gp, gw))
+    s = O.mandelbulb_trig(15, 2.0)
+    d_o, it_o, r_o, dr_o = O.de_points(s, pts, powers)
+    zero = powers == 0.0
+    assert (dr[zero] == 1.0).all() and (dr_o[zero] == 1.0).all()
+    assert (it[zero].astype(np.uint32) == it_o[zero]).all()
+    assert_same_or_both_nan(r[zero], r_o[zero])
+    assert_same_or_both_nan(dr[zero], dr_o[zero])
+    assert_de_parity(d[zero], d_o[zero])
+    assert np.isfinite(d[zero]).all()  # the ladder's r^-1 instead of r^(2^32 - 1)
+    rest = ~zero
+    assert np.mean(it[rest].astype(np.uint32) == it_o[rest]) > 0.99
+    cond = ulp_stable(s, pts[rest], d_o[rest], powers[rest])
+    rel = np.abs(d[rest].astype(np.float64) - d_o[rest]) / np.maximum(np.abs(d_o[rest].astype(np.float64)), 1e-30)
+    print(f"powers 1,-2,-3: iters equal {np.mean(it[rest].astype(np.uint32) == it_o[rest]):.5f}, "
+          f"ulp-stable {cond.mean():.3f} of which rel>1e-5 {np.mean(rel[cond] > 1e-5):.6f}")
+    assert np.mean(rel[cond] > 1e-5) < 1e-3
+    # a uniform power of 0 on the shape itself (no per-point array)
+    d0 = dev.de_points(ca.MandelbulbTrig(15, 2.0, 0.0, ctx=ctx), gp).cpu().numpy()
+    d0_o = O.de_points(O.mandelbulb_trig(15, 2.0, 0.0), pts)[0]
+    assert_de_parity(d0, d0_o)

@@ -15,6 +15,40 @@ def angle(a, b):
     return np.arccos(np.clip(c, -1, 1))
 
 
+def check_trig_job(d_job, nr_job, job, sample=None, seed=0, tag=""):
+    """Parity bar of one real-power grid job (its points p = -1.5 + 3 i / (n - 1)
+    in f32, GridTable order): on the points whose f32 reference distance is
+    ulp-stable (a one-ulp move of any coordinate changes it by < 1e-6
+    relative), fewer than 0.1% differ from the oracle by more than 1e-5
+    relative -- the bar of the other trig tests; elsewhere the f32 orbit is
+    chaotic and transcendentals that differ in the last ulp (ocml here,
+    glibc in the reference) cannot agree.  The forward-mode normals of the
+    same ulp-stable points are within 1e-3 rad of the FP64 dual-number
+    gradient on at least 99% of them.  `sample` limits the check to that many
+    seeded points of the job."""
+    from test_gpu_de import grid_points, ulp_stable
+    n = job["n"]
+    pts = grid_points(n, n - 1.0)
+    sel = np.arange(len(pts))
+    if sample is not None and sample < len(pts):
+        sel = np.sort(np.random.default_rng(seed).choice(len(pts), sample, replace=False))
+    shape = O.mandelbulb_trig(job["max_iters"], 2.0, job["power"])
+    q = pts[sel]
+    dref = O.de_points(shape, q)[0]
+    dg = np.asarray(d_job)[sel]
+    rel = np.abs(dg.astype(np.float64) - dref) / np.maximum(np.abs(dref.astype(np.float64)), 1e-30)
+    cond = ulp_stable(shape, q, dref)
+    _, g64, _ = O.de_grad_f64(shape, q)
+    ok = cond & np.isfinite(g64[:, :3]).all(1) & (np.linalg.norm(g64[:, :3], axis=1) > 0)
+    ang = angle(np.asarray(nr_job)[sel][ok], g64[ok, :3])
+    print(f"{tag}n={n} power={job['power']:.3f} cap={job['max_iters']}: {len(sel)} points, ulp-stable "
+          f"{cond.mean():.3f}, of which rel>1e-5 {np.mean(rel[cond] > 1e-5):.5f}; normals <=1e-3 rad "
+          f"{np.mean(ang <= 1e-3):.4f} (p99 {np.percentile(ang, 99) if ang.size else 0:.2e})")
+    assert cond.mean() > 0.5
+    assert np.mean(rel[cond] > 1e-5) < 1e-3
+    assert np.mean(ang <= 1e-3) >= 0.99
+
+
 def make_jobs(kind):
     if kind == "p8":
         return [dict(n=n, power=8.0, max_iters=cap) for n, cap in ((9, 15), (16, 8), (12, 30), (20, 11), (7, 1))]
@@ -69,13 +103,16 @@ def test_trig_jobs_statistics(ctx, torch_cuda):
     start = torch.zeros(1, dtype=torch.int64, device="cuda")
     js.run(dists, nrm, done, start)
     torch.cuda.synchronize()
-    d = dists.cpu().numpy()
+    d, nr = dists.cpu().numpy(), nrm.cpu().numpy()
     for i, j in enumerate(jobs):
         a, b = js.offsets[i], js.offsets[i + 1]
         dref, _ = O.de_grid(O.mandelbulb_trig(j["max_iters"], 2.0, j["power"]), (-1.5,) * 3, (1.5,) * 3, j["n"],
                             j["n"] - 1.0)
-        rel = np.abs(d[a:b].astype(np.float64) - dref) / np.maximum(np.abs(dref.astype(np.float64)), 1e-30)
-        assert np.mean(rel < 1e-5) > 0.9, (i, np.mean(rel < 1e-5))
+        from test_gpu_de import grid_points
+        assert np.array_equal(O.de_points(O.mandelbulb_trig(j["max_iters"], 2.0, j["power"]),
+                                          grid_points(j["n"], j["n"] - 1.0))[0].view(np.uint32),
+                              dref.view(np.uint32))
+        check_trig_job(d[a:b], nr[a:b], j, tag=f"job {i}: ")
     js.close()
 
 
@@ -148,3 +185,47 @@ def test_jobs_reject_bad_input(ctx, torch_cuda):
         dev.JobSet(ctx, _lib.CTH_SHAPE_MANDELBULB_TRIG, 2.0, [dict(n=4, power=7.0, max_iters=0)])
     with pytest.raises(ca.CthError):
         dev.JobSet(ctx, _lib.CTH_SHAPE_SPHERE, 2.0, [dict(n=4, power=7.0, max_iters=3)])
+
+
+def test_graph_counters_not_shared_with_eager_launches(torch_cuda):
+    """A captured launch keeps its work counters for every replay.  Those
+    belong to the graph, not to the context's ring of 4096 pairs, so an eager
+    launch on another stream that the ring gives the same slot (the 4096th
+    launch after the capture) can run concurrently with a replay without
+    either skipping or repeating work.  Span sampling claims chunks from its
+    counter (dynamic schedule)."""
+    import cantucci_amd as ca
+    from cantucci_amd import _lib
+    from cantucci_amd import device as dev
+    from cantucci_amd.workloads import half_diagonals
+    torch = torch_cuda
+    c = ca.Context(0, "parity")
+    c.set_launch(_lib.CTH_KERNEL_SPANS, 256, 8, 256, 32, dynamic=1)
+    shape = ca.Mandelbulb.classic(15, 2.0, ctx=c)
+    leaves, _ = O.refine(O.mandelbulb(15, 2.0), [-1.5, -1.5, -1.5, 1.5, 1.5, 1.5], 2, 6, half_diagonals(3.0, 6))
+    spans = torch.from_numpy(leaves[:3000].copy()).cuda()
+    tiny = spans[:1].clone()
+    ref, _ = dev.sample_spans(shape, spans, 8, with_cross=False)
+    torch.cuda.synchronize()
+    out_g = torch.empty_like(ref)
+    out_e = torch.empty_like(ref)
+    tiny_out = torch.empty((1, 729), device="cuda")
+    s, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    g = dev.Graph(lambda: dev.sample_spans(shape, spans, 8, with_cross=False, out=out_g), s)
+    for rep in range(2):
+        out_g.fill_(float("nan"))
+        out_e.fill_(float("nan"))
+        torch.cuda.synchronize()
+        # 4095 eager launches after the capture (or the previous overlap), so
+        # that the next eager launch would take the graph's ring slot
+        for _ in range(4095):
+            dev.sample_spans(shape, tiny, 8, with_cross=False, out=tiny_out)
+        torch.cuda.synchronize()
+        g.launch()
+        with torch.cuda.stream(s2):  # another non-blocking stream: overlaps the replay
+            dev.sample_spans(shape, spans, 8, with_cross=False, out=out_e)
+        torch.cuda.synchronize()
+        assert torch.equal(out_g.nan_to_num(7.0), ref.nan_to_num(7.0)), rep
+        assert torch.equal(out_e.nan_to_num(7.0), ref.nan_to_num(7.0)), rep
+    g.close()
+    c.close()

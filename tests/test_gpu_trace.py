@@ -76,35 +76,133 @@ def test_camera_render_matches_oracle(ctx, torch_cuda):
     assert np.percentile(rel, 99.9) < 1e-5
 
 
-def test_get_focii_semantics(ctx, torch_cuda):
-    """The reference's focus-point marcher: absolute epsilon 1e-6, 100 steps,
-    no escape, returning the position after the hitting step
-    (src/mesh/mod.rs:205-243)."""
+def test_render_graph_replay(ctx, torch_cuda):
+    """The 32-pose render captured in a HIP graph replays to the eager result
+    every time, with eager work and synchronisation between the replays."""
     import cantucci_amd as ca
     from cantucci_amd import device as dev
-    from cantucci_amd.workloads import look_at_basis
+    from cantucci_amd.workloads import orbit_poses
     torch = torch_cuda
-    eye = np.array([-3.0, 0.0, 0.0])
-    s, u, f = look_at_basis(eye, (0.0, 0.0, 0.0))
-    # 5 x 5 near-plane cell centres (FOCUS_POINTS = 5), vertical fov 1 rad (src/app.rs:98)
-    th = math.tan(0.5)
-    dirs = []
-    for x in range(5):
-        for y in range(5):
-            a, b = (-1 + (2 * x + 1) / 5) * th * 1.5, (-1 + (2 * y + 1) / 5) * th
-            d = f + s * a + u * b
-            dirs.append(d / np.linalg.norm(d))
-    dirs = np.array(dirs, F32)
-    origin = eye.astype(F32)[None]
-    shape = ca.Mandelbulb.classic(6, 2.5, ctx=ctx)  # the app's shape (src/app.rs:105)
-    d, st, p = dev.sphere_trace(shape, gpu(torch, origin), gpu(torch, dirs), 100, 1e-6, 0.0, float("inf"),
-                                with_pos=True)
+    W, H = 96, 54
+    cams = gpu(torch, orbit_poses(32, seed=0, W=W, H=H))
+    shape = ca.Mandelbulb.classic(15, 2.0, ctx=ctx)
+    ref_d, ref_s = dev.render_depth(shape, cams, W, H)
     torch.cuda.synchronize()
-    d_o, s_o, p_o = O.sphere_trace(O.mandelbulb(6, 2.5), origin, dirs, 100, 0.0, float("inf"), 1e-6, with_pos=True)
-    assert (st.cpu().numpy().astype(np.uint32) == s_o).mean() >= 0.96
-    hit = np.isfinite(d_o)
-    assert hit.any()
-    assert np.allclose(p.cpu().numpy()[hit], p_o[hit], rtol=1e-5, atol=1e-6)
+    depth = torch.empty_like(ref_d)
+    steps = torch.empty_like(ref_s)
+    s = torch.cuda.Stream()
+    g = dev.Graph(lambda: dev.render_depth(shape, cams, W, H, depth=depth, steps=steps), s)
+    for rep in range(3):
+        depth.fill_(-1.0)
+        steps.fill_(-1)
+        torch.cuda.synchronize()
+        g.launch()
+        s.synchronize()
+        assert torch.equal(depth.nan_to_num(-2.0), ref_d.nan_to_num(-2.0)), rep
+        assert torch.equal(steps, ref_s), rep
+        dev.de_points(shape, torch.rand((5000, 3), device="cuda"))  # eager work between replays
+        torch.cuda.synchronize()
+    g.close()
+
+
+def _ref_cameras(n, seed, fov=1.0, aspect=16 / 9, near=4e-5):
+    """Reference cameras (position, unit direction, fov, aspect, near, far)
+    around the Mandelbulb: positions at radius 1.6-3.5 looking near the origin."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        v = rng.normal(size=3)
+        pos = v / np.linalg.norm(v) * rng.uniform(1.6, 3.5)
+        d = -pos + rng.uniform(-0.2, 0.2, 3)
+        d = (d / np.linalg.norm(d)).astype(F32)
+        out.append(np.concatenate([pos.astype(F32), d, np.array([fov, aspect, near, 10.0], F32)]))
+    return np.array(out, F32)
+
+
+@pytest.mark.parametrize("near,fp", [(4e-5, 5), (0.1, 5), (0.01, 4), (4e-5, 1)])
+def test_focus_rays_bitexact(ctx, torch_cuda, near, fp):
+    """get_focii's rays built on the device from the reference camera
+    (src/mesh/mod.rs:205-231, src/camera.rs:69-103,186-189) equal the
+    oracle's bit for bit (fov 1 rad, the app's: its tangent is correctly
+    rounded by glibc too); on a sphere, whose distance is exact arithmetic,
+    the marched focus points, depths and step counts are bit-identical."""
+    import cantucci_amd as ca
+    from cantucci_amd import device as dev
+    torch = torch_cuda
+    cams = _ref_cameras(16, 5 + fp, near=near)
+    shape = ca.Sphere((0.05, -0.02, 0.01), 0.9, ctx=ctx)
+    focii, depth, steps, dirs = dev.get_focii(shape, gpu(torch, cams), fp, with_dirs=True)
+    torch.cuda.synchronize()
+    focii, depth, steps, dirs = (t.cpu().numpy() for t in (focii, depth, steps, dirs))
+    sh = O.sphere((0.05, -0.02, 0.01), 0.9)
+    for i, cam in enumerate(cams):
+        pos_o, depth_o, steps_o, dirs_o = O.get_focii(sh, cam, fp)
+        assert (dirs[i].view(np.uint32) == dirs_o.view(np.uint32)).all(), i
+        assert (steps[i].astype(np.uint32) == steps_o).all(), i
+        assert (depth[i].view(np.uint32) == depth_o.view(np.uint32)).all(), i
+        hit = np.isfinite(depth_o)
+        assert (focii[i][hit].view(np.uint32) == pos_o[hit].view(np.uint32)).all(), i
+        # a miss (get_focii drops it) marches off to +-inf and NaN within its
+        # 100 steps; a NaN the arithmetic creates has the sign bit set on x86
+        # and clear on the GPU, so misses compare up to the NaN sign
+        fm, om = focii[i][~hit], pos_o[~hit]
+        assert (np.isnan(fm) == np.isnan(om)).all() and (fm[~np.isnan(fm)] == om[~np.isnan(om)]).all(), i
+    assert np.isfinite(depth).any()
+    if fp > 1:
+        assert np.isinf(depth).any()
+
+
+def test_get_focii_mandelbulb(ctx, torch_cuda):
+    """get_focii with the app's shape and projection (classic(6, 2.5), fov
+    1 rad, near 4e-5: src/app.rs:98,105), 64 cameras x 25 rays: directions
+    bit-identical to the oracle; the marched points agree except where the
+    final ln (correctly rounded here, glibc's logf in the reference) differs
+    by an ulp on some step, which can move a ray by an ulp from then on."""
+    import cantucci_amd as ca
+    from cantucci_amd import device as dev
+    torch = torch_cuda
+    cams = _ref_cameras(64, 11)
+    shape = ca.Mandelbulb.classic(6, 2.5, ctx=ctx)
+    focii, depth, steps, dirs = dev.get_focii(shape, gpu(torch, cams), 5, with_dirs=True)
+    torch.cuda.synchronize()
+    focii, depth, steps, dirs = (t.cpu().numpy() for t in (focii, depth, steps, dirs))
+    sh = O.mandelbulb(6, 2.5)
+    res = [O.get_focii(sh, cam, 5) for cam in cams]
+    pos_o = np.stack([r[0] for r in res])
+    depth_o = np.stack([r[1] for r in res])
+    steps_o = np.stack([r[2] for r in res])
+    dirs_o = np.stack([r[3] for r in res])
+    assert (dirs.view(np.uint32) == dirs_o.view(np.uint32)).all()
+    hit, hit_o = np.isfinite(depth), np.isfinite(depth_o)
+    same = steps.astype(np.uint32) == steps_o
+    both = hit & hit_o
+    dpos = np.linalg.norm(focii[both].astype(np.float64) - pos_o[both], axis=1)
+    bit = (focii.view(np.uint32) == pos_o.view(np.uint32)).all(-1)
+    print(f"focus rays {depth.size}: hits {hit_o.mean():.3f}, hit agreement {np.mean(hit == hit_o):.4f}, "
+          f"steps equal {same.mean():.4f}, points bit-equal {bit.mean():.4f}, "
+          f"max |dpos| (both hit) {dpos.max() if dpos.size else 0:.2e}")
+    assert hit_o.mean() > 0.2
+    assert np.mean(hit == hit_o) > 0.97
+    assert same.mean() > 0.9
+    assert np.percentile(dpos, 99) < 1e-4
+
+
+def test_get_focii_host_mirror(ctx):
+    """ShapeMesh::get_focii through the host entry point: the hit points in
+    ray order (the reference's filter_map), equal to the device batch."""
+    import cantucci_amd as ca
+    from cantucci_amd.mesh import get_focii
+    shape = ca.Sphere((0.0, 0.0, 0.0), 0.8, ctx=ctx)
+    pts = get_focii(shape, (2.5, 0.4, 0.3), (-2.5, -0.4, -0.3), 0.5, 1.5, 0.1, 10.0, 5)
+    d = np.array([-2.5, -0.4, -0.3], F32)
+    d = d * (F32(1.0) / np.sqrt((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]))
+    cam = np.concatenate([np.array([2.5, 0.4, 0.3], F32), d, np.array([0.5, 1.5, 0.1, 10.0], F32)])
+    pos_o, depth_o, _, _ = O.get_focii(O.sphere((0, 0, 0), 0.8), cam, 5)
+    exp = pos_o[np.isfinite(depth_o)]
+    assert 0 < len(pts) < 25
+    assert (pts.view(np.uint32) == exp.view(np.uint32)).all()
+    with pytest.raises(ValueError):
+        get_focii(shape, (1, 0, 0), (0, 0, 0), 1.0, 1.0, 0.1, 10.0)
 
 
 def test_trace_rejects_bad_arguments(ctx, torch_cuda):
@@ -116,3 +214,9 @@ def test_trace_rejects_bad_arguments(ctx, torch_cuda):
         dev.sphere_trace(shape, torch.zeros((1, 3), device="cuda"), torch.ones((4, 3), device="cuda"), 0)
     d, s = dev.sphere_trace(shape, torch.zeros((1, 3), device="cuda"), torch.ones((0, 3), device="cuda"), 10)
     assert d.numel() == 0
+    cams = torch.zeros((1, 10), device="cuda")
+    for fp, steps in ((0, 100), (256, 100), (5, 0)):
+        with pytest.raises(ca.CthError):
+            dev.get_focii(shape, cams, fp, max_steps=steps)
+    with pytest.raises(ValueError):
+        dev.get_focii(shape, torch.zeros((1, 9), device="cuda"))
