@@ -324,11 +324,19 @@ def normals_fd(shape: Shape, pts: torch.Tensor, h: float, powers: torch.Tensor |
 
 class MandelbulbDistance(torch.autograd.Function):
     """torch.autograd.Function over the C ABI: forward = cth_min_distance_dev,
-    backward = cth_de_vjp_dev (forward recomputed per lane, reverse sweep)."""
+    backward = the vector-Jacobian product grad_d * (dd/dp, dd/dpower), with
+    the forward iteration recomputed per lane.  Two kernels compute it:
+    "forward" (default) carries the orbit's Jacobian through every loop body
+    in one launch and needs no workspace (cth_de_grad_dev); "reverse" records
+    the orbit and sweeps back through the power map (cth_de_vjp_dev).  With 3
+    point coordinates and a power as the only inputs, the forward sweep does
+    less work: the reverse sweep has to re-evaluate every transcendental of
+    the map on its way back."""
 
     @staticmethod
-    def forward(ctx_, pts, powers, shape):
+    def forward(ctx_, pts, powers, shape, sweep):
         ctx_.shape = shape
+        ctx_.sweep = sweep
         ctx_.save_for_backward(pts, powers)
         return de_points(shape, pts.detach(), None if powers is None else powers.detach())
 
@@ -336,14 +344,19 @@ class MandelbulbDistance(torch.autograd.Function):
     def backward(ctx_, grad_d):
         pts, powers = ctx_.saved_tensors
         shape = ctx_.shape
-        _, gp, gpow = de_vjp(shape, pts.detach(), None if powers is None else powers.detach(),
-                             grad_d.contiguous(), with_power=True)
-        return gp, (gpow if (powers is not None and ctx_.needs_input_grad[1]) else None), None
+        fn = de_grad if ctx_.sweep == "forward" else de_vjp
+        _, gp, gpow = fn(shape, pts.detach(), None if powers is None else powers.detach(), grad_d.contiguous(),
+                         with_power=True)
+        return gp, (gpow if (powers is not None and ctx_.needs_input_grad[1]) else None), None, None
 
 
-def distance(shape: Shape, pts: torch.Tensor, powers: torch.Tensor | None = None) -> torch.Tensor:
-    """Differentiable distance estimate (autograd through the adjoint kernel)."""
-    return MandelbulbDistance.apply(pts, powers, shape)
+def distance(shape: Shape, pts: torch.Tensor, powers: torch.Tensor | None = None, sweep: str = "forward"
+             ) -> torch.Tensor:
+    """Differentiable distance estimate; `sweep` picks the backward kernel
+    ("forward": tangent-linear, "reverse": orbit record + reverse sweep)."""
+    if sweep not in ("forward", "reverse"):
+        raise ValueError("sweep must be 'forward' or 'reverse'")
+    return MandelbulbDistance.apply(pts, powers, shape, sweep)
 
 
 class JobSet:

@@ -140,17 +140,28 @@ def test_fd_normals_match_oracle_stencil(ctx, torch_cuda):
         assert np.mean(ang < 1e-3) > 0.98
 
 
-def test_autograd_function(ctx, torch_cuda):
+@pytest.mark.parametrize("sweep", ["forward", "reverse"])
+def test_autograd_function(ctx, torch_cuda, sweep):
+    """The autograd backward is the chosen kernel's vector-Jacobian product,
+    bit for bit, and the two sweeps agree with each other and with the FP64
+    dual-number gradient on well-conditioned points."""
     import cantucci_amd as ca
     from cantucci_amd import device as dev
     torch = torch_cuda
-    pts = torch.tensor(np.random.default_rng(7).uniform(-1.2, 1.2, (256, 3)).astype(F32), device="cuda",
-                       requires_grad=True)
+    pts_np = np.random.default_rng(7).uniform(-1.2, 1.2, (256, 3)).astype(F32)
+    pts = torch.tensor(pts_np, device="cuda", requires_grad=True)
     powers = torch.full((256,), 7.5, device="cuda", requires_grad=True)
     shape = ca.MandelbulbTrig(15, 2.0, 8.0, ctx=ctx)
-    d = dev.distance(shape, pts, powers)
+    d = dev.distance(shape, pts, powers, sweep=sweep)
     (d * 3.0).sum().backward()
-    _, gp, gpow = dev.de_vjp(shape, pts.detach(), powers.detach(), torch.full((256,), 3.0, device="cuda"),
-                             with_power=True)
+    fn = dev.de_grad if sweep == "forward" else dev.de_vjp
+    _, gp, gpow = fn(shape, pts.detach(), powers.detach(), torch.full((256,), 3.0, device="cuda"), with_power=True)
     torch.cuda.synchronize()
     assert torch.equal(pts.grad, gp) and torch.equal(powers.grad, gpow)
+    _, g64, it64 = O.de_grad_f64(O.mandelbulb_trig(15, 2.0), pts_np, np.full(256, 7.5, F32))
+    it_o = O.de_points(O.mandelbulb_trig(15, 2.0), pts_np, np.full(256, 7.5, F32))[1]
+    ok = (it_o == it64) & np.isfinite(g64).all(1)
+    ang = angle(gp.cpu().numpy()[ok], 3.0 * g64[ok, :3])
+    assert np.median(ang) < 1e-5 and np.mean(ang <= 1e-3) > 0.97
+    with pytest.raises(ValueError):
+        dev.distance(shape, pts, powers, sweep="sideways")
