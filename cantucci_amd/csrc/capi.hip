@@ -113,6 +113,7 @@ int cth_ctx_create(int device, cth_ctx** out) {
     if (rc == CTH_OK)
         rc = check_hip(hipMemset(c->counters, 0, 2 * COUNTER_RING * sizeof(uint32_t)), "hipMemset counters");
     if (rc == CTH_OK) rc = check_hip(hipHostMalloc((void**)&c->pinned, 8 * sizeof(uint64_t)), "hipHostMalloc");
+    if (rc == CTH_OK) rc = vjp_scratch_init(device);
     if (rc == CTH_OK) rc = check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
     if (rc != CTH_OK) {
         cth_ctx_destroy(c);
@@ -399,6 +400,29 @@ int cth_de_vjp_dev(cth_ctx* c, const cth_shape* s, const float* pts, const float
         gp = nullptr;
     }
     return launch_adjoint(*c, *s, pts, powers, (uint32_t)n, grad_d, d, grad_p, gp, nullptr, st);
+}
+
+int cth_de_vjp_noalloc_dev(cth_ctx* c, const cth_shape* s, const float* pts, const float* powers, size_t n,
+                           const float* grad_d, float* d, float* grad_p, float* grad_power, void* stream) {
+    if (!c) return fail(CTH_E_INVALID, "null context");
+    if (int rc = check_shape(s)) return rc;
+    if (n && !pts) return fail(CTH_E_INVALID, "null buffer");
+    if (s->kind == CTH_SHAPE_SPHERE) return fail(CTH_E_UNSUPPORTED, "use the closed form for a sphere");
+    const size_t need = vjp_workspace_bytes(s->max_iters, n);
+    if (need > kVjpScratchBytes)
+        return fail(CTH_E_INVALID, "n * ((max_iters + 1) * 16 + 16) bytes exceeds the 16 MiB scratch of "
+                                   "cth_de_vjp_noalloc_dev (" + std::to_string(need) + " bytes needed)");
+    void* ws = vjp_scratch(c->device);
+    if (!ws) return fail(CTH_E_DEVICE, "the device's scratch is not set up");
+    hipStream_t st = pick_stream(c, stream);
+    float* gp = grad_power;
+    if (grad_power && s->kind != CTH_SHAPE_MANDELBULB_TRIG) {
+        int rc = launch_fill_u32(reinterpret_cast<uint32_t*>(grad_power), n, 0u, st);
+        if (rc) return rc;
+        gp = nullptr;
+    }
+    return launch_adjoint(*c, *s, pts, powers, (uint32_t)n, grad_d, d, grad_p, gp, nullptr, st, ws,
+                          kVjpScratchBytes);
 }
 
 int cth_normals_dev(cth_ctx* c, const cth_shape* s, const float* pts, const float* powers, size_t n, float* d,

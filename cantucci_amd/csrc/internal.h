@@ -145,8 +145,18 @@ int launch_trace_focii(Ctx& c, const cth_shape& s, const cth_ref_camera* cams, u
 int launch_trace_rays(Ctx& c, const cth_shape& s, const float* origins, int per_ray_origin, const float* dirs,
                       uint32_t n, uint32_t max_steps, float eps_abs, float eps_rel, float t_max, float* depth,
                       uint32_t* steps, float* hit_pos, hipStream_t st);
+// Reverse-sweep VJP.  workspace NULL: stream-ordered allocations; else the
+// caller's ws_bytes of device memory (at least vjp_workspace_bytes).
 int launch_adjoint(Ctx& c, const cth_shape& s, const float* pts, const float* powers, uint32_t n,
-                   const float* grad_d, float* d, float* grad_p, float* grad_power, float* normals, hipStream_t st);
+                   const float* grad_d, float* d, float* grad_p, float* grad_power, float* normals, hipStream_t st,
+                   void* workspace = nullptr, size_t ws_bytes = 0);
+// The allocation-free VJP's device scratch (kernels_adjoint.hip): its size,
+// the per-device set-up (hipGetSymbolAddress with `device` current, done by
+// cth_ctx_create) and the device address it recorded.
+constexpr size_t kVjpScratchBytes = size_t(16) << 20;
+int vjp_scratch_init(int device);
+void* vjp_scratch(int device);
+size_t vjp_workspace_bytes(uint32_t max_iters, size_t n);
 int launch_tangent(Ctx& c, const cth_shape& s, const float* pts, const float* powers, uint32_t n,
                    const float* grad_d, float* d, float* grad_p, float* grad_power, float* normals, hipStream_t st);
 int launch_fd_normals(Ctx& c, const cth_shape& s, const float* pts, const float* powers, uint32_t n, float h,

@@ -20,6 +20,8 @@
 // neither phase pays for the spread of iteration counts.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "internal.h"
 #include "launch.h"
 #include "point_sources.h"
@@ -395,8 +397,48 @@ __global__ void k_fd_combine(const float* __restrict__ taps, uint32_t n, float* 
 
 static inline uint32_t cdiv_adj(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free VJP (cth_de_vjp_noalloc_dev): the orbit
+// records and end states of one call, in a file-scope device array.  Every
+// GPU the library runs on holds its own copy from the code object's load
+// until the process ends.  The host refers to a device's copy only through
+// the address hipGetSymbolAddress gives with that device current (taken once
+// per device at context creation); the array's name is a host-side stand-in.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[kVjpScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+int vjp_scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return fail(CTH_E_DEVICE, "device index above 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return CTH_OK;
+    void* ptr = nullptr;
+    int rc = check_hip(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (rc) return rc;
+    g_scratch_ptr[device] = ptr;
+    return CTH_OK;
+}
+
+void* vjp_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+
+size_t vjp_workspace_bytes(uint32_t max_iters, size_t n) {
+    const size_t stride = (size_t)max_iters + 1;
+    const size_t orbit = (n * stride * sizeof(float4) + 255) & ~size_t(255);
+    return orbit + n * sizeof(AdjRecord);
+}
+
 int launch_adjoint(Ctx& c, const cth_shape& s, const float* pts, const float* powers, uint32_t n,
-                   const float* grad_d, float* d, float* grad_p, float* grad_power, float* normals, hipStream_t st) {
+                   const float* grad_d, float* d, float* grad_p, float* grad_power, float* normals, hipStream_t st,
+                   void* workspace, size_t ws_bytes) {
     if (n == 0) return CTH_OK;
     if (s.kind == CTH_SHAPE_SPHERE) return fail(CTH_E_UNSUPPORTED, "use the closed form for a sphere");
     MandelUniform u;
@@ -405,13 +447,26 @@ int launch_adjoint(Ctx& c, const cth_shape& s, const float* pts, const float* po
     if (rc) return rc;
     const size_t stride = (size_t)u.max_iters + 1;
     if (stride > 4096) return fail(CTH_E_INVALID, "max_iters above 4095 for the adjoint");
-    // Workspace in batches of at most ~1 GiB of orbit records.
     size_t per_point = stride * sizeof(float4) + sizeof(AdjRecord);
-    size_t batch = std::max<size_t>(1, std::min<size_t>(n, (size_t(1) << 30) / per_point));
     float4* orbit = nullptr;
     AdjRecord* rec = nullptr;
-    rc = check_hip(hipMallocAsync((void**)&orbit, batch * stride * sizeof(float4), st), "hipMallocAsync orbit");
-    if (!rc) rc = check_hip(hipMallocAsync((void**)&rec, batch * sizeof(AdjRecord), st), "hipMallocAsync records");
+    size_t batch = 0;
+    const bool own = workspace == nullptr;
+    if (own) {
+        // Workspace in batches of at most ~1 GiB of orbit records.
+        batch = std::max<size_t>(1, std::min<size_t>(n, (size_t(1) << 30) / per_point));
+        rc = check_hip(hipMallocAsync((void**)&orbit, batch * stride * sizeof(float4), st), "hipMallocAsync orbit");
+        if (!rc) rc = check_hip(hipMallocAsync((void**)&rec, batch * sizeof(AdjRecord), st), "hipMallocAsync records");
+    } else {
+        // The caller's workspace holds the whole call: orbit records, then
+        // (256-byte aligned) the end states.  The forward kernel writes every
+        // record the reverse kernel reads, so nothing needs clearing.
+        if (vjp_workspace_bytes(u.max_iters, n) > ws_bytes) return fail(CTH_E_INVALID, "workspace too small");
+        batch = n;
+        orbit = static_cast<float4*>(workspace);
+        rec = reinterpret_cast<AdjRecord*>(static_cast<char*>(workspace) +
+                                           ((n * stride * sizeof(float4) + 255) & ~size_t(255)));
+    }
     for (size_t off = 0; off < n && !rc; off += batch) {
         uint32_t m = (uint32_t)std::min<size_t>(batch, n - off);
         SrcPoints src{pts + 3 * off, powers ? powers + off : nullptr, s.power};
@@ -438,8 +493,8 @@ int launch_adjoint(Ctx& c, const cth_shape& s, const float* pts, const float* po
                                out, q2);
         rc = check_hip(hipGetLastError(), "adjoint reverse launch");
     }
-    if (orbit) (void)hipFreeAsync(orbit, st);
-    if (rec) (void)hipFreeAsync(rec, st);
+    if (own && orbit) (void)hipFreeAsync(orbit, st);
+    if (own && rec) (void)hipFreeAsync(rec, st);
     return rc;
 }
 

@@ -335,6 +335,20 @@ int cth_render_depth_dev(cth_ctx* ctx, const cth_shape* shape, const cth_camera*
 int cth_de_vjp_dev(cth_ctx* ctx, const cth_shape* shape, const float* pts, const float* powers, size_t n,
                    const float* grad_d, float* d, float* grad_p, float* grad_power, void* stream);
 
+/* cth_de_vjp_dev without any allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid it).  The orbit
+ * workspace is a 16 MiB device array of the library, one per GPU, held from
+ * the library's load until the process ends; the call allocates and frees
+ * nothing.  A call needs n * ((max_iters + 1) * 16 + 16) bytes of it, plus
+ * at most 255 bytes of alignment; larger calls fail with CTH_E_INVALID
+ * (15 iterations: up to 61,680 points per call).  Results are those of
+ * cth_de_vjp_dev, bit for bit.  Every call on one GPU uses the same array, so
+ * calls that overlap in time on one GPU (other streams, other contexts,
+ * other threads) race and must be ordered by the caller. */
+int cth_de_vjp_noalloc_dev(cth_ctx* ctx, const cth_shape* shape, const float* pts, const float* powers, size_t n,
+                           const float* grad_d, float* d, float* grad_p, float* grad_power, void* stream);
+
 /* The same product as cth_de_vjp_dev, computed in forward (tangent-linear)
  * mode: the Jacobian of the orbit with respect to p (and the power) is carried
  * through every loop body, so no orbit workspace is needed.  One launch. */
