@@ -30,6 +30,19 @@ SPACE = {
     "dynamic": [0, 1],
 }
 
+# The trace kernel applies finished evaluations' steps in the iteration
+# they finish in, and refill_min only batches the fetch of new rays, which
+# pays off at small batches.
+SPACES = {
+    "trace": {
+        "block": [256],
+        "blocks_per_cu": [4, 8, 12, 16],
+        "chunk": [256, 1024],
+        "refill_min": [1, 2, 4, 8, 16],
+        "dynamic": [0, 1],
+    },
+}
+
 KERNEL_OF = {
     "grid": _lib.CTH_KERNEL_DE,
     "fd": _lib.CTH_KERNEL_DE,
@@ -165,7 +178,7 @@ def main() -> None:
     cache.setdefault(key, {})
     for name in args.workloads.split(","):
         t = time.time()
-        best, results = tune(ctx, KERNEL_OF[name], runs[name])
+        best, results = tune(ctx, KERNEL_OF[name], runs[name], SPACES.get(name, SPACE))
         cache[key][name] = {
             "config": best,
             "best_ms": results[0][0],

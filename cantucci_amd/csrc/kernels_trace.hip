@@ -1,14 +1,16 @@
-// Sphere tracing with two-level lane refill.
+// Sphere tracing with lane refill.
 //
 // The reference marches 25 focus rays per frame on the CPU
 // (ShapeMesh::get_focii, src/mesh/mod.rs:205-243): d = DE(pos); pos += dir * d;
 // stop when d < EPSILON, at most 100 steps, no escape test.  Here every lane
 // owns one ray; each ray step is one distance evaluation whose loop bodies run
 // in the same branch-free ticks as the plain distance kernel.  When a lane's
-// evaluation finishes, the lane applies the step (pos += dir * d, t += d) and
-// either starts the ray's next evaluation or writes the ray and takes a new
-// one, so lanes stay busy although both the evaluation lengths and the ray
-// step counts differ wildly.  The hit test is d < eps_abs + eps_rel * t
+// evaluation finishes, the lane applies the step (pos += dir * d, t += d) in
+// the same iteration and either starts the ray's next evaluation or writes
+// the ray, so lanes stay busy although both the evaluation lengths and the
+// ray step counts differ wildly.  (On the camera workload an evaluation runs
+// 1.6 loop bodies on average and a third run none, so the steps are as much
+// of the work as the bodies.)  The hit test is d < eps_abs + eps_rel * t
 // (eps_rel = 0 gives the reference's absolute test; eps_abs = 0 the relative
 // one), a ray escapes when t > t_max (+inf: never, like the reference).
 #include <hip/hip_runtime.h>
@@ -157,11 +159,20 @@ template <class Ev, class Rays, bool kCount = false>
 __global__ __launch_bounds__(256) void k_trace(
     Rays rays, MandelUniform u, TraceParams tp, TraceOut out, WorkQueue q) {
     Ev ev;
-    // Wave-uniform lane masks, as in refill_loop (work_loop.h): lanes with an
-    // evaluation in progress, and lanes whose evaluation finished and whose
-    // step is not applied yet.  `has_ray` is only read at the batched refill.
+    // Every lane that holds a ray is in one of two states, kept as
+    // wave-uniform lane masks: running loop bodies of a distance evaluation
+    // (live_m), or holding a finished evaluation whose step is due (step_m:
+    // an evaluation from beyond the bailout radius runs no body, so it is
+    // finished as soon as it starts).  One iteration of the loop runs one
+    // loop body on the live lanes and then, on the lanes whose evaluation is
+    // finished, the step: pos += dir d, t += d, the end tests, and the next
+    // evaluation's start.  A lane therefore does one unit of work per
+    // iteration whichever state it is in, and an evaluation's step never
+    // waits for the rest of the wave.  Lanes whose ray ended take new rays
+    // in batches of at least refill_min (ray generation is the costly part
+    // of a start).
     const uint64_t wave = wave_mask(true);
-    uint64_t live_m = 0, pend_m = 0;
+    uint64_t live_m = 0, step_m = 0, ray_m = 0;
     bool has_ray = false;
     uint32_t idx = 0, nsteps = 0;
     uint32_t tick = 0, t0 = 0;  // wave's body count; tick at which the lane's evaluation started
@@ -171,6 +182,13 @@ __global__ __launch_bounds__(256) void k_trace(
     bool exhausted = false;
     WorkTally<kCount> tally;
 
+    // Start the lane's next evaluation at pos; zero loop bodies when |pos|
+    // is beyond the bailout radius (its step is then due at once).
+    auto start = [&]() {
+        ev.start(pos[0], pos[1], pos[2], u.power, u);
+        t0 = tick;
+        r = ev.mag();
+    };
     // One sphere-tracing step of the lane's ray by distance d.  When the ray
     // ends (hit, escape or the step limit) its outputs are written and the
     // lane has no ray.
@@ -194,31 +212,10 @@ __global__ __launch_bounds__(256) void k_trace(
     };
 
     for (;;) {
-        if ((uint32_t)__popcll(wave & ~live_m) >= q.refill_min) {
-            // step_m: lanes whose evaluation finished (after loop bodies, or
-            // with none when the point is beyond the bailout radius) and whose
-            // step is not applied yet; new_m: lanes that just took a ray.
-            // Steps are applied and the next evaluations started over and
-            // over while some lane has a zero-body step; new rays are fetched
-            // only when none has, so that rays are generated in batches.
-            uint64_t step_m = pend_m, new_m = 0;
-            for (;;) {
-                if (__builtin_amdgcn_inverse_ballot_w64(step_m)) step(ev.finish(r));
-                const uint64_t start_m = (step_m | new_m) & wave_mask(has_ray);
-                if (__builtin_amdgcn_inverse_ballot_w64(start_m)) {
-                    ev.start(pos[0], pos[1], pos[2], u.power, u);
-                    t0 = tick;
-                    r = ev.mag();
-                }
-                step_m = wave_mask(r > u.bailout) & start_m;  // zero loop bodies: finished already
-                live_m |= start_m & ~step_m;
-                tally.eval(start_m);
-                tally.bail(step_m);
-                new_m = 0;
-                if (step_m != 0) continue;
-                // New rays for lanes without one.
-                const uint64_t need = wave_mask(!has_ray);
-                if (exhausted || need == 0) break;
+        // New rays for the lanes without one.
+        uint64_t need = wave & ~ray_m;
+        if (!exhausted && (uint32_t)__popcll(need) >= q.refill_min) {
+            while (need != 0) {
                 if (c_next >= c_end) {
                     const uint32_t b = next_chunk(q, round);
                     if (b >= q.total) {
@@ -231,31 +228,59 @@ __global__ __launch_bounds__(256) void k_trace(
                 const uint32_t avail = c_end - c_next;
                 const uint32_t nneed = (uint32_t)__popcll(need);
                 const uint32_t rank = lane_rank(need);
-                const bool take = !has_ray && rank < avail;
+                const bool take = __builtin_amdgcn_inverse_ballot_w64(need) && rank < avail;
                 if (take) {
                     idx = c_next + rank;
                     rays.get(idx, pos, dir);
                     t = 0.0f;
                     nsteps = 0;
                     has_ray = true;
+                    start();
                 }
-                new_m = wave_mask(take);
+                const uint64_t took = wave_mask(take);
+                const uint64_t zero = wave_mask(r > u.bailout) & took;
+                tally.eval(took);
+                tally.bail(zero);
+                live_m |= took & ~zero;
+                step_m |= zero;
+                ray_m |= took;
+                need &= ~took;
                 c_next += (nneed < avail) ? nneed : avail;
             }
-            pend_m = 0;
-            if (live_m == 0) break;
         }
-        const bool live = __builtin_amdgcn_inverse_ballot_w64(live_m);
-        const uint64_t cap_m = wave_mask(t0 == tick + 1u - u.max_iters) & live_m;  // the max_iters-th body
-        ev.body(r, u, live);
-        ++tick;
-        tally.body(live_m);
-        const float rn = ev.mag();
-        const uint64_t bail_m = wave_mask(rn > u.bailout) & live_m & ~cap_m;
-        tally.bail(bail_m);
-        if (__builtin_amdgcn_inverse_ballot_w64(live_m & ~cap_m)) r = rn;
-        live_m &= ~(cap_m | bail_m);
-        pend_m |= cap_m | bail_m;
+        // Every lane with a ray is live or has a step due; none left means
+        // the index space is exhausted (a wave without rays always fetches).
+        if (ray_m == 0) break;
+        uint64_t done_m = 0;
+        if (live_m != 0) {
+            const bool live = __builtin_amdgcn_inverse_ballot_w64(live_m);
+            const uint64_t cap_m = wave_mask(t0 == tick + 1u - u.max_iters) & live_m;  // the max_iters-th body
+            ev.body(r, u, live);
+            ++tick;
+            tally.body(live_m);
+            const float rn = ev.mag();
+            const uint64_t bail_m = wave_mask(rn > u.bailout) & live_m & ~cap_m;
+            tally.bail(bail_m);
+            // at the cap r stays the magnitude before the last body
+            // (src/shape/mandelbulb.rs:65-78)
+            if (__builtin_amdgcn_inverse_ballot_w64(live_m & ~cap_m)) r = rn;
+            done_m = cap_m | bail_m;
+            live_m &= ~done_m;
+        }
+        const uint64_t go_m = done_m | step_m;
+        if (go_m != 0) {
+            if (__builtin_amdgcn_inverse_ballot_w64(go_m)) {
+                step(ev.finish(r));
+                if (has_ray) start();
+            }
+            const uint64_t cont = go_m & wave_mask(has_ray);
+            const uint64_t zero = wave_mask(r > u.bailout) & cont;
+            tally.eval(cont);
+            tally.bail(zero);
+            ray_m &= ~(go_m & ~cont);
+            live_m |= cont & ~zero;
+            step_m = zero;
+        }
     }
     tally.flush(q.stats);
     if (q.dynamic) wave_retire(q);
