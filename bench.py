@@ -249,6 +249,11 @@ def main() -> None:
             "sample_1thread": "4 of the 64 leaves on one thread, scaled to 64"}
 
     extra = {}
+    if args.mode == "parity":
+        try:
+            extra["contract_mode"] = bench_contract(args, oc, hd, leaves_buf, dists, nverts, nquads)
+        except Exception as e:  # a secondary leg must not hide the headline
+            extra["contract_mode"] = {"error": f"{type(e).__name__}: {e}"}
     for name, fn in (("grid_64", bench_grid), ("trace_1080p", bench_trace), ("normals_shell", bench_normals),
                      ("mixed_jobs", bench_jobs), ("reference_default_mesh", bench_ref_default),
                      ("point_query", bench_point_query)):
@@ -296,6 +301,85 @@ def main() -> None:
         "extra": extra,
     }
     print(json.dumps(line))
+
+
+def bench_contract(args, oc, hd, leaves_buf, parity_dists, nverts_parity, nquads_parity):
+    """The headline step with FMA contraction (CTH_MODE_CONTRACT: the parity
+    sequence with products fused into following sums, IEEE division and
+    square roots): step time, both kernels' rooflines, and the distance of its
+    corner distances from the parity mode's on every corner (parity is the
+    reference's operation sequence; the FP64 comparison is the GPU test
+    tests/test_gpu_contract.py)."""
+    import torch
+
+    import cantucci_amd as ca
+    from cantucci_amd import autotune
+    from cantucci_amd import device as dev
+
+    ctx = ca.Context(0, "contract")
+    autotune.apply(ctx, "spans")
+    autotune.apply(ctx, "mesh")
+    shape = ca.Mandelbulb.classic(oc["max_iters"], oc["bailout"], ctx=ctx)
+    res = oc["res"]
+    nleaves = leaves_buf.shape[0]
+    # this mode's own refinement can split a few centres differently
+    lv, lc = dev.refine(shape, oc["root"], oc["min_depth"], oc["max_depth"], hd)
+    nleaves_contract = lv.shape[0]
+    center_evals = int(sum(int(lc[d]) for d in range(oc["min_depth"], oc["max_depth"] + 1)))
+    lv_buf = torch.empty((max(nleaves, nleaves_contract), 6), device="cuda")
+    dists = torch.empty((lv_buf.shape[0], (res + 1) ** 3), device="cuda")
+    cross = torch.empty((lv_buf.shape[0], (res ** 3 + 63) // 64), dtype=torch.int64, device="cuda")
+    del lv
+    out = {}
+
+    def step():
+        lv, _ = dev.refine(shape, oc["root"], oc["min_depth"], oc["max_depth"], hd, leaves=lv_buf)
+        m = lv.shape[0]
+        dev.sample_spans(shape, lv, res, with_cross=False, out=dists[:m])
+        out["m"] = dev.mesh(shape, lv, dists[:m], res, "fd6", cross=cross[:m])
+
+    step()
+    torch.cuda.synchronize()
+    reps = 3
+    t = time.perf_counter()
+    for _ in range(reps):
+        step()
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t) / reps
+    voff, ioff, _, _ = out.pop("m")
+    nverts, nquads = int(voff[-1].item()), int(ioff[-1].item()) // 6
+    evals = center_evals + nleaves_contract * (res + 1) ** 3 + 7 * nverts
+    lv = lv_buf[:nleaves_contract]
+    dv = dists[:nleaves_contract]
+    span_ms = event_ms(lambda: dev.sample_spans(shape, lv, res, with_cross=False, out=dv), 3)
+    span_flop = de_flop(*count_work(ctx, lambda: dev.sample_spans(shape, lv, res, with_cross=False, out=dv)))
+    _, _, vdef, _ = dev.mesh(shape, lv, dv, res, "deferred", cross=cross[:nleaves_contract])
+    vdef = vdef.clone()
+    vwork = torch.empty_like(vdef)
+    taps_ms = event_ms(lambda: dev.mesh_vertices(shape, vwork, "fd6"), 3, before=lambda: vwork.copy_(vdef))
+    vwork.copy_(vdef)
+    taps_flop = de_flop(*count_work(ctx, lambda: dev.mesh_vertices(shape, vwork, "fd6"))) + FLOP_VERTEX * nverts
+    del vdef, vwork
+    # the contraction mode's corner distances on the parity run's leaves
+    dp = parity_dists.ravel()
+    dc = dists[:nleaves]
+    dev.sample_spans(shape, leaves_buf, res, with_cross=False, out=dc)
+    dc = dc.ravel()
+    fin = torch.isfinite(dc) & torch.isfinite(dp)
+    rel = (dc[fin].double() - dp[fin].double()).abs() / dp[fin].double().abs().clamp_min(1e-30)
+    tail = float((rel > 1e-5).double().mean().item())
+    flips = float((torch.signbit(dc) != torch.signbit(dp)).double().mean().item())
+    del rel, fin
+    ctx.close()
+    return {"workload": "the octree step in CTH_MODE_CONTRACT (FMA contraction, IEEE div/sqrt)",
+            "value": evals / (ms / 1e3), "ms_per_step": ms, "vertices": nverts, "quads": nquads,
+            "vertices_parity": nverts_parity, "quads_parity": nquads_parity,
+            "taps": {"kernel_ms": taps_ms, "algorithmic_flop_per_launch": taps_flop,
+                     "frac": taps_flop / (taps_ms / 1e3) / PEAK_F32},
+            "spans": {"kernel_ms": span_ms, "algorithmic_flop_per_launch": span_flop,
+                      "frac": span_flop / (span_ms / 1e3) / PEAK_F32},
+            "surface_leaves": nleaves_contract, "surface_leaves_parity": nleaves,
+            "corners_rel_gt_1e-5_vs_parity": tail, "corner_sign_differs_from_parity": flips}
 
 
 def _host_cpu() -> str:
@@ -408,8 +492,6 @@ def bench_point_query(ctx, args):
     """Latency of one host-buffer query, the reference's per-frame calls:
     the fly camera's min_distance_from (src/control/fly.rs:91-92) and the 25
     focus rays of get_focii (src/mesh/mod.rs:205-243)."""
-    import numpy as np
-
     import cantucci_amd as ca
     from cantucci_amd import device as dev  # noqa: F401  (torch loaded first)
 
@@ -422,19 +504,11 @@ def bench_point_query(ctx, args):
     for _ in range(reps):
         shape.min_distance_from(p)
     one = (time.perf_counter() - t) / reps
-    rng = np.random.default_rng(0)
-    dirs = rng.normal(size=(25, 3)) * 0.1 + np.array([1.0, 0.0, 0.0])
-    dirs = (dirs / np.linalg.norm(dirs, axis=1, keepdims=True)).astype(np.float32)
-    origin = np.array([-3.0, 0.0, 0.0], np.float32)
-    import ctypes as C
-    from cantucci_amd._lib import check, lib
-    depth = np.empty(25, np.float32)
-    steps = np.empty(25, np.uint32)
-    pos = np.empty((25, 3), np.float32)
+    from cantucci_amd.mesh import get_focii
 
+    # the app's projection (src/app.rs:98): fov 1 rad, near 4e-5, far 10
     def focii():
-        check(lib.cth_sphere_trace(ctx.handle, C.byref(shape.desc), origin.ctypes.data, 0, dirs.ctypes.data, 25, 100,
-                                   1e-6, 0.0, float("inf"), depth.ctypes.data, steps.ctypes.data, pos.ctypes.data))
+        get_focii(shape, (-3.0, 0.0, 0.0), (1.0, 0.0, 0.0), 1.0, 16 / 9, 0.00004, 10.0, 5)
 
     for _ in range(20):
         focii()
@@ -442,7 +516,8 @@ def bench_point_query(ctx, args):
     for _ in range(500):
         focii()
     trace = (time.perf_counter() - t) / 500
-    return {"workload": "classic(6, 2.5): one min_distance_from; 25 focus rays (eps 1e-6, 100 steps)",
+    return {"workload": "classic(6, 2.5): one min_distance_from; get_focii's 25 focus rays built and marched "
+                        "on the device (eps 1e-6, 100 steps)",
             "us_min_distance_from": one * 1e6, "us_focus_rays": trace * 1e6}
 
 

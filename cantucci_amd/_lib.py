@@ -23,6 +23,7 @@ CTH_SHAPE_SPHERE = 2
 
 CTH_MODE_PARITY = 0
 CTH_MODE_FAST = 1
+CTH_MODE_CONTRACT = 2
 
 CTH_NORMALS_FD6 = 0
 CTH_NORMALS_TANGENT = 1

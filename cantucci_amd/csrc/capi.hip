@@ -137,7 +137,8 @@ void cth_ctx_destroy(cth_ctx* c) {
 
 int cth_ctx_set_mode(cth_ctx* c, int mode) {
     if (!c) return fail(CTH_E_INVALID, "null context");
-    if (mode != CTH_MODE_PARITY && mode != CTH_MODE_FAST) return fail(CTH_E_INVALID, "unknown mode");
+    if (mode != CTH_MODE_PARITY && mode != CTH_MODE_FAST && mode != CTH_MODE_CONTRACT)
+        return fail(CTH_E_INVALID, "unknown mode");
     c->mode = mode;
     return CTH_OK;
 }

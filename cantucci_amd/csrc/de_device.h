@@ -277,6 +277,71 @@ struct P8Exact {
 };
 
 // ---------------------------------------------------------------------------
+// P8 with FMA contraction (CTH_MODE_CONTRACT): the parity evaluator's
+// operations, except that the compiler may fuse a product and a following
+// sum into one FMA, as a contracting compiler would for the reference; the
+// division and the square roots stay correctly rounded and the guards are
+// the same.  Not bit-exact: each fused operation skips one rounding, and
+// the chaotic tail of non-escaping orbits moves.
+// ---------------------------------------------------------------------------
+struct P8Contract : P8Exact {
+    __device__ __forceinline__ float mag() {
+#pragma clang fp contract(fast)
+        x2 = x * x;
+        y2 = y * y;
+        w2 = x2 + y2;
+        z2 = z * z;
+        return sqrt_rn_wave(w2 + z2);
+    }
+    __device__ __forceinline__ void body(float r, const MandelUniform& u, bool live) {
+#pragma clang fp contract(fast)
+        float r2 = r * r;
+        float r4 = r2 * r2;
+        float r7 = (r * r2) * r4;
+        float ndr = ((r7 * 8.0f) * dr) + 1.0f;
+        float x4 = x2 * x2, x6 = x4 * x2, x8 = x4 * x4;
+        float y4 = y2 * y2, y6 = y4 * y2, y8 = y4 * y4;
+        float z4 = z2 * z2, z6 = z4 * z2, z8 = z4 * z4;
+        float w4 = w2 * w2, w6 = w2 * w4, w8 = w4 * w4;
+        float num = ((z8 - (28.0f * z6) * w2) + (70.0f * z4) * w4) - (28.0f * z2) * w6;
+        // The ranges of P8Exact::body: the guard depends on w2 only.
+        float q, sw2;
+        const bool unscaled = u.bailout <= 16.0f && wave_mask(!(w2 >= 0x1p-15f)) == 0;
+        if (unscaled) {
+            q = div_rn_unscaled(num, w8);
+            sw2 = sqrt_rn_unscaled(w2);
+        } else {
+            q = div_rn(num, w8);
+            sw2 = sqrt_rn_wave(w2);
+        }
+        float a = 1.0f + q;
+        float nx = a * ((((x8 - (28.0f * x6) * y2) + (70.0f * x4) * y4) - (28.0f * x2) * y6) - y8);
+        float ny = (((8.0f * a) * x) * y) * (((x6 - (7.0f * x4) * y2) + (7.0f * x2) * y4) - y6);
+        float nz = (((8.0f * z) * sw2) * (z2 - w2)) * ((z4 - (6.0f * z2) * w2) + w4);
+        if (!unscaled) {
+            const bool axis = on_z_axis(x, y);
+            if (wave_mask(axis)) {
+                if (live && axis) {
+                    float c = div_rn(z, r);
+                    if (__builtin_isinf(c)) pos_nan = true;
+                    float th = acosf(c);
+                    float r8 = r4 * r4;
+                    nx = 0.0f;
+                    ny = 0.0f;
+                    nz = r8 * cosf(th * 8.0f);
+                }
+            }
+        }
+        if (live) {
+            dr = ndr;
+            x = nx + px;
+            y = ny + py;
+            z = nz + pz;
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
 // Exact trig evaluator (reference `Mandelbulb<P != 8>`, and the real-power
 // extension with powf).  Transcendentals are ocml's.
 // ---------------------------------------------------------------------------
@@ -450,5 +515,16 @@ struct SphereEv {
     __device__ __forceinline__ void body(float, const MandelUniform&, bool) {}
     __device__ __forceinline__ float finish(float) const { return sphere_de(px, py, pz, c, radius); }
 };
+
+// Calls f(Ev{}) with the P8 evaluator of an arithmetic mode.
+template <class F>
+__host__ __forceinline__ void with_p8(int mode, F&& f) {
+    if (mode == CTH_MODE_FAST)
+        f(P8Fast{});
+    else if (mode == CTH_MODE_CONTRACT)
+        f(P8Contract{});
+    else
+        f(P8Exact{});
+}
 
 }  // namespace cth

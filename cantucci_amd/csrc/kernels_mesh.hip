@@ -1082,18 +1082,16 @@ int mesh_vertices(Ctx& c, const cth_shape& s, int normals, uint64_t nverts, floa
         uint32_t blocks = 0;
         WorkQueue q = make_queue(c, (uint32_t)nverts, c.shape_mesh, blocks, st);
         dim3 grid(blocks), block(c.shape_mesh.block);
-        if (q.stats) {
-            if (family == FAM_P8 && c.mode == CTH_MODE_FAST)
-                hipLaunchKernelGGL((k_mesh_taps<P8Fast, true>), grid, block, 0, st, verts, u, q);
-            else if (family == FAM_P8)
-                hipLaunchKernelGGL((k_mesh_taps<P8Exact, true>), grid, block, 0, st, verts, u, q);
-            else
-                hipLaunchKernelGGL((k_mesh_taps<TrigExact, true>), grid, block, 0, st, verts, u, q);
-        } else if (family == FAM_P8) {
-            if (c.mode == CTH_MODE_FAST)
-                hipLaunchKernelGGL((k_mesh_taps<P8Fast>), grid, block, 0, st, verts, u, q);
-            else
-                hipLaunchKernelGGL((k_mesh_taps<P8Exact>), grid, block, 0, st, verts, u, q);
+        if (family == FAM_P8) {
+            with_p8(c.mode, [&](auto ev) {
+                using Ev = decltype(ev);
+                if (q.stats)
+                    hipLaunchKernelGGL((k_mesh_taps<Ev, true>), grid, block, 0, st, verts, u, q);
+                else
+                    hipLaunchKernelGGL((k_mesh_taps<Ev>), grid, block, 0, st, verts, u, q);
+            });
+        } else if (q.stats) {
+            hipLaunchKernelGGL((k_mesh_taps<TrigExact, true>), grid, block, 0, st, verts, u, q);
         } else {
             hipLaunchKernelGGL((k_mesh_taps<TrigExact>), grid, block, 0, st, verts, u, q);
         }

@@ -306,18 +306,16 @@ static int run_trace(Ctx& c, const cth_shape& s, const Rays& rays, uint32_t tota
     dim3 grid(blocks), block(c.shape_trace.block);
     if (family == FAM_SPHERE) {
         hipLaunchKernelGGL((k_trace<SphereEv, Rays>), grid, block, 0, st, rays, u, tp, out, q);
-    } else if (q.stats) {
-        if (family == FAM_P8 && c.mode == CTH_MODE_FAST)
-            hipLaunchKernelGGL((k_trace<P8Fast, Rays, true>), grid, block, 0, st, rays, u, tp, out, q);
-        else if (family == FAM_P8)
-            hipLaunchKernelGGL((k_trace<P8Exact, Rays, true>), grid, block, 0, st, rays, u, tp, out, q);
-        else
-            hipLaunchKernelGGL((k_trace<TrigExact, Rays, true>), grid, block, 0, st, rays, u, tp, out, q);
     } else if (family == FAM_P8) {
-        if (c.mode == CTH_MODE_FAST)
-            hipLaunchKernelGGL((k_trace<P8Fast, Rays>), grid, block, 0, st, rays, u, tp, out, q);
-        else
-            hipLaunchKernelGGL((k_trace<P8Exact, Rays>), grid, block, 0, st, rays, u, tp, out, q);
+        with_p8(c.mode, [&](auto ev) {
+            using Ev = decltype(ev);
+            if (q.stats)
+                hipLaunchKernelGGL((k_trace<Ev, Rays, true>), grid, block, 0, st, rays, u, tp, out, q);
+            else
+                hipLaunchKernelGGL((k_trace<Ev, Rays>), grid, block, 0, st, rays, u, tp, out, q);
+        });
+    } else if (q.stats) {
+        hipLaunchKernelGGL((k_trace<TrigExact, Rays, true>), grid, block, 0, st, rays, u, tp, out, q);
     } else {
         hipLaunchKernelGGL((k_trace<TrigExact, Rays>), grid, block, 0, st, rays, u, tp, out, q);
     }

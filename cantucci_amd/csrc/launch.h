@@ -61,21 +61,16 @@ inline int run_refill(Ctx& c, int family, const MandelUniform& u, const Src& src
     uint32_t blocks = 0;
     WorkQueue q = make_queue(c, total, ls, blocks, st);
     dim3 grid(blocks), block(ls.block);
-    bool fast = c.mode == CTH_MODE_FAST;
-    if (q.stats) {
-        if (family == FAM_P8) {
-            if (fast)
-                hipLaunchKernelGGL((k_refill<P8Fast, Src, Sink, true>), grid, block, 0, st, src, u, sink, q);
+    if (family == FAM_P8) {
+        with_p8(c.mode, [&](auto ev) {
+            using Ev = decltype(ev);
+            if (q.stats)
+                hipLaunchKernelGGL((k_refill<Ev, Src, Sink, true>), grid, block, 0, st, src, u, sink, q);
             else
-                hipLaunchKernelGGL((k_refill<P8Exact, Src, Sink, true>), grid, block, 0, st, src, u, sink, q);
-        } else {
-            hipLaunchKernelGGL((k_refill<TrigExact, Src, Sink, true>), grid, block, 0, st, src, u, sink, q);
-        }
-    } else if (family == FAM_P8) {
-        if (fast)
-            hipLaunchKernelGGL((k_refill<P8Fast, Src, Sink>), grid, block, 0, st, src, u, sink, q);
-        else
-            hipLaunchKernelGGL((k_refill<P8Exact, Src, Sink>), grid, block, 0, st, src, u, sink, q);
+                hipLaunchKernelGGL((k_refill<Ev, Src, Sink>), grid, block, 0, st, src, u, sink, q);
+        });
+    } else if (q.stats) {
+        hipLaunchKernelGGL((k_refill<TrigExact, Src, Sink, true>), grid, block, 0, st, src, u, sink, q);
     } else {
         hipLaunchKernelGGL((k_refill<TrigExact, Src, Sink>), grid, block, 0, st, src, u, sink, q);
     }

@@ -27,7 +27,7 @@ class Context:
         self.set_mode(mode)
 
     def set_mode(self, mode: str) -> None:
-        m = {"parity": _lib.CTH_MODE_PARITY, "fast": _lib.CTH_MODE_FAST}[mode]
+        m = {"parity": _lib.CTH_MODE_PARITY, "fast": _lib.CTH_MODE_FAST, "contract": _lib.CTH_MODE_CONTRACT}[mode]
         check(lib.cth_ctx_set_mode(self.handle, m))
         self.mode = mode
 

@@ -65,11 +65,15 @@ enum {
     CTH_SHAPE_SPHERE = 2
 };
 
-/* Arithmetic modes of a context.
- *   CTH_MODE_PARITY  the reference's f32 operation sequence (no FMA, IEEE div/sqrt):
- *                    loop state bit-identical, final ln within an ulp.
- *   CTH_MODE_FAST    FMAs, hardware reciprocal/sqrt; not bit-exact. */
-enum { CTH_MODE_PARITY = 0, CTH_MODE_FAST = 1 };
+/* Arithmetic modes of a context (they select the P8 evaluator; the trig map
+ * and the sphere always run their parity arithmetic).
+ *   CTH_MODE_PARITY    the reference's f32 operation sequence (no FMA, IEEE
+ *                      div/sqrt): loop state bit-identical, final ln within an ulp.
+ *   CTH_MODE_FAST      FMAs, hardware reciprocal/sqrt; not bit-exact.
+ *   CTH_MODE_CONTRACT  the parity sequence with FMA contraction (a product
+ *                      and a following sum fused), IEEE div/sqrt kept: what a
+ *                      contracting compiler would make of the reference. */
+enum { CTH_MODE_PARITY = 0, CTH_MODE_FAST = 1, CTH_MODE_CONTRACT = 2 };
 
 /* Shape descriptor (the reference's type-level parameters made runtime):
  * `Mandelbulb::new(max_iters, bailout)` (src/shape/mandelbulb.rs:19-26, asserts
