@@ -179,7 +179,9 @@ def main() -> None:
             "algorithmic_flop_per_launch": taps_flop, "loop_bodies": taps_w[0], "bailed_evals": taps_w[1],
             "evals": taps_w[2], "vertices": nverts,
             "achieved": taps_flop / (taps_ms / 1e3) / 1e12, "frac": taps_flop / (taps_ms / 1e3) / PEAK_F32,
-            "traffic": traffic["taps"], "algorithmic_bytes": nverts * (24 + 16),
+            "traffic": traffic["taps"],
+            # position and half-steps read, the whole 28-byte record written
+            "algorithmic_bytes": nverts * (24 + 28),
         },
         "spans": {
             "kernel": f"k_refill<P8{suffix}, SrcSpans, SinkDist>", "kernel_ms": span_ms,

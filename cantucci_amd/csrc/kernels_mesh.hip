@@ -808,12 +808,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
             }
             hold_m = 0;
             for (;;) {
-                // every lane here is idle; a lane with all 7 taps writes its vertex
+                // every lane here is idle; a lane with all 7 taps writes its
+                // vertex: the whole 28-byte record, position unchanged.  A
+                // 16-byte write of normal and distance alone dirties every
+                // 32-byte sector the record touches anyway (measured: 28 B
+                // per vertex written back either way), so the record is
+                // written whole and in one place.
                 if (has_v && t == 7u && !__builtin_amdgcn_inverse_ballot_w64(live_m)) {
                     float tv[7];
 #pragma unroll
                     for (int j = 0; j < 7; ++j) tv[j] = slot[256 * (6 + j)];
-                    put_fd_normal(verts + 7 * (size_t)vid, tv);
+                    float* rec = verts + 7 * (size_t)vid;
+                    rec[0] = slot[0];
+                    rec[1] = slot[256];
+                    rec[2] = slot[512];
+                    put_fd_normal(rec, tv);
                     has_v = false;
                 }
                 const uint64_t need = wave_mask(!has_v) & ~live_m;
