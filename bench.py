@@ -33,15 +33,20 @@ PEAK_HBM = 8.0e12     # MI355X HBM3E (B/s)
 FLOP_PER_BODY = 75    # P8 loop body: magnitude 5, compare 1, dr 7, rotation 59, add 3
 FLOP_BAILED = 6       # final magnitude + compare of a point that bails
 FLOP_FINAL = 4        # ln, *r, *0.5, /dr
+FLOP_DEAD = 62        # rotation + add of the last body of an evaluation at the iteration cap:
+                      # the rotated point is never read, so it is not counted
 FLOP_VERTEX = 19      # 6-tap normal per vertex: 6 stencil adds, 3 differences, normalize 10
 FLOP_STEP = 11        # sphere-tracing step: pos += dir d (6), t += d, eps + eps_rel t (2), 2 compares
 FLOP_RAY = 33         # pinhole ray: u 6, v 5, direction 12, normalize 10
 
 
 def de_flop(bodies: int, bails: int, evals: int) -> float:
-    """Algorithmic FLOP of P8 distance evaluations: 75 per loop body, 6 per
+    """Algorithmic FLOP of P8 distance evaluations: 75 per loop body, of which
+    only 13 (magnitude, compare, dr) in the last body of an evaluation that
+    reaches the iteration cap (every evaluation that does not bail), 6 per
     evaluation that bails (its last magnitude and compare), 4 per final formula."""
-    return FLOP_PER_BODY * bodies + FLOP_BAILED * bails + FLOP_FINAL * evals
+    capped = evals - bails
+    return FLOP_PER_BODY * bodies - FLOP_DEAD * capped + FLOP_BAILED * bails + FLOP_FINAL * evals
 
 
 def count_work(ctx, fn):
@@ -591,8 +596,7 @@ def bench_trace(ctx, args):
     """32 seeded orbit poses x 1920x1080 primary rays, 256 steps, hit d < 1e-4 t,
     escape t > 10: eager launches, and the same launch captured in a HIP graph
     and replayed.  The trace kernel's algorithmic FLOP come from the work
-    counters (75 per loop body, 6 per bailout, 4 per final formula, 11 per
-    step, 33 per ray)."""
+    counters (de_flop, 11 per step, 33 per ray)."""
     import torch
 
     import cantucci_amd as ca

@@ -185,6 +185,10 @@ struct MandelUniform {
 // ---------------------------------------------------------------------------
 struct P8Exact {
     static constexpr int kWavesPerSimd = 8;  // fits 64 VGPRs in the refill kernel
+    // The evaluator has last_dr(): the work loops end a point that reaches
+    // the iteration cap after max_iters - 1 bodies and apply the last body's
+    // dr update alone (see last_dr).
+    static constexpr bool kLastDr = true;
     float px, py, pz, x, y, z, dr;
     float x2, y2, w2, z2;  // squares of the last magnitude, reused by rotate
     bool pos_nan;          // the orbit's first NaN came from acosf(+-inf)
@@ -270,6 +274,16 @@ struct P8Exact {
             z = nz + pz;
         }
     }
+    // The dr update of a body, alone: dr = r.powi(7) * 8.0 * dr + 1.0 (:73).
+    // When a point's max_iters-th body is its last (it did not bail before),
+    // only this part of it is live: the rotated point it computes is never
+    // read (:65-78, the loop ends and r is not recomputed).
+    __device__ __forceinline__ void last_dr(float r) {
+        float r2 = r * r;
+        float r4 = r2 * r2;
+        float r7 = (r * r2) * r4;
+        dr = ((r7 * 8.0f) * dr) + 1.0f;
+    }
     // 0.5 * (r.ln() * r) / dr   (:77-78)
     __device__ __forceinline__ float finish(float r) const {
         return finish_exact(r, dr, pos_nan);
@@ -285,6 +299,13 @@ struct P8Exact {
 // the chaotic tail of non-escaping orbits moves.
 // ---------------------------------------------------------------------------
 struct P8Contract : P8Exact {
+    __device__ __forceinline__ void last_dr(float r) {
+#pragma clang fp contract(fast)
+        float r2 = r * r;
+        float r4 = r2 * r2;
+        float r7 = (r * r2) * r4;
+        dr = ((r7 * 8.0f) * dr) + 1.0f;
+    }
     __device__ __forceinline__ float mag() {
 #pragma clang fp contract(fast)
         x2 = x * x;
@@ -347,6 +368,7 @@ struct P8Contract : P8Exact {
 // ---------------------------------------------------------------------------
 struct TrigExact {
     static constexpr int kWavesPerSimd = 1;  // no constraint (64 VGPRs would spill)
+    static constexpr bool kLastDr = true;
     float px, py, pz, x, y, z, dr, P;
     int Pi;       // integer power of the powi ladder (when `ladder`)
     bool ladder;  // the powi ladder (integral power) rather than powf
@@ -425,6 +447,11 @@ struct TrigExact {
         eval(r, t);
         commit(t, live);
     }
+    // The last body's dr update alone (P8Exact::last_dr).
+    __device__ __forceinline__ void last_dr(float r) {
+        const float rp = ladder ? powi_ladder(r, Pi - 1) : powf(r, P - 1.0f);
+        dr = ((rp * P) * dr) + 1.0f;
+    }
     __device__ __forceinline__ float finish(float r) const {
         return finish_exact(r, dr, pos_nan);
     }
@@ -435,6 +462,7 @@ struct TrigExact {
 // ---------------------------------------------------------------------------
 struct P8Fast {
     static constexpr int kWavesPerSimd = 8;
+    static constexpr bool kLastDr = true;
     float px, py, pz, x, y, z, dr;
     float x2, y2, w2, z2;
 
@@ -481,6 +509,12 @@ struct P8Fast {
             z = nz + pz;
         }
     }
+    __device__ __forceinline__ void last_dr(float r) {
+        float r2 = r * r;
+        float r4 = r2 * r2;
+        float r7 = (r * r2) * r4;
+        dr = __builtin_fmaf(r7 * 8.0f, dr, 1.0f);
+    }
     __device__ __forceinline__ float finish(float r) const {
         return canon_nan((0.5f * (__logf(r) * r)) * __builtin_amdgcn_rcpf(dr));
     }
@@ -499,6 +533,7 @@ __device__ __forceinline__ float sphere_de(float px, float py, float pz, const f
 // exact distance.  The centre and radius travel in the uniforms' spare fields.
 struct SphereEv {
     static constexpr int kWavesPerSimd = 8;
+    static constexpr bool kLastDr = false;
     float px, py, pz, dr;
     float c[3], radius;
     __device__ __forceinline__ void start(float ax, float ay, float az, float, const MandelUniform& u) {

@@ -792,6 +792,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
     // not all evaluated) is only read at the batched refill.
     const uint64_t wave = wave_mask(true);
     uint64_t live_m = 0, hold_m = 0;
+    uint64_t last_m = 0;  // held taps whose last body's dr update is due (Ev::kLastDr, as in refill_loop)
+    const bool cap0 = Ev::kLastDr && u.max_iters == 1u;  // at the cap before any full body
     bool has_v = false;
     uint32_t vid = 0, t = 0;
     uint32_t tick = 0, t0 = 0;  // wave's body count; tick at which the lane's tap started
@@ -803,10 +805,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
     for (;;) {
         if ((uint32_t)__popcll(wave & ~live_m) >= q.refill_min) {
             if (__builtin_amdgcn_inverse_ballot_w64(hold_m)) {
+                if constexpr (Ev::kLastDr) {
+                    if (__builtin_amdgcn_inverse_ballot_w64(last_m)) ev.last_dr(r);
+                }
                 slot[256 * (6 + t)] = ev.finish(r);
                 ++t;
             }
             hold_m = 0;
+            last_m = 0;
             for (;;) {
                 // every lane here is idle; a lane with all 7 taps writes its
                 // vertex: the whole 28-byte record, position unchanged.  A
@@ -856,6 +862,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
                     if (exhausted || (wave_mask(!has_v) & ~live_m) == 0) break;
                     continue;
                 }
+                bool now = false;  // the tap needs no full body
                 if (__builtin_amdgcn_inverse_ballot_w64(start_m)) {
                     // tap t: p, or p + unit_a * (+-delta_a) (src/mesh/buffer.rs:259-264)
                     float c[3] = {slot[0], slot[256], slot[512]};
@@ -869,30 +876,53 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
                     ev.start(c[0], c[1], c[2], 0.0f, u);
                     t0 = tick;
                     r = ev.mag();
-                    if (r > u.bailout) {
+                    now = r > u.bailout || cap0;
+                    if (now) {
+                        if constexpr (Ev::kLastDr) {
+                            if (!(r > u.bailout)) ev.last_dr(r);
+                        }
                         slot[256 * (6 + t)] = ev.finish(r);
                         ++t;
                     }
                 }
                 const uint64_t zero_m = wave_mask(r > u.bailout) & start_m;
-                live_m |= start_m & ~zero_m;
+                const uint64_t now_m = wave_mask(now) & start_m;
+                live_m |= start_m & ~now_m;
                 tally.eval(start_m);
                 tally.bail(zero_m);
+                tally.body(now_m & ~zero_m);
             }
             if (live_m == 0) break;
         }
         const bool live = __builtin_amdgcn_inverse_ballot_w64(live_m);
-        const uint64_t cap_m = wave_mask(t0 == tick + 1u - u.max_iters) & live_m;  // the max_iters-th body
-        ev.body(r, u, live);
-        ++tick;
-        tally.body(live_m);
-        const float rn = ev.mag();
-        const uint64_t bail_m = wave_mask(rn > u.bailout) & live_m & ~cap_m;
-        tally.bail(bail_m);
-        // at the cap r stays the magnitude before the last body (src/shape/mandelbulb.rs:65-78)
-        if (__builtin_amdgcn_inverse_ballot_w64(live_m & ~cap_m)) r = rn;
-        live_m &= ~(cap_m | bail_m);
-        hold_m |= cap_m | bail_m;
+        if constexpr (Ev::kLastDr) {
+            // as in refill_loop: a tap at the cap leaves after max_iters - 1
+            // bodies and its last body's dr update is applied when it is held
+            ev.body(r, u, live);
+            ++tick;
+            tally.body(live_m);
+            const float rn = ev.mag();
+            const uint64_t bail_m = wave_mask(rn > u.bailout) & live_m;
+            tally.bail(bail_m);
+            if (live) r = rn;
+            const uint64_t cap_m = wave_mask(t0 == tick + 1u - u.max_iters) & live_m & ~bail_m;
+            tally.body(cap_m);
+            last_m |= cap_m;
+            live_m &= ~(cap_m | bail_m);
+            hold_m |= cap_m | bail_m;
+        } else {
+            const uint64_t cap_m = wave_mask(t0 == tick + 1u - u.max_iters) & live_m;  // the max_iters-th body
+            ev.body(r, u, live);
+            ++tick;
+            tally.body(live_m);
+            const float rn = ev.mag();
+            const uint64_t bail_m = wave_mask(rn > u.bailout) & live_m & ~cap_m;
+            tally.bail(bail_m);
+            // at the cap r stays the magnitude before the last body (src/shape/mandelbulb.rs:65-78)
+            if (__builtin_amdgcn_inverse_ballot_w64(live_m & ~cap_m)) r = rn;
+            live_m &= ~(cap_m | bail_m);
+            hold_m |= cap_m | bail_m;
+        }
     }
     tally.flush(q.stats);
     if (q.dynamic) wave_retire(q);

@@ -31,6 +31,7 @@ __device__ __forceinline__ float grad_final(float r, float dr, float rq, float d
 // ---------------------------------------------------------------------------
 struct P8Tangent : P8Exact {
     static constexpr int kWavesPerSimd = 1;  // the tangent state does not fit 64 VGPRs
+    static constexpr bool kLastDr = false;   // the work loops run its last body in full
     float T[3][3];    // dz/dp
     float tdr[3];     // d dr / dp
     float rq_prev[3]; // dr/dp of the magnitude entering the last body
@@ -138,6 +139,7 @@ template <int NQ>
 struct TrigTangentN : TrigExact {
     static_assert(NQ == 3 || NQ == 4, "3 point columns, optionally the power");
     static constexpr int kWavesPerSimd = 1;
+    static constexpr bool kLastDr = false;  // the work loops run its last body in full
     float T[3][NQ];
     float tdr[NQ];
     float rq_prev[NQ];

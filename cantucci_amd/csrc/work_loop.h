@@ -124,6 +124,12 @@ __device__ __forceinline__ void wave_retire(const WorkQueue& q) {
 // (__builtin_amdgcn_inverse_ballot_w64).  A point's body count is the wave's
 // tick counter (scalar) minus the tick at which the lane took the point, so
 // the iteration cap costs one compare per tick and no per-lane counter.
+//
+// Evaluators with Ev::kLastDr end a point that reaches the iteration cap
+// after max_iters - 1 bodies: of its max_iters-th body only the dr update is
+// live (Ev::last_dr), and it is applied when the point is written (`last_m`
+// marks those lanes).  r is then the magnitude that body would have read,
+// which is the magnitude the reference ends with.
 template <class Ev, bool kCount, class Src, class Sink>
 __device__ __forceinline__ void refill_loop(const Src& src, const MandelUniform& u, const Sink& sink,
                                             const WorkQueue& q) {
@@ -131,6 +137,7 @@ __device__ __forceinline__ void refill_loop(const Src& src, const MandelUniform&
     const uint64_t wave = wave_mask(true);  // the wave's lanes
     uint64_t live_m = 0;  // lanes whose point needs another loop body
     uint64_t hold_m = 0;  // lanes holding a finished point not yet written
+    uint64_t last_m = 0;  // held points whose last body's dr update is due (kLastDr)
     uint32_t tick = 0;    // loop bodies run by the wave
     uint32_t idx = 0;
     uint32_t t0 = 0;      // tick at which the lane took its point
@@ -143,8 +150,14 @@ __device__ __forceinline__ void refill_loop(const Src& src, const MandelUniform&
 
     for (;;) {
         if ((uint32_t)__popcll(wave & ~live_m) >= q.refill_min) {
-            if (__builtin_amdgcn_inverse_ballot_w64(hold_m)) sink.put(idx, ev.finish(r), kh, r, ev.dr, ev);
+            if (__builtin_amdgcn_inverse_ballot_w64(hold_m)) {
+                if constexpr (Ev::kLastDr) {
+                    if (__builtin_amdgcn_inverse_ballot_w64(last_m)) ev.last_dr(r);
+                }
+                sink.put(idx, ev.finish(r), kh, r, ev.dr, ev);
+            }
             hold_m = 0;
+            last_m = 0;
             // Hand new points to idle lanes.  A point that bails before its
             // first body (|p| > bailout) is written on the spot and the lane
             // refilled again.
@@ -176,27 +189,59 @@ __device__ __forceinline__ void refill_loop(const Src& src, const MandelUniform&
                 const uint64_t bailed = wave_mask(r > u.bailout) & took;
                 tally.eval(took);
                 tally.bail(bailed);
-                if (__builtin_amdgcn_inverse_ballot_w64(bailed)) sink.put(idx, ev.finish(r), 0u, r, ev.dr, ev);
-                live_m |= took & ~bailed;
+                // With kLastDr and max_iters == 1 a point that does not bail
+                // at once is at its cap before any full body.
+                const uint64_t capped = (Ev::kLastDr && u.max_iters == 1u) ? took & ~bailed : 0;
+                tally.body(capped);
+                if (__builtin_amdgcn_inverse_ballot_w64(bailed | capped)) {
+                    const bool c = __builtin_amdgcn_inverse_ballot_w64(capped);
+                    if constexpr (Ev::kLastDr) {
+                        if (c) ev.last_dr(r);
+                    }
+                    sink.put(idx, ev.finish(r), c ? 1u : 0u, r, ev.dr, ev);
+                }
+                live_m |= took & ~(bailed | capped);
                 c_next += (nneed < avail) ? nneed : avail;
             }
             if (live_m == 0) break;
         }
         const bool live = __builtin_amdgcn_inverse_ballot_w64(live_m);
-        // Lanes making their max_iters-th body in this tick (max_iters >= 1).
-        const uint64_t cap_m = wave_mask(t0 == tick + 1u - u.max_iters) & live_m;
-        ev.body(r, u, live);
-        ++tick;
-        tally.body(live_m);
-        const float rn = ev.mag();
-        const uint64_t bail_m = wave_mask(rn > u.bailout) & live_m & ~cap_m;
-        tally.bail(bail_m);
-        // At the iteration cap, r stays the magnitude computed before the last
-        // body (the reference's off-by-one, src/shape/mandelbulb.rs:65-78).
-        if (__builtin_amdgcn_inverse_ballot_w64(live_m & ~cap_m)) r = rn;
-        const uint64_t done_m = cap_m | bail_m;
-        if (done_m != 0 && sink.wants_k()) {
-            if (__builtin_amdgcn_inverse_ballot_w64(done_m)) kh = tick - t0;
+        uint64_t done_m;
+        if constexpr (Ev::kLastDr) {
+            ev.body(r, u, live);
+            ++tick;
+            tally.body(live_m);
+            const float rn = ev.mag();
+            const uint64_t bail_m = wave_mask(rn > u.bailout) & live_m;
+            tally.bail(bail_m);
+            if (live) r = rn;
+            // Lanes that have now run max_iters - 1 bodies without bailing:
+            // their max_iters-th body is the dr update alone (max_iters >= 2
+            // here; max_iters == 1 ends at the start).
+            const uint64_t cap_m = wave_mask(t0 == tick + 1u - u.max_iters) & live_m & ~bail_m;
+            tally.body(cap_m);
+            done_m = cap_m | bail_m;
+            last_m |= cap_m;
+            if (done_m != 0 && sink.wants_k()) {
+                if (__builtin_amdgcn_inverse_ballot_w64(done_m))
+                    kh = tick - t0 + (__builtin_amdgcn_inverse_ballot_w64(cap_m) ? 1u : 0u);
+            }
+        } else {
+            // Lanes making their max_iters-th body in this tick (max_iters >= 1).
+            const uint64_t cap_m = wave_mask(t0 == tick + 1u - u.max_iters) & live_m;
+            ev.body(r, u, live);
+            ++tick;
+            tally.body(live_m);
+            const float rn = ev.mag();
+            const uint64_t bail_m = wave_mask(rn > u.bailout) & live_m & ~cap_m;
+            tally.bail(bail_m);
+            // At the iteration cap, r stays the magnitude computed before the
+            // last body (the reference's off-by-one, src/shape/mandelbulb.rs:65-78).
+            if (__builtin_amdgcn_inverse_ballot_w64(live_m & ~cap_m)) r = rn;
+            done_m = cap_m | bail_m;
+            if (done_m != 0 && sink.wants_k()) {
+                if (__builtin_amdgcn_inverse_ballot_w64(done_m)) kh = tick - t0;
+            }
         }
         live_m &= ~done_m;
         hold_m |= done_m;
