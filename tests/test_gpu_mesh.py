@@ -5,7 +5,9 @@ Bar: with the same corner distances, vertex counts, offsets and every index
 vertex positions (the reference's f32 operation order).  The 6-tap normals and
 distance_from_surface are distance evaluations again: bit-identical for the
 sphere, and for the Mandelbulb equal except where the final ln differs by an
-ulp between glibc's logf and the correctly rounded device log.
+ulp between glibc's logf and the correctly rounded device log; with the
+oracle's loop state and a correctly rounded ln in the reference's formulas
+they are bit-identical.
 """
 import numpy as np
 import pytest
@@ -158,8 +160,13 @@ def test_octree_leaf_mesh_parity(ctx, torch_cuda):
     fin = np.isfinite(v[:, 6]) & np.isfinite(v_o[:, 6])
     rel = np.abs(v[fin, 6].astype(np.float64) - v_o[fin, 6]) / np.maximum(np.abs(v_o[fin, 6].astype(np.float64)), 1e-30)
     assert rel.max() < 1e-5
-    # 6-tap normals: the same up to those ulps of the tap distances
+    # 6-tap normals: the same up to those ulps of the tap distances, and the
+    # reference's formulas exactly once the taps' ln is correctly rounded
     assert nrm_same.mean() > 0.95
+    n_cr = _cr_fd_normals(O.mandelbulb(15, 2.0), spans, voff, v, 8)
+    n_same = (same_bits(v[:, 3:6], n_cr) | (np.isnan(v[:, 3:6]) & np.isnan(n_cr))).all(1)
+    print(f"normal = the formulas with a correctly rounded ln on {n_same.mean():.6f}")
+    assert n_same.mean() > 0.9995
     fin = np.isfinite(v[:, 3:6]).all(1) & np.isfinite(v_o[:, 3:6]).all(1)
     ang = angle(v[fin, 3:6], v_o[fin, 3:6])
     print(f"6-tap normal angle vs oracle: p99 {np.percentile(ang, 99):.2e} p99.9 {np.percentile(ang, 99.9):.2e}")
@@ -170,6 +177,70 @@ def test_octree_leaf_mesh_parity(ctx, torch_cuda):
     assert (np.signbit(d.reshape(d_o.shape)) == np.signbit(d_o)).all()
     voff_e, ioff_e, _, ix_e = O.mesh_batch(O.mandelbulb(15, 2.0), spans, 8)
     assert_topology(voff, ioff, ix, voff_e, ioff_e, ix_e)
+
+
+def _cr_distances(shape_o, pts):
+    """The reference's distances at pts from the oracle's loop state, with the
+    final ln correctly rounded (as the kernels compute it) instead of glibc's
+    logf; also the iteration counts."""
+    _, it, r, dr = O.de_points(shape_o, pts)
+    ln_cr = np.log(r.astype(np.float64)).astype(F32)
+    with np.errstate(all="ignore"):
+        return (F32(0.5) * (ln_cr * r)) / dr, it
+
+
+def _cr_fd_normals(shape_o, spans, voff, verts, res):
+    """6-tap normals of the vertices (src/mesh/buffer.rs:256-272: half-step
+    0.7 (e' - s') / res of the adjusted span, taps p +- unit_a * delta_a,
+    g = DE(+) - DE(-), normalize as g * (1 / |g|)) from _cr_distances."""
+    sp = np.asarray(spans, F32).reshape(-1, 6)
+    counts = np.diff(voff.astype(np.int64))
+    s0 = np.repeat(sp[:, :3], counts, axis=0)
+    e0 = np.repeat(sp[:, 3:], counts, axis=0)
+    fres = F32(res)
+    of = (e0 - s0) / fres
+    s2, e2 = s0 + -of, e0 + of
+    delta = (F32(0.7) * (e2 - s2)) / fres
+    p = verts[:, :3]
+    g = np.empty_like(p)
+    for a in range(3):
+        pp, pm = p.copy(), p.copy()
+        pp[:, a] = p[:, a] + delta[:, a]
+        pm[:, a] = p[:, a] + -delta[:, a]
+        g[:, a] = _cr_distances(shape_o, pp)[0] - _cr_distances(shape_o, pm)[0]
+    with np.errstate(all="ignore"):
+        mag = np.sqrt((g[:, 0] * g[:, 0] + g[:, 1] * g[:, 1]) + g[:, 2] * g[:, 2])
+        return g * (F32(1.0) / mag)[:, None]
+
+
+@pytest.mark.parametrize("mi", [1, 2, 3])
+def test_mesh_taps_short_caps(ctx, torch_cuda, mi):
+    """The tap kernel at small iteration caps, where most taps end at the cap
+    (with max_iters 1 every tap that does not bail at its start does, before
+    any full body): vertex positions bit-exact, and each distance and 6-tap
+    normal equal to the reference's formulas on the oracle's loop state with
+    the final ln correctly rounded."""
+    import cantucci_amd as ca
+    torch = torch_cuda
+    spans = octree_leaves(300, seed=11)
+    shape = ca.Mandelbulb.classic(mi, 2.0, ctx=ctx)
+    sh = O.mandelbulb(mi, 2.0)
+    d, voff, ioff, v, ix = run_gpu_mesh(torch, shape, spans, 8)
+    voff_o, ioff_o, v_o, ix_o = O.mesh_batch(sh, spans, 8, dists=d)
+    assert_topology(voff, ioff, ix, voff_o, ioff_o, ix_o)
+    nv = int(voff[-1])
+    assert nv > 100
+    assert same_bits(v[:, :3], v_o[:, :3]).all()
+    d_cr, it_o = _cr_distances(sh, v[:, :3])
+    capped = (it_o == mi).mean()
+    cr_same = same_bits(v[:, 6], d_cr) | (np.isnan(v[:, 6]) & np.isnan(d_cr))
+    n_cr = _cr_fd_normals(sh, spans, voff, v, 8)
+    n_same = (same_bits(v[:, 3:6], n_cr) | (np.isnan(v[:, 3:6]) & np.isnan(n_cr))).all(1)
+    print(f"max_iters {mi}: {nv} vertices, {capped:.3f} at the cap; distance and normal = the formulas with a "
+          f"correctly rounded ln on {cr_same.mean():.6f} / {n_same.mean():.6f}")
+    assert capped > 0.3
+    assert cr_same.mean() > 0.9999
+    assert n_same.mean() > 0.9995
 
 
 def test_tangent_normals_on_mesh(ctx, torch_cuda):
