@@ -83,6 +83,18 @@ def test_octree_workload_full_size(ctx, torch_cuda):
         assert b - a == int(voff_o[k + 1] - voff_o[k]) and e - c == int(ioff_o[k + 1] - ioff_o[k])
         assert (ix[c:e] == ix_o[int(ioff_o[k]):int(ioff_o[k + 1])]).all()
         assert (v[a:b, :3].view(np.uint32) == v_o[int(voff_o[k]):int(voff_o[k + 1]), :3].view(np.uint32)).all()
+    # their distances and 6-tap normals: the reference's formulas on the
+    # oracle's loop state, with the final ln correctly rounded
+    from test_gpu_mesh import _cr_distances, _cr_fd_normals
+    rows = np.concatenate([np.arange(int(vo[leaf]), int(vo[leaf + 1])) for leaf in sel])
+    vs = v[rows]
+    d_cr, _ = _cr_distances(O.mandelbulb(15, 2.0), vs[:, :3])
+    n_cr = _cr_fd_normals(O.mandelbulb(15, 2.0), lv[sel], voff_o, vs, res)
+    d_same = (vs[:, 6].view(np.uint32) == d_cr.view(np.uint32)) | (np.isnan(vs[:, 6]) & np.isnan(d_cr))
+    n_same = ((vs[:, 3:6].view(np.uint32) == n_cr.view(np.uint32)) | (np.isnan(vs[:, 3:6]) & np.isnan(n_cr))).all(1)
+    print(f"{len(rows)} sampled vertices: distance / normal = the formulas with a correctly rounded ln on "
+          f"{d_same.mean():.6f} / {n_same.mean():.6f}")
+    assert d_same.mean() > 0.9999 and n_same.mean() > 0.9995
 
 
 def finite_normals(verts):
