@@ -47,6 +47,18 @@ def assert_de_parity(d_gpu, d_ref, max_ulp=4, min_bitexact=0.99):
     assert rel.max() < 1e-5
 
 
+def assert_cr_formula(d_gpu, r_o, dr_o):
+    """The distances equal the reference's final formula 0.5 * (ln(r) * r) / dr
+    (src/shape/mandelbulb.rs:77-78) on the oracle's loop state with the ln
+    correctly rounded, as the device computes it: bit for bit, NaNs by
+    pattern, up to the ~2^-18 chance per value of a double-rounding miss."""
+    ln_cr = np.log(r_o.astype(np.float64)).astype(F32)
+    with np.errstate(all="ignore"):
+        d_cr = (F32(0.5) * (ln_cr * r_o)) / dr_o
+    same = same_bits(d_gpu, d_cr) | (np.isnan(d_gpu) & np.isnan(d_cr))
+    assert same.mean() > 0.9999, f"{same.mean():.6f} equal to the formula with a correctly rounded ln"
+
+
 def assert_same_or_both_nan(a, b):
     """Bit-identical, except that the sign of an intermediate NaN is not part
     of the reference's observable behaviour (only the final distance's is)."""
@@ -102,6 +114,7 @@ def test_grid64_p8_parity(ctx, torch_cuda, divisor):
     assert (it2.cpu().numpy().astype(np.uint32) == it_o).all()
     assert_same_or_both_nan(r.cpu().numpy(), r_o)
     assert_same_or_both_nan(dr.cpu().numpy(), dr_o)
+    assert_cr_formula(d.cpu().numpy(), r_o, dr_o)
 
 
 def edge_points():
@@ -135,6 +148,7 @@ def test_points_p8_state_bitexact(ctx, torch_cuda, params):
     nan = np.isnan(dr_o)
     assert (np.isnan(drg) == nan).all() and same_bits(drg[~nan], dr_o[~nan]).all()
     assert_de_parity(d.cpu().numpy(), d_o)
+    assert_cr_formula(d.cpu().numpy(), r_o, dr_o)
 
 
 @pytest.mark.parametrize("P", [2, 3, 5, 9])
