@@ -52,8 +52,8 @@ def assert_cr_formula(d_gpu, r_o, dr_o):
     (src/shape/mandelbulb.rs:77-78) on the oracle's loop state with the ln
     correctly rounded, as the device computes it: bit for bit, NaNs by
     pattern, up to the ~2^-18 chance per value of a double-rounding miss."""
-    ln_cr = np.log(r_o.astype(np.float64)).astype(F32)
     with np.errstate(all="ignore"):
+        ln_cr = np.log(r_o.astype(np.float64)).astype(F32)
         d_cr = (F32(0.5) * (ln_cr * r_o)) / dr_o
     same = same_bits(d_gpu, d_cr) | (np.isnan(d_gpu) & np.isnan(d_cr))
     assert same.mean() > 0.9999, f"{same.mean():.6f} equal to the formula with a correctly rounded ln"

@@ -184,8 +184,8 @@ def _cr_distances(shape_o, pts):
     final ln correctly rounded (as the kernels compute it) instead of glibc's
     logf; also the iteration counts."""
     _, it, r, dr = O.de_points(shape_o, pts)
-    ln_cr = np.log(r.astype(np.float64)).astype(F32)
     with np.errstate(all="ignore"):
+        ln_cr = np.log(r.astype(np.float64)).astype(F32)
         return (F32(0.5) * (ln_cr * r)) / dr, it
 
 
