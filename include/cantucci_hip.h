@@ -400,8 +400,11 @@ size_t cth_jobs_offset(const cth_jobset* jobs, uint32_t i);
 
 /* Run every job in one persistent launch: dists (total floats), normals
  * (nullable, 3 x total; forward-mode gradient, unit length), done_ticks
- * (device, one per job: wall-clock stamp of the job's last write-back),
- * start_tick (device, one: earliest wave start).  Capture-safe. */
+ * (device, one per job, zeroed by the call: wall-clock stamp of the job's
+ * last write-back of a point that ran at least one loop body; points beyond
+ * the bailout radius are written when they are taken and not stamped, so a
+ * job made only of such points keeps 0), start_tick (device, one: earliest
+ * wave start).  Capture-safe. */
 int cth_jobs_run_dev(cth_ctx* ctx, const cth_jobset* jobs, float* dists, float* normals, uint64_t* done_ticks,
                      uint64_t* start_tick, void* stream);
 
