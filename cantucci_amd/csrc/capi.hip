@@ -334,9 +334,10 @@ int cth_get_focii_dev(cth_ctx* c, const cth_shape* s, const cth_ref_camera* cams
                       uint32_t* steps, float* dirs, void* stream) {
     if (!c) return fail(CTH_E_INVALID, "null context");
     if (int rc = check_shape(s)) return rc;
-    if (focus_points == 0 || focus_points > 255) return fail(CTH_E_INVALID, "focus_points must be in [1, 255]");
+    if (focus_points > 255) return fail(CTH_E_INVALID, "focus_points must be in [0, 255]");
     if (max_steps == 0) return fail(CTH_E_INVALID, "max_steps must be >= 1");
-    if (ncams && (!cams || !focii || !depth || !steps)) return fail(CTH_E_INVALID, "null buffer");
+    if (focus_points == 0 || ncams == 0) return CTH_OK;
+    if (!cams || !focii || !depth || !steps) return fail(CTH_E_INVALID, "null buffer");
     return launch_trace_focii(*c, *s, cams, ncams, focus_points, max_steps, eps, focii, depth, steps, dirs,
                               pick_stream(c, stream));
 }
@@ -345,9 +346,11 @@ int cth_get_focii(cth_ctx* c, const cth_shape* s, const cth_ref_camera* cam, uin
                   size_t* nhits) {
     if (!c) return fail(CTH_E_INVALID, "null context");
     if (int rc = check_shape(s)) return rc;
-    if (!cam || !focii || !nhits) return fail(CTH_E_INVALID, "null buffer");
-    if (focus_points == 0 || focus_points > 255) return fail(CTH_E_INVALID, "focus_points must be in [1, 255]");
+    if (!cam || !nhits) return fail(CTH_E_INVALID, "null buffer");
+    if (focus_points > 255) return fail(CTH_E_INVALID, "focus_points must be in [0, 255]");
     *nhits = 0;
+    if (focus_points == 0) return CTH_OK;  // the reference's iter::square(0) is empty
+    if (!focii) return fail(CTH_E_INVALID, "null buffer");
     const size_t n = size_t(focus_points) * focus_points;
     void* b[4];
     int rc = staging(*c, WS_HOST_IN, {sizeof(cth_ref_camera), n * 12, n * 4, n * 4}, b);

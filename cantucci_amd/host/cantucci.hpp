@@ -1,8 +1,10 @@
-// C++17 host mirror of cantucci's `Shape` trait and mesh-buffer entry point,
-// over the C ABI of libcantucci_hip.so (include/cantucci_hip.h).
+// C++17 host mirror of cantucci's `Shape` trait, mesh-buffer entry point and
+// focus-point marcher, over the C ABI of libcantucci_hip.so
+// (include/cantucci_hip.h).
 //
 // The reference's interface (src/shape/mod.rs:24-105, src/shape/mandelbulb.rs:12-33,
-// src/shape/sphere.rs:6-52, src/mesh/buffer.rs:23-42) is kept method for method,
+// src/shape/sphere.rs:6-52, src/mesh/buffer.rs:23-42, src/camera.rs:14-34,108-144,
+// src/mesh/mod.rs:205-243) is kept method for method,
 // with the same names, argument meaning and results.  Where the reference
 // panics (an `assert!`, or `unwrap` on a missing upper bound) this layer throws
 // cantucci::Error carrying the library status; nothing aborts.  Every query
@@ -12,6 +14,7 @@
 // Header-only; link with -lcantucci_hip.
 #pragma once
 
+#include <cmath>
 #include <cstddef>
 #include <cstdint>
 #include <optional>
@@ -254,5 +257,68 @@ struct MeshBuffer {
         return {std::move(buf), Timings{(uint32_t)nv, (uint32_t)(ni / 6)}};
     }
 };
+
+// cgmath `Vector3<f32>`.
+struct Vector3 {
+    float x, y, z;
+};
+
+// `Projection` (src/camera.rs:108-127): vertical field of view (radians),
+// aspect ratio, near and far plane.
+struct Projection {
+    float fov;
+    float aspect_ratio;
+    float near_plane;
+    float far_plane;
+
+    // Projection::new(fov, near..far, (w, h))  (:135-144): aspect w / h;
+    // throws where the reference asserts a non-zero screen dimension.
+    static Projection make(float fov, float near_plane, float far_plane, uint32_t w, uint32_t h) {
+        if (w == 0 || h == 0) throw Error(CTH_E_INVALID, "screen dimension must not be zero");
+        return Projection{fov, (float)w / (float)h, near_plane, far_plane};
+    }
+};
+
+// `Camera` (src/camera.rs:14-34): position, unit view direction and
+// projection; the up vector is always +z.
+class Camera {
+  public:
+    // Camera::new(pos, dir, proj)  (:26-34): throws where the reference
+    // asserts dir != 0, and keeps dir.normalize() = dir * (1 / |dir|).
+    Camera(Point3 position, Vector3 direction, Projection projection) : position(position), projection(projection) {
+        if (direction.x == 0.0f && direction.y == 0.0f && direction.z == 0.0f)
+            throw Error(CTH_E_INVALID, "camera direction must not be zero");
+        const float m = std::sqrt((direction.x * direction.x + direction.y * direction.y) + direction.z * direction.z);
+        const float inv = 1.0f / m;
+        this->direction = {direction.x * inv, direction.y * inv, direction.z * inv};
+    }
+    Point3 position;
+    Vector3 direction;
+    Projection projection;
+};
+
+// `ShapeMesh::get_focii(&self, camera, focus_points)` (src/mesh/mod.rs:205-243)
+// for the mesh's shape: the points where the focus_points^2 rays through
+// the near-plane cell centres hit it (100 steps, a hit when the distance is
+// below 1e-6), in ray order; the rays are built and marched on the device.
+inline std::vector<Point3> get_focii(const Shape& shape, const Camera& camera, uint8_t focus_points) {
+    cth_ref_camera c;
+    c.position[0] = camera.position.x;
+    c.position[1] = camera.position.y;
+    c.position[2] = camera.position.z;
+    c.direction[0] = camera.direction.x;
+    c.direction[1] = camera.direction.y;
+    c.direction[2] = camera.direction.z;
+    c.fov = camera.projection.fov;
+    c.aspect = camera.projection.aspect_ratio;
+    c.near_plane = camera.projection.near_plane;
+    c.far_plane = camera.projection.far_plane;
+    std::vector<Point3> out((size_t)focus_points * focus_points);
+    size_t hits = 0;
+    check(cth_get_focii(shape.context().get(), &shape.desc(), &c, focus_points,
+                        out.empty() ? nullptr : reinterpret_cast<float*>(out.data()), &hits));
+    out.resize(hits);
+    return out;
+}
 
 }  // namespace cantucci

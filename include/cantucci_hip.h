@@ -297,7 +297,8 @@ int cth_sphere_trace(cth_ctx* ctx, const cth_shape* shape, const float* origins,
  * receives the position after the last step (get_focii returns it for hits),
  * depth the distance marched on a hit and +inf on a miss (get_focii drops
  * those rays), steps the evaluations; dirs (nullable, 3 floats per ray) the
- * ray directions.  focus_points in [1, 255] (the reference's u8).
+ * ray directions.  focus_points in [0, 255] (the reference's u8; 0 gives
+ * no rays, as the reference's get_focii returns no points).
  * The view matrix is a rotation plus a translation, so its inverse is taken
  * as the matrix with columns s, u, -f and position; cgmath inverts the
  * general 4x4 matrix by cofactors, which can differ in the entries' last
@@ -311,7 +312,8 @@ int cth_get_focii_dev(cth_ctx* ctx, const cth_shape* shape, const cth_ref_camera
 /* ShapeMesh::get_focii for one camera (host buffers, synchronous), with the
  * reference's constants (eps 1e-6, 100 steps): the hit positions in ray
  * order, as the reference's filter_map collects them.  focii must hold
- * focus_points^2 points (3 floats each); *nhits receives their number. */
+ * focus_points^2 points (3 floats each; may be null when focus_points is
+ * 0); *nhits receives their number. */
 int cth_get_focii(cth_ctx* ctx, const cth_shape* shape, const cth_ref_camera* cam, uint32_t focus_points,
                   float* focii, size_t* nhits);
 

@@ -1,7 +1,8 @@
 // Parity of the C++ host mirror (cantucci_amd/host/cantucci.hpp) against the
 // oracle, through the trait methods a caller of the reference uses:
 // Shape::{min,max,bounded}_distance_from and their batch forms, contains,
-// bounding_box, Mandelbulb::new/classic asserts, and MeshBuffer::generate_for_box.
+// bounding_box, Mandelbulb::new/classic asserts, MeshBuffer::generate_for_box,
+// and ShapeMesh::get_focii with Camera::new / Projection::new.
 // Needs a gfx950 device.  Prints one line per check; exits non-zero on the
 // first failure.
 #include <cmath>
@@ -20,6 +21,10 @@ void orc_de_points_f32(const cth_shape* shape, const float* pts, const float* po
                        uint32_t* iters, float* r_out, float* dr_out, int threads);
 void orc_surface_nets(const cth_shape* shape, const float* span, uint32_t res, float* dists_out, float* verts_out,
                       size_t verts_cap, uint32_t* idx_out, size_t idx_cap, size_t* nverts, size_t* nidx);
+void orc_focus_dirs(const float* cam, uint32_t fp, float* dirs);
+void orc_sphere_trace(const cth_shape* shape, const float* origins, const float* dirs, size_t n, int origin_stride,
+                      uint32_t max_steps, float eps_abs, float eps_rel, float t_max, float* depth, uint32_t* steps,
+                      float* hit_pos, int threads);
 }
 
 namespace {
@@ -189,6 +194,34 @@ int run() {
         Span flat{{0.0f, 0.0f, 0.0f}, {0.0f, 1.0f, 1.0f}};
         expect(throws_status(CTH_E_INVALID, [&] { MeshBuffer::generate_for_box(flat, m, 8); }),
                "generate_for_box asserts start < end");
+    }
+
+    // ShapeMesh::get_focii with the app's shape and near plane (src/app.rs:98,105)
+    {
+        auto m = Mandelbulb<8>::classic(ctx, 6, 2.5f);
+        Camera cam({-2.2f, 0.4f, 0.3f}, {1.0f, -0.15f, -0.1f}, Projection::make(1.0f, 0.00004f, 10.0f, 1600, 900));
+        auto f = get_focii(m, cam, 5);
+        const float c10[10] = {cam.position.x,  cam.position.y,  cam.position.z,
+                               cam.direction.x, cam.direction.y, cam.direction.z,
+                               cam.projection.fov, cam.projection.aspect_ratio,
+                               cam.projection.near_plane, cam.projection.far_plane};
+        std::vector<float> dirs(75), depth(25), pos(75);
+        std::vector<uint32_t> steps(25);
+        orc_focus_dirs(c10, 5, dirs.data());
+        orc_sphere_trace(&m.desc(), c10, dirs.data(), 25, 0, 100, 1e-6f, 0.0f, INFINITY, depth.data(), steps.data(),
+                         pos.data(), 1);
+        std::vector<Point3> want;
+        for (int i = 0; i < 25; ++i)
+            if (depth[i] != INFINITY) want.push_back({pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]});
+        bool same = f.size() == want.size() && !f.empty();
+        if (same) same = std::memcmp(f.data(), want.data(), 12 * f.size()) == 0;
+        std::printf("get_focii: %zu of 25 rays hit\n", f.size());
+        expect(same, "get_focii (Mandelbulb<8>::classic(6, 2.5)) bit-identical");
+        expect(get_focii(m, cam, 0).empty(), "get_focii with no focus points is empty");
+        expect(throws_status(CTH_E_INVALID, [&] { Camera(cam.position, {0.0f, 0.0f, 0.0f}, cam.projection); }),
+               "Camera::new asserts a non-zero direction");
+        expect(throws_status(CTH_E_INVALID, [&] { Projection::make(1.0f, 0.1f, 10.0f, 0, 900); }),
+               "Projection::new asserts a non-zero screen dimension");
     }
 
     std::printf("%s: %d failure(s)\n", failures ? "FAILED" : "PASSED", failures);

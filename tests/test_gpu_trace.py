@@ -215,8 +215,13 @@ def test_trace_rejects_bad_arguments(ctx, torch_cuda):
     d, s = dev.sphere_trace(shape, torch.zeros((1, 3), device="cuda"), torch.ones((0, 3), device="cuda"), 10)
     assert d.numel() == 0
     cams = torch.zeros((1, 10), device="cuda")
-    for fp, steps in ((0, 100), (256, 100), (5, 0)):
+    for fp, steps in ((256, 100), (5, 0)):
         with pytest.raises(ca.CthError):
             dev.get_focii(shape, cams, fp, max_steps=steps)
+    # no focus points: no rays and no hits, as the reference's iter::square(0)
+    f, d, s = dev.get_focii(shape, cams, 0)
+    assert f.shape == (1, 0, 3) and d.numel() == 0
+    from cantucci_amd.mesh import get_focii
+    assert get_focii(shape, (-3, 0, 0), (1, 0, 0), 1.0, 16 / 9, 0.00004, 10.0, focus_points=0).shape == (0, 3)
     with pytest.raises(ValueError):
         dev.get_focii(shape, torch.zeros((1, 9), device="cuda"))
