@@ -164,6 +164,8 @@ _SIGS = {
     "cth_mesh_emit_dev": (_I, [_P, _SH, _P, _P, _S, _U, _I, _P, _P, C.c_uint64, _P, _P, _P]),
     "cth_mesh_vertices_dev": (_I, [_P, _SH, _P, C.c_uint64, _I, _P]),
     "cth_mesh_span": (_I, [_P, _SH, C.POINTER(CthSpan), _U, _I, _P, _S, _P, _S, C.POINTER(_S), C.POINTER(_S)]),
+    "cth_mesh_spans": (_I, [_P, _SH, _P, _S, _U, _I, _P, _S, _P, _S, _P, _P, C.POINTER(_S), C.POINTER(_S)]),
+    "cth_focus_split": (_I, [_P, _P, _S, _P, _S, _P, _P]),
     "cth_focus_split_dev": (_I, [_P, _P, _S, _P, _P, _S, _P, _P, _P]),
     "cth_split_leaves_dev": (_I, [_P, _P, _S, _P, _P, _S, C.POINTER(_S), _P]),
 }

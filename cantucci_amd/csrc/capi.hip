@@ -667,6 +667,93 @@ int cth_mesh_span(cth_ctx* c, const cth_shape* s, const cth_span* span, uint32_t
     return rc ? rc : rs;
 }
 
+int cth_mesh_spans(cth_ctx* c, const cth_shape* s, const cth_span* spans, size_t nspans, uint32_t res, int normals,
+                   cth_vertex* vertices, size_t vert_cap, uint32_t* indices, size_t idx_cap, uint64_t* voff,
+                   uint64_t* ioff, size_t* nverts, size_t* nidx) {
+    if (!c) return fail(CTH_E_INVALID, "null context");
+    if (int rc = check_shape(s)) return rc;
+    if (!nverts || !nidx || (nspans && !spans)) return fail(CTH_E_INVALID, "null buffer");
+    if (res == 0 || (res & (res - 1)) != 0 || res > 64)
+        return fail(CTH_E_INVALID, "mesh resolution must be a power of two in [1, 64]");
+    if (normals != CTH_NORMALS_FD6 && normals != CTH_NORMALS_TANGENT && normals != CTH_NORMALS_DEFERRED)
+        return fail(CTH_E_INVALID, "unknown normals mode");
+    if (normals == CTH_NORMALS_TANGENT && s->kind == CTH_SHAPE_SPHERE)
+        return fail(CTH_E_UNSUPPORTED, "tangent normals need a Mandelbulb kind");
+    if (nspans >= (size_t(1) << 31)) return fail(CTH_E_INVALID, "more than 2^31 spans");
+    for (size_t i = 0; i < nspans; ++i)
+        for (int a = 0; a < 3; ++a)
+            if (!(spans[i].start[a] < spans[i].end[a]))
+                return fail(CTH_E_INVALID, "span start must be below end on every axis");
+    *nverts = 0;
+    *nidx = 0;
+    if (nspans == 0) {
+        if (voff) voff[0] = 0;
+        if (ioff) ioff[0] = 0;
+        return CTH_OK;
+    }
+    const size_t per = size_t(res + 1) * (res + 1) * (res + 1);
+    void* b[4];
+    int rc = staging(*c, WS_HOST_IN,
+                     {nspans * sizeof(cth_span), nspans * per * sizeof(float), (nspans + 1) * sizeof(uint64_t),
+                      (nspans + 1) * sizeof(uint64_t)},
+                     b);
+    if (rc) return rc;
+    uint64_t* voff_d = (uint64_t*)b[2];
+    uint64_t* ioff_d = (uint64_t*)b[3];
+    rc = check_hip(hipMemcpyAsync(b[0], spans, nspans * sizeof(cth_span), hipMemcpyHostToDevice, c->stream), "H2D");
+    if (!rc)
+        rc = cth_sample_spans_dev(c, s, (const float*)b[0], nspans, res, (float*)b[1], nullptr, nullptr, c->stream);
+    uint64_t nv = 0, ni = 0;
+    if (!rc) rc = cth_mesh_count_dev(c, (const float*)b[1], nspans, res, voff_d, ioff_d, nullptr, &nv, &ni, c->stream);
+    if (rc) return rc;
+    *nverts = (size_t)nv;
+    *nidx = (size_t)ni;
+    if (voff)
+        rc = check_hip(hipMemcpyAsync(voff, voff_d, (nspans + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream),
+                       "D2H");
+    if (!rc && ioff)
+        rc = check_hip(hipMemcpyAsync(ioff, ioff_d, (nspans + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream),
+                       "D2H");
+    if (!rc && (vertices || indices)) {
+        if ((vertices && vert_cap < nv) || (indices && idx_cap < ni) || (nv && !vertices) || (ni && !indices))
+            rc = fail(CTH_E_INVALID, "vertex or index buffer too small for the meshes");
+        void* o[2];
+        if (!rc) rc = staging(*c, WS_HOST_OUT, {nv * sizeof(cth_vertex), ni * sizeof(uint32_t) + 4}, o);
+        if (!rc)
+            rc = cth_mesh_emit_dev(c, s, (const float*)b[0], (const float*)b[1], nspans, res, normals, voff_d, ioff_d,
+                                   nv, (cth_vertex*)o[0], (uint32_t*)o[1], c->stream);
+        if (!rc && nv)
+            rc = check_hip(hipMemcpyAsync(vertices, o[0], nv * sizeof(cth_vertex), hipMemcpyDeviceToHost, c->stream),
+                           "D2H");
+        if (!rc && ni)
+            rc = check_hip(hipMemcpyAsync(indices, o[1], ni * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream),
+                           "D2H");
+    }
+    int rs = check_hip(hipStreamSynchronize(c->stream), "sync");
+    return rc ? rc : rs;
+}
+
+int cth_focus_split(cth_ctx* c, const cth_span* leaves, size_t n, const cth_point3* focii, size_t nfocii,
+                    const float eye[3], uint32_t* flags) {
+    if (!c) return fail(CTH_E_INVALID, "null context");
+    if (!eye || (n && (!leaves || !flags)) || (nfocii && !focii)) return fail(CTH_E_INVALID, "null buffer");
+    if (n == 0) return CTH_OK;
+    if (n >= (size_t(1) << 31) || nfocii >= (size_t(1) << 31)) return fail(CTH_E_INVALID, "batch larger than 2^31");
+    void* b[3];
+    int rc = staging(*c, WS_HOST_IN, {n * sizeof(cth_span), nfocii * sizeof(cth_point3), n * sizeof(uint32_t)}, b);
+    if (rc) return rc;
+    rc = check_hip(hipMemcpyAsync(b[0], leaves, n * sizeof(cth_span), hipMemcpyHostToDevice, c->stream), "H2D");
+    if (!rc && nfocii)
+        rc = check_hip(hipMemcpyAsync(b[1], focii, nfocii * sizeof(cth_point3), hipMemcpyHostToDevice, c->stream),
+                       "H2D");
+    if (!rc)
+        rc = cth_focus_split_dev(c, (const float*)b[0], n, (const float*)b[1], nullptr, nfocii, eye, (uint32_t*)b[2],
+                                 c->stream);
+    if (!rc) rc = check_hip(hipMemcpyAsync(flags, b[2], n * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream), "D2H");
+    int rs = check_hip(hipStreamSynchronize(c->stream), "sync");
+    return rc ? rc : rs;
+}
+
 // Host-pointer Shape-trait queries: copy in, run, copy out.
 enum Query { Q_MIN, Q_MAX, Q_BOUNDED, Q_CONTAINS };
 

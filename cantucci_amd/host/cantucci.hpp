@@ -321,4 +321,124 @@ inline std::vector<Point3> get_focii(const Shape& shape, const Camera& camera, u
     return out;
 }
 
+// `create_spans` (src/octree/mod.rs:315-329): the 8 children of a span,
+// child bit 2 = upper x half, bit 1 = y, bit 0 = z, around the centre
+// start + (end - start) / 2 (SpanExt::center, :21-23), so that the children
+// tile the parent without gaps.
+inline std::vector<Span> create_spans(const Span& p) {
+    const Point3 c{p.start.x + (p.end.x - p.start.x) / 2.0f, p.start.y + (p.end.y - p.start.y) / 2.0f,
+                   p.start.z + (p.end.z - p.start.z) / 2.0f};
+    std::vector<Span> out(8);
+    for (int i = 0; i < 8; ++i) {
+        const bool hx = i & 4, hy = i & 2, hz = i & 1;
+        out[i].start = {hx ? c.x : p.start.x, hy ? c.y : p.start.y, hz ? c.z : p.start.z};
+        out[i].end = {hx ? p.end.x : c.x, hy ? p.end.y : c.y, hz ? p.end.z : c.z};
+    }
+    return out;
+}
+
+// `ShapeMesh` (src/mesh/mod.rs:22-178), the mesh manager of the reference's
+// app, without its rendering half (the wgpu pipeline and per-leaf views):
+// the leaves of an octree over the shape's bounding box, each holding the
+// MeshBuffer of its span at resolution 64 once meshed.  The reference meshes
+// the empty leaves on a CPU thread pool and picks the buffers up in later
+// frames; update() meshes all of them in one batch on the device
+// (cth_mesh_spans) before it returns.  Leaves are kept in depth-first order,
+// a split leaf replaced by its 8 children in create_spans order.  The shape
+// must outlive the ShapeMesh.
+class ShapeMesh {
+  public:
+    static constexpr uint32_t RESOLUTION = 64;  // (:131)
+    static constexpr uint8_t FOCUS_POINTS = 5;  // (:86)
+
+    struct Leaf {
+        Span span;
+        std::optional<MeshBuffer> mesh;  // MeshStatus::Ready; empty until meshed
+    };
+
+    // ShapeMesh::new (:45-74): the bounding box, split twice: 64 leaves.
+    explicit ShapeMesh(const Shape& shape) : shape_(shape) {
+        for (const Span& child : create_spans(shape.bounding_box()))
+            for (const Span& leaf : create_spans(child)) leaves_.push_back(Leaf{leaf, std::nullopt});
+    }
+
+    // ShapeMesh::update (:82-170): every meshed leaf that holds one of the
+    // camera's focus points, with the camera nearer to that point than twice
+    // the leaf's width, is split (its mesh dropped); then every empty leaf is
+    // meshed.  The split test runs on the device (cth_focus_split).
+    void update(const Camera& camera) {
+        const std::vector<Point3> focii = get_focii(camera, FOCUS_POINTS);
+        std::vector<cth_span> spans(leaves_.size());
+        for (size_t i = 0; i < leaves_.size(); ++i) spans[i] = to_cth(leaves_[i].span);
+        std::vector<uint32_t> flags(leaves_.size(), 0u);
+        const float eye[3] = {camera.position.x, camera.position.y, camera.position.z};
+        check(cth_focus_split(shape_.context().get(), spans.data(), spans.size(),
+                              reinterpret_cast<const cth_point3*>(focii.data()), focii.size(), eye, flags.data()));
+        std::vector<Leaf> next;
+        for (size_t i = 0; i < leaves_.size(); ++i) {
+            if (flags[i] && leaves_[i].mesh) {  // only leaves showing a mesh are split (:94)
+                for (const Span& child : create_spans(leaves_[i].span)) next.push_back(Leaf{child, std::nullopt});
+            } else {
+                next.push_back(std::move(leaves_[i]));
+            }
+        }
+        leaves_ = std::move(next);
+        mesh_empty_leaves();
+    }
+
+    // ShapeMesh::get_focii (:205-243).
+    std::vector<Point3> get_focii(const Camera& camera, uint8_t focus_points) const {
+        return cantucci::get_focii(shape_, camera, focus_points);
+    }
+
+    const std::vector<Leaf>& leaves() const { return leaves_; }
+
+    // Totals of the meshes made so far (the reference's batch_timings counts, :38-41).
+    Timings batch_timings{0, 0};
+
+  private:
+    static cth_span to_cth(const Span& s) {
+        cth_span c;
+        c.start[0] = s.start.x;
+        c.start[1] = s.start.y;
+        c.start[2] = s.start.z;
+        c.end[0] = s.end.x;
+        c.end[1] = s.end.y;
+        c.end[2] = s.end.z;
+        return c;
+    }
+
+    void mesh_empty_leaves() {
+        std::vector<size_t> todo;
+        std::vector<cth_span> spans;
+        for (size_t i = 0; i < leaves_.size(); ++i)
+            if (!leaves_[i].mesh) {
+                todo.push_back(i);
+                spans.push_back(to_cth(leaves_[i].span));
+            }
+        if (todo.empty()) return;
+        cth_ctx* c = shape_.context().get();
+        size_t nv = 0, ni = 0;
+        check(cth_mesh_spans(c, &shape_.desc(), spans.data(), spans.size(), RESOLUTION, CTH_NORMALS_FD6, nullptr, 0,
+                             nullptr, 0, nullptr, nullptr, &nv, &ni));
+        std::vector<Vertex> verts(nv);
+        std::vector<uint32_t> idx(ni);
+        std::vector<uint64_t> voff(spans.size() + 1), ioff(spans.size() + 1);
+        check(cth_mesh_spans(c, &shape_.desc(), spans.data(), spans.size(), RESOLUTION, CTH_NORMALS_FD6,
+                             reinterpret_cast<cth_vertex*>(verts.data()), nv, idx.data(), ni, voff.data(), ioff.data(),
+                             &nv, &ni));
+        for (size_t k = 0; k < todo.size(); ++k) {
+            MeshBuffer buf;
+            buf.vertices.assign(verts.begin() + voff[k], verts.begin() + voff[k + 1]);
+            buf.indices.assign(idx.begin() + ioff[k], idx.begin() + ioff[k + 1]);
+            batch_timings.vertices += (uint32_t)buf.vertices.size();
+            batch_timings.faces += (uint32_t)(buf.indices.size() / 6);
+            leaves_[todo[k]].mesh = std::move(buf);
+        }
+    }
+
+    const Shape& shape_;
+    std::vector<Leaf> leaves_;
+};
+
 }  // namespace cantucci

@@ -493,6 +493,26 @@ int cth_mesh_span(cth_ctx* ctx, const cth_shape* shape, const cth_span* span, ui
                   cth_vertex* vertices, size_t vert_cap, uint32_t* indices, size_t idx_cap, size_t* nverts,
                   size_t* nidx);
 
+/* MeshBuffer::generate_for_box for a batch of spans (host buffers,
+ * synchronous): the mesh jobs that ShapeMesh::update hands its thread pool,
+ * one per empty leaf at resolution 64 (src/mesh/mod.rs:129-161), in one pass
+ * on the device.  The meshes are concatenated in span order, each span's
+ * indices local to its own vertices; voff / ioff (nspans + 1 entries each,
+ * nullable) receive the offsets of each span's vertices and indices, and
+ * *nverts / *nidx the totals.  vertices / indices may be NULL to query the
+ * sizes; when non-NULL they must hold vert_cap / idx_cap entries of at least
+ * those sizes (else CTH_E_INVALID).  Each span needs start < end on every
+ * axis; res is a power of two in [1, 64]. */
+int cth_mesh_spans(cth_ctx* ctx, const cth_shape* shape, const cth_span* spans, size_t nspans, uint32_t res,
+                   int normals, cth_vertex* vertices, size_t vert_cap, uint32_t* indices, size_t idx_cap,
+                   uint64_t* voff, uint64_t* ioff, size_t* nverts, size_t* nidx);
+
+/* Host-buffer variant of cth_focus_split_dev (synchronous): flags[i] = 1
+ * when one of the nfocii focus points lies in leaf i and the eye is nearer
+ * to it than twice the leaf's width (src/mesh/mod.rs:91-104). */
+int cth_focus_split(cth_ctx* ctx, const cth_span* leaves, size_t n, const cth_point3* focii, size_t nfocii,
+                    const float eye[3], uint32_t* flags);
+
 /* Host-buffer variant of cth_sample_spans_dev (synchronous). */
 int cth_sample_spans(cth_ctx* ctx, const cth_shape* shape, const cth_span* spans, size_t nspans, uint32_t res,
                      float* dists, uint64_t* cross_bits);

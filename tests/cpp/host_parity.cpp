@@ -2,7 +2,8 @@
 // oracle, through the trait methods a caller of the reference uses:
 // Shape::{min,max,bounded}_distance_from and their batch forms, contains,
 // bounding_box, Mandelbulb::new/classic asserts, MeshBuffer::generate_for_box,
-// and ShapeMesh::get_focii with Camera::new / Projection::new.
+// ShapeMesh::get_focii with Camera::new / Projection::new, and ShapeMesh::new /
+// update (split rule, create_spans, the leaves' meshes).
 // Needs a gfx950 device.  Prints one line per check; exits non-zero on the
 // first failure.
 #include <cmath>
@@ -22,6 +23,12 @@ void orc_de_points_f32(const cth_shape* shape, const float* pts, const float* po
 void orc_surface_nets(const cth_shape* shape, const float* span, uint32_t res, float* dists_out, float* verts_out,
                       size_t verts_cap, uint32_t* idx_out, size_t idx_cap, size_t* nverts, size_t* nidx);
 void orc_focus_dirs(const float* cam, uint32_t fp, float* dirs);
+void orc_focus_split(const float* leaves, size_t n, const float* focii, const float* depth, size_t nf,
+                     const float* eye, uint32_t* flags);
+void orc_create_spans(const float* span, float* out48);
+void orc_mesh_batch(const cth_shape* shape, const float* spans, size_t nspans, uint32_t res, const float* dists_in,
+                    int threads, uint64_t* voff, uint64_t* ioff, float* verts, size_t vcap, uint32_t* idx,
+                    size_t icap);
 void orc_sphere_trace(const cth_shape* shape, const float* origins, const float* dirs, size_t n, int origin_stride,
                       uint32_t max_steps, float eps_abs, float eps_rel, float t_max, float* depth, uint32_t* steps,
                       float* hit_pos, int threads);
@@ -222,6 +229,93 @@ int run() {
                "Camera::new asserts a non-zero direction");
         expect(throws_status(CTH_E_INVALID, [&] { Projection::make(1.0f, 0.1f, 10.0f, 0, 900); }),
                "Projection::new asserts a non-zero screen dimension");
+    }
+
+    // ShapeMesh::new and two updates, against the oracle's split rule,
+    // create_spans and surface nets
+    {
+        auto m = Mandelbulb<8>::classic(ctx, 6, 2.5f);
+        ShapeMesh sm(m);
+        expect(sm.leaves().size() == 64, "ShapeMesh::new splits the bounding box twice (64 leaves)");
+        Camera cam({-1.6f, 0.2f, 0.1f}, {1.0f, -0.1f, -0.05f}, Projection::make(1.0f, 0.00004f, 10.0f, 1600, 900));
+        // the oracle's leaves: the bounding box split twice in create_spans order
+        std::vector<float> want;
+        {
+            const float root[6] = {-1.2f, -1.2f, -1.2f, 1.2f, 1.2f, 1.2f};
+            float l1[48], l2[48];
+            orc_create_spans(root, l1);
+            for (int i = 0; i < 8; ++i) {
+                orc_create_spans(l1 + 6 * i, l2);
+                want.insert(want.end(), l2, l2 + 48);
+            }
+        }
+        auto same_spans = [&](const std::vector<float>& w) {
+            if (w.size() != 6 * sm.leaves().size()) return false;
+            for (size_t i = 0; i < sm.leaves().size(); ++i)
+                if (std::memcmp(&sm.leaves()[i].span, w.data() + 6 * i, 24) != 0) return false;
+            return true;
+        };
+        expect(same_spans(want), "ShapeMesh::new leaves equal the oracle's create_spans");
+        // meshes of leaves against the oracle's surface nets at resolution 64
+        // on the device's corner distances (the oracle's own differ by the
+        // final ln's last bits): topology and positions bit for bit
+        auto meshes_match = [&](const std::vector<size_t>& which) {
+            std::vector<float> dists(65 * 65 * 65);
+            for (size_t i : which) {
+                const auto& leaf = sm.leaves()[i];
+                if (!leaf.mesh) return false;
+                const float sp6[6] = {leaf.span.start.x, leaf.span.start.y, leaf.span.start.z,
+                                      leaf.span.end.x,   leaf.span.end.y,   leaf.span.end.z};
+                check(cth_sample_spans(ctx.get(), &m.desc(), reinterpret_cast<const cth_span*>(sp6), 1, 64,
+                                       dists.data(), nullptr));
+                uint64_t vo[2], io[2];
+                orc_mesh_batch(&m.desc(), sp6, 1, 64, dists.data(), 4, vo, io, nullptr, 0, nullptr, 0);
+                const size_t nv = vo[1], ni = io[1];
+                std::vector<float> v(7 * nv);
+                std::vector<uint32_t> ix(ni);
+                orc_mesh_batch(&m.desc(), sp6, 1, 64, dists.data(), 4, vo, io, v.data(), nv, ix.data(), ni);
+                if (leaf.mesh->vertices.size() != nv || leaf.mesh->indices != ix) return false;
+                for (size_t k = 0; k < nv; ++k)
+                    if (std::memcmp(leaf.mesh->vertices[k].position, v.data() + 7 * k, 12) != 0) return false;
+            }
+            return true;
+        };
+        sm.update(cam);  // nothing is meshed yet, so nothing splits; the 64 leaves are meshed
+        std::vector<size_t> all64(64);
+        for (size_t i = 0; i < 64; ++i) all64[i] = i;
+        expect(same_spans(want) && meshes_match(all64), "update 1: the 64 leaves meshed as the oracle meshes them");
+        // the oracle's split rule on the focus points
+        const float c10[10] = {cam.position.x,  cam.position.y,  cam.position.z,
+                               cam.direction.x, cam.direction.y, cam.direction.z,
+                               cam.projection.fov, cam.projection.aspect_ratio,
+                               cam.projection.near_plane, cam.projection.far_plane};
+        std::vector<float> dirs(75), depth(25), pos(75);
+        std::vector<uint32_t> steps(25), flags(64);
+        orc_focus_dirs(c10, 5, dirs.data());
+        orc_sphere_trace(&m.desc(), c10, dirs.data(), 25, 0, 100, 1e-6f, 0.0f, INFINITY, depth.data(), steps.data(),
+                         pos.data(), 1);
+        orc_focus_split(want.data(), 64, pos.data(), depth.data(), 25, c10, flags.data());
+        std::vector<float> want2;
+        std::vector<size_t> children;  // positions of the new leaves
+        size_t nsplit = 0;
+        for (size_t i = 0; i < 64; ++i) {
+            if (flags[i]) {
+                float ch[48];
+                orc_create_spans(want.data() + 6 * i, ch);
+                for (int k = 0; k < 8; ++k) children.push_back(want2.size() / 6 + k);
+                want2.insert(want2.end(), ch, ch + 48);
+                ++nsplit;
+            } else {
+                want2.insert(want2.end(), want.begin() + 6 * i, want.begin() + 6 * i + 6);
+            }
+        }
+        sm.update(cam);
+        std::printf("ShapeMesh update 2: %zu leaves split, %zu leaves, %u vertices meshed in all\n", nsplit,
+                    sm.leaves().size(), sm.batch_timings.vertices);
+        expect(nsplit > 0 && same_spans(want2), "update 2: the split leaves equal the oracle's split rule");
+        bool all_meshed = true;
+        for (const auto& leaf : sm.leaves()) all_meshed &= leaf.mesh.has_value();
+        expect(all_meshed && meshes_match(children), "update 2: the new leaves meshed as the oracle meshes them");
     }
 
     std::printf("%s: %d failure(s)\n", failures ? "FAILED" : "PASSED", failures);

@@ -126,3 +126,14 @@ def test_device_wrapper_argument_checks():
         dev._per_point(torch.zeros(4, dtype=torch.float64), 4, "grad_d")
     assert dev._per_point(None, 4, "powers") is None
     assert dev._per_point(torch.zeros((2, 2)), 4, "powers").numel() == 4
+
+
+def test_create_spans_mirror_matches_oracle():
+    """The Python mirror's create_spans (used by ShapeMesh) against the
+    oracle's, on dyadic and non-dyadic spans."""
+    import numpy as np
+    import oracle as O
+    from cantucci_amd.mesh import create_spans
+    for span in ([-1.2] * 3 + [1.2] * 3, [0.1, -0.7, 0.3, 0.35, -0.2, 0.9], [-1e-3, 2.0, -5.0, 1e-3, 2.5, 7.0]):
+        got = np.array([[*a, *b] for a, b in create_spans((span[:3], span[3:]))], np.float32)
+        assert (got.view(np.uint32) == O.create_spans(span).view(np.uint32)).all()

@@ -347,3 +347,51 @@ def test_mesh_edge_cases(ctx, torch_cuda):
         MeshBuffer.generate_for_box(((0, 0, 0), (1, 1, 1)), shape, 12)  # not a power of two
     with pytest.raises(ca.CthError):
         MeshBuffer.generate_for_box(((0, 0, 0), (1, 1, 1)), shape, 8, normals="tangent")  # sphere: closed form
+
+
+def test_shape_mesh_new_and_update(ctx, torch_cuda):
+    """ShapeMesh::new and two updates (src/mesh/mod.rs:45-170) against the
+    oracle: the 64 leaves of new(), the leaves the focus-point rule splits,
+    and every new leaf's mesh at resolution 64 on the device's corner
+    distances (topology and positions bit for bit)."""
+    import cantucci_amd as ca
+    from cantucci_amd import device as dev
+    from cantucci_amd.mesh import Camera, ShapeMesh
+    torch = torch_cuda
+    shape = ca.Mandelbulb.classic(6, 2.5, ctx=ctx)
+    sh = O.mandelbulb(6, 2.5)
+    sm = ShapeMesh(shape)
+    want = np.concatenate([O.create_spans(c) for c in O.create_spans([-1.2] * 3 + [1.2] * 3)])
+    spans_of = lambda: np.array([[*a, *b] for (a, b), _ in sm.leaves], F32)
+    assert same_bits(spans_of(), want).all() and len(sm.leaves) == 64
+
+    def check_meshes(which):
+        sp = spans_of()[which]
+        d, _ = dev.sample_spans(shape, gpu(torch, sp), 64, with_cross=False)
+        voff_o, ioff_o, v_o, ix_o = O.mesh_batch(sh, sp, 64, dists=d.cpu().numpy())
+        for k, i in enumerate(which):
+            buf = sm.leaves[i][1]
+            a, b = int(voff_o[k]), int(voff_o[k + 1])
+            assert buf is not None and len(buf.vertices) == b - a
+            assert (buf.indices == ix_o[int(ioff_o[k]):int(ioff_o[k + 1])]).all()
+            assert same_bits(buf.vertices["position"], v_o[a:b, :3]).all()
+
+    cam = Camera((-1.6, 0.2, 0.1), (1.0, -0.1, -0.05), 1.0, 16 / 9, 0.00004, 10.0)
+    sm.update(cam)  # nothing is meshed yet, so nothing splits; the 64 leaves are meshed
+    assert same_bits(spans_of(), want).all()
+    check_meshes(list(range(64)))
+    c10 = np.array([*cam.position, *cam.direction, cam.fov, cam.aspect, cam.near_plane, cam.far_plane], F32)
+    pos, depth, _, _ = O.get_focii(sh, c10, 5)
+    flags = O.focus_split(want, pos, c10[:3], depth)
+    want2, children = [], []
+    for i, f in enumerate(flags):
+        if f:
+            children.extend(range(len(want2), len(want2) + 8))
+            want2.extend(O.create_spans(want[i]))
+        else:
+            want2.append(want[i])
+    sm.update(cam)
+    print(f"{int(flags.sum())} leaves split, {len(sm.leaves)} leaves, {sm.batch_timings.vertices} vertices meshed")
+    assert flags.sum() > 0 and same_bits(spans_of(), np.array(want2, F32)).all()
+    assert all(buf is not None for _, buf in sm.leaves)
+    check_meshes(children)

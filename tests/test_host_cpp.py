@@ -1,4 +1,4 @@
-"""The C++ host mirror of the `Shape` trait and `MeshBuffer::generate_for_box`
+"""The C++ host mirror of the `Shape` trait, `MeshBuffer::generate_for_box` and `ShapeMesh`
 (cantucci_amd/host/cantucci.hpp), exercised by the compiled program
 tests/cpp/host_parity against the oracle."""
 import subprocess
