@@ -562,9 +562,23 @@ def bench_ref_default(ctx, args):
     torch.cuda.synchronize()
     gpu_ms = 1e3 * (time.perf_counter() - t) / reps
     evals = len(spans) * (res + 1) ** 3 + 7 * nverts
+    # The same jobs as the app runs them: ShapeMesh::new and its first
+    # update (the split test, then the 64 leaves meshed in one device batch
+    # and copied into per-leaf host MeshBuffers), through the host mirror.
+    from cantucci_amd.mesh import Camera, ShapeMesh
+    cam = Camera((-3.0, 0.0, 0.0), (1.0, 0.0, 0.0), 1.0, 16 / 9, 0.00004, 10.0)
+    ShapeMesh(shape).update(cam)
+    ts = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        sm = ShapeMesh(shape)
+        sm.update(cam)
+        ts.append(time.perf_counter() - t)
     return {"workload": "64 leaves of [-1.2,1.2]^3 at res 64, classic(6, 2.5), whole surface nets",
             "vertices": nverts, "quads": nidx // 6, "evals": evals, "gpu_ms": gpu_ms,
-            "gpu_evals_per_s": evals / gpu_ms * 1e3}
+            "gpu_evals_per_s": evals / gpu_ms * 1e3,
+            "shape_mesh_first_update_ms": 1e3 * sorted(ts)[len(ts) // 2],
+            "shape_mesh_vertices": sm.batch_timings.vertices}
 
 
 def _reference_default_spans():

@@ -344,8 +344,9 @@ inline std::vector<Span> create_spans(const Span& p) {
 // the empty leaves on a CPU thread pool and picks the buffers up in later
 // frames; update() meshes all of them in one batch on the device
 // (cth_mesh_spans) before it returns.  Leaves are kept in depth-first order,
-// a split leaf replaced by its 8 children in create_spans order.  The shape
-// must outlive the ShapeMesh.
+// a split leaf replaced by its 8 children in create_spans order.  The
+// ShapeMesh keeps its own copy of the shape's descriptor (the reference
+// holds an Arc<dyn Shape>); the shape's Context must outlive it.
 class ShapeMesh {
   public:
     static constexpr uint32_t RESOLUTION = 64;  // (:131)
@@ -437,7 +438,7 @@ class ShapeMesh {
         }
     }
 
-    const Shape& shape_;
+    Shape shape_;  // the descriptor and context of the caller's shape
     std::vector<Leaf> leaves_;
 };
 
