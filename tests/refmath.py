@@ -83,3 +83,24 @@ def de_f64(p, mapfn, P, max_iters, bailout):
             iters += active
         d = 0.5 * np.log(r) * r / dr
     return d, iters
+
+
+def cr_logf(x, band: float = 2.0 ** -18):
+    """ln of f32 inputs correctly rounded to f32, and a mask of the "hard"
+    inputs, whose ln lies within `band` f32 ulps of a rounding boundary (a
+    midpoint between two neighbouring f32 values).
+
+    The ln is taken in f64 (error below 2^-26 f32 ulps), so outside the mask
+    its rounding to f32 is the correctly rounded value.  Inside the mask any
+    implementation whose error bound is below `band` f32 ulps (the device's
+    f64 logarithm is one, with a relative error below 2^-43, i.e. 2^-19 ulps)
+    may round either way."""
+    x = np.asarray(x, np.float32)
+    with np.errstate(all="ignore"):
+        y = np.log(x.astype(np.float64))
+        f = y.astype(np.float32)
+        nb = np.nextafter(f, np.where(y > f, np.float32(np.inf), np.float32(-np.inf)))
+        mid = (f.astype(np.float64) + nb.astype(np.float64)) * 0.5
+        gap = np.abs(nb.astype(np.float64) - f.astype(np.float64))
+        hard = np.isfinite(y) & (y != f) & (np.abs(y - mid) < band * gap)
+    return f, hard

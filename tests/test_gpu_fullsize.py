@@ -4,8 +4,8 @@ that do not depend on size plus seeded samples against the oracle.
 * octree workload: level counts and leaves identical to the oracle's
   refinement; crossing words of span sampling and of the mesh count pass
   identical; per-leaf vertex count = popcount of the leaf's crossing words;
-  every index below its leaf's vertex count; 300 seeded leaves meshed by the
-  oracle from the same corner distances identical.
+  every index below its leaf's vertex count; and every output (all corner
+  distances, vertices and indices) against the oracle.
 * 32 x 1080p sphere tracing: one seeded row per pose traced by the oracle.
 * 2^22 shell points: tangent (and 6-tap FD) normals against the FP64 dual
   gradient on a seeded sample, also without the points whose stencil changes
@@ -16,6 +16,7 @@ import numpy as np
 import pytest
 
 import oracle as O
+import refmath as R
 
 pytestmark = pytest.mark.gpu
 F32 = np.float32
@@ -33,15 +34,26 @@ def popcount64(words):
     return c
 
 
-def test_octree_workload_full_size(ctx, torch_cuda):
+def _octree_workload(ctx):
+    """Refinement, span sampling and 6-tap meshing of the octree workload on
+    the device, as the benchmark runs them."""
     import cantucci_amd as ca
     from cantucci_amd import device as dev
     from cantucci_amd import workloads as wl
-    torch = torch_cuda
     oc = wl.OCTREE
     hd = wl.half_diagonals(3.0, oc["max_depth"])
     shape = ca.Mandelbulb.classic(oc["max_iters"], oc["bailout"], ctx=ctx)
     leaves, lc = dev.refine(shape, oc["root"], oc["min_depth"], oc["max_depth"], hd)
+    dists, cross = dev.sample_spans(shape, leaves, oc["res"])
+    return shape, hd, leaves, lc, dists, cross
+
+
+def test_octree_workload_full_size(ctx, torch_cuda):
+    from cantucci_amd import device as dev
+    from cantucci_amd import workloads as wl
+    torch = torch_cuda
+    oc = wl.OCTREE
+    shape, hd, leaves, lc, dists, cross = _octree_workload(ctx)
     torch.cuda.synchronize()
     leaves_o, lc_o = O.refine(O.mandelbulb(15, 2.0), oc["root"], oc["min_depth"], oc["max_depth"], hd)
     assert list(lc) == list(lc_o) and int(lc[-1]) == 1252864
@@ -49,7 +61,6 @@ def test_octree_workload_full_size(ctx, torch_cuda):
     assert lv.shape == (921604, 6) and (lv.view(np.uint32) == leaves_o.view(np.uint32)).all()
 
     res = oc["res"]
-    dists, cross = dev.sample_spans(shape, leaves, res)
     cross2 = torch.full_like(cross, -1)
     voff, ioff, verts, idx = dev.mesh(shape, leaves, dists, res, "fd6", cross=cross2)
     torch.cuda.synchronize()
@@ -66,35 +77,85 @@ def test_octree_workload_full_size(ctx, torch_cuda):
     print(f"{nleaf} leaves, {int(voff[-1])} vertices, {int(ioff[-1]) // 6} quads")
     assert finite_normals(verts) > 0.999
 
-    # seeded sample of leaves against the oracle
-    sel = np.sort(np.random.default_rng(0).choice(nleaf, 300, replace=False))
-    d_s = dists[torch.from_numpy(sel).cuda()].cpu().numpy()
-    d_o, cb_o, _ = O.sample_spans(O.mandelbulb(15, 2.0), lv[sel], res)
-    bit = (d_s.view(np.uint32) == d_o.view(np.uint32)) | (np.isnan(d_s) & np.isnan(d_o))
-    assert bit.mean() > 0.99 and (np.signbit(d_s) == np.signbit(d_o)).all()
-    voff_o, ioff_o, v_o, ix_o = O.mesh_batch(O.mandelbulb(15, 2.0), lv[sel], res, dists=d_s)
-    vo = voff.cpu().numpy()
-    io = ioff.cpu().numpy()
-    v = verts.cpu().numpy()
-    ix = idx.cpu().numpy().view(np.uint32)
-    for k, leaf in enumerate(sel):
-        a, b = int(vo[leaf]), int(vo[leaf + 1])
-        c, e = int(io[leaf]), int(io[leaf + 1])
-        assert b - a == int(voff_o[k + 1] - voff_o[k]) and e - c == int(ioff_o[k + 1] - ioff_o[k])
-        assert (ix[c:e] == ix_o[int(ioff_o[k]):int(ioff_o[k + 1])]).all()
-        assert (v[a:b, :3].view(np.uint32) == v_o[int(voff_o[k]):int(voff_o[k + 1]), :3].view(np.uint32)).all()
-    # their distances and 6-tap normals: the reference's formulas on the
-    # oracle's loop state, with the final ln correctly rounded
+
+def test_octree_workload_every_output(ctx, torch_cuda):
+    """Every output of the octree workload against the oracle, in chunks of
+    40,000 leaves (671.8 M corner distances, 75.2 M vertices):
+      * each corner distance has the sign of the oracle's (glibc ln) and
+        equals the reference's final formula on the oracle's loop state with
+        the ln correctly rounded, wherever that ln is not a hard case for
+        rounding (refmath.cr_logf);
+      * each leaf's vertex and index counts, every index and every vertex
+        position equal the oracle's surface nets run on the device's corner
+        distances;
+      * each vertex distance and 6-tap normal equals the reference's formulas
+        on the oracle's loop state with the same ln, outside the vertices with
+        a hard case among their taps."""
+    from cantucci_amd import device as dev
+    from cantucci_amd import workloads as wl
     from test_gpu_mesh import _cr_distances, _cr_fd_normals
-    rows = np.concatenate([np.arange(int(vo[leaf]), int(vo[leaf + 1])) for leaf in sel])
-    vs = v[rows]
-    d_cr, _ = _cr_distances(O.mandelbulb(15, 2.0), vs[:, :3])
-    n_cr = _cr_fd_normals(O.mandelbulb(15, 2.0), lv[sel], voff_o, vs, res)
-    d_same = (vs[:, 6].view(np.uint32) == d_cr.view(np.uint32)) | (np.isnan(vs[:, 6]) & np.isnan(d_cr))
-    n_same = ((vs[:, 3:6].view(np.uint32) == n_cr.view(np.uint32)) | (np.isnan(vs[:, 3:6]) & np.isnan(n_cr))).all(1)
-    print(f"{len(rows)} sampled vertices: distance / normal = the formulas with a correctly rounded ln on "
-          f"{d_same.mean():.6f} / {n_same.mean():.6f}")
-    assert d_same.mean() > 0.9999 and n_same.mean() > 0.9995
+    torch = torch_cuda
+    res = wl.OCTREE["res"]
+    shape, _, leaves, _, dists, _ = _octree_workload(ctx)
+    voff, ioff, verts, idx = dev.mesh(shape, leaves, dists, res, "fd6")
+    torch.cuda.synchronize()
+    lv = leaves.cpu().numpy()
+    vo = voff.cpu().numpy().astype(np.int64)
+    io = ioff.cpu().numpy().astype(np.int64)
+    sh = O.mandelbulb(15, 2.0)
+    n1 = res + 1
+    t = np.arange(n1, dtype=F32) / F32(res)
+    tot = dict(corners=0, c_diff=0, c_hard=0, verts=0, d_diff=0, n_diff=0, hard=0)
+    for a in range(0, len(lv), 40000):
+        b = min(len(lv), a + 40000)
+        L = lv[a:b]
+        # corners as the reference forms them (src/mesh/buffer.rs:64-83):
+        # s' = s + -(e - s) / res, e' = e + (e - s) / res, s' + (e' - s') * (i / res)
+        of = (L[:, 3:] - L[:, :3]) / F32(res)
+        s1 = L[:, :3] + -of
+        across = (L[:, 3:] + of) - s1
+        ax = s1[:, :, None] + across[:, :, None] * t[None, None, :]
+        P = np.empty((b - a, n1, n1, n1, 3), F32)
+        P[..., 0] = ax[:, 0, :, None, None]
+        P[..., 1] = ax[:, 1, None, :, None]
+        P[..., 2] = ax[:, 2, None, None, :]
+        d_gpu = dists[a:b].cpu().numpy()
+        d_o, _, r, dr = O.de_points(sh, P.reshape(-1, 3))
+        del P
+        dg = d_gpu.reshape(-1)
+        assert (np.signbit(dg) == np.signbit(d_o)).all(), f"leaves {a}..{b}: a corner sign differs"
+        ln_cr, hard = R.cr_logf(r)
+        with np.errstate(all="ignore"):
+            d_cr = (F32(0.5) * (ln_cr * r)) / dr
+        same = same_bits(dg, d_cr) | (np.isnan(dg) & np.isnan(d_cr))
+        assert (same | hard).all(), f"leaves {a}..{b}: {int((~same & ~hard).sum())} corner distances differ"
+        tot["corners"] += dg.size
+        tot["c_diff"] += int((~same).sum())
+        tot["c_hard"] += int(hard.sum())
+        del d_o, r, dr, ln_cr, hard, d_cr, same
+        voff_o, ioff_o, v_o, ix_o = O.mesh_batch(sh, L, res, dists=d_gpu)
+        assert (vo[a:b + 1] - vo[a] == voff_o).all() and (io[a:b + 1] - io[a] == ioff_o).all()
+        assert (idx[io[a]:io[b]].cpu().numpy().view(np.uint32) == ix_o.view(np.uint32)).all()
+        v = verts[vo[a]:vo[b]].cpu().numpy()
+        assert same_bits(v[:, :3], v_o[:, :3]).all(), f"leaves {a}..{b}: a vertex position differs"
+        d_cr, _, d_hard = _cr_distances(sh, v[:, :3])
+        n_cr, n_hard = _cr_fd_normals(sh, L, voff_o, v, res)
+        d_same = same_bits(v[:, 6], d_cr) | (np.isnan(v[:, 6]) & np.isnan(d_cr))
+        n_same = (same_bits(v[:, 3:6], n_cr) | (np.isnan(v[:, 3:6]) & np.isnan(n_cr))).all(1)
+        assert (d_same | d_hard).all() and (n_same | n_hard).all(), f"leaves {a}..{b}: a vertex differs"
+        tot["verts"] += len(v)
+        tot["d_diff"] += int((~d_same).sum())
+        tot["n_diff"] += int((~n_same).sum())
+        tot["hard"] += int((d_hard | n_hard).sum())
+    print(f"{tot['corners']} corner distances: {tot['c_diff']} differ from the formula, all among the "
+          f"{tot['c_hard']} hard cases for rounding; {tot['verts']} vertices (positions, indices identical): "
+          f"{tot['d_diff']} distances and {tot['n_diff']} normals differ, all among the {tot['hard']} with a "
+          f"hard case")
+    assert tot["corners"] == 921604 * n1 ** 3 and tot["verts"] == int(vo[-1])
+
+
+def same_bits(a, b):
+    return np.asarray(a, F32).view(np.uint32) == np.asarray(b, F32).view(np.uint32)
 
 
 def finite_normals(verts):

@@ -749,3 +749,34 @@ def test_get_focii_on_a_sphere():
     assert np.abs(depth[clear] - t_star).max() < 1e-5
     assert np.abs(np.linalg.norm(pos[hit].astype(np.float64) - c, axis=1) - 0.8).max() < 2e-6
     assert (steps <= 100).all() and (steps[~hit] == 100).all()
+
+
+def test_cr_logf_against_decimal():
+    """The correctly rounded f32 ln that the device distances are checked with
+    (refmath.cr_logf) against a 50-digit decimal ln: outside its hard mask the
+    value is the f32 nearest to the exact ln; every input it flags as hard has
+    an exact ln within the band of a rounding boundary; hard inputs are about
+    2^-17 of all (the band is 2^-18 ulps on either side of a boundary)."""
+    import decimal
+    ctx = decimal.Context(prec=50)
+    D = decimal.Decimal
+    rng = np.random.default_rng(5)
+    x = np.concatenate([rng.uniform(0.25, 64, 3000), np.exp(rng.uniform(-80, 80, 2000)),
+                        1 + rng.uniform(-1 / 16, 1 / 16, 1000), [1.0, 2.0, 0.5, 2.0 ** -126, 3.4e38]]).astype(F32)
+    many = rng.uniform(0.25, 4.0, 4_000_000).astype(F32)
+    _, hard_many = R.cr_logf(many)
+    assert 0.5 * 2.0 ** -17 < hard_many.mean() < 2.0 * 2.0 ** -17
+    x = np.concatenate([x, many[hard_many][:40]])
+    f, hard = R.cr_logf(x)
+    assert hard.sum() >= 20
+    for xi, fi, hi in zip(x, f, hard):
+        ln = ctx.ln(D(float(xi)))
+        lo = D(float(np.nextafter(fi, F32(-np.inf))))
+        up = D(float(np.nextafter(fi, F32(np.inf))))
+        fd = D(float(fi))
+        if not hi:
+            assert abs(ln - fd) <= abs(ln - lo) and abs(ln - fd) <= abs(ln - up), float(xi)
+        else:
+            side = up if ln > fd else lo
+            mid = (fd + side) / 2
+            assert abs(ln - mid) <= D(2.0 ** -18) * abs(side - fd), float(xi)
