@@ -5,6 +5,7 @@ Each wrapper names the reference function it restates; the C++ source
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import os
 import subprocess
@@ -22,7 +23,7 @@ K_SPHERE = 2
 __all__ = [
     "OrcShape", "mandelbulb", "mandelbulb_trig", "sphere", "de_points", "de_points_f64", "de_grad_f64",
     "de_grid", "sample_spans", "surface_nets", "mesh_batch", "focus_split", "create_spans", "refine", "ray_dirs", "focus_dirs", "get_focii", "sphere_trace", "fd_diff",
-    "powi", "build", "K_MANDELBULB", "K_MANDELBULB_TRIG", "K_SPHERE", "default_threads",
+    "powi", "build", "K_MANDELBULB", "K_MANDELBULB_TRIG", "K_SPHERE", "default_threads", "ln_mode",
 ]
 
 
@@ -70,6 +71,8 @@ for _name, _res, _args in [
     ("orc_ray_dirs", None, [_P, _U, _U, _U, _U, _P]),
     ("orc_focus_dirs", None, [_P, _U, _P]),
     ("orc_sphere_trace", None, [_SH, _P, _P, _S, _I, _U, _F, _F, _F, _P, _P, _P, _I]),
+    ("orc_sphere_trace_marked", None, [_SH, _P, _P, _S, _I, _U, _F, _F, _F, _P, _P, _P, _P, _I]),
+    ("orc_set_ln_mode", None, [_I]),
     ("orc_fd_diff", None, [_SH, _P, _P, _S, _F, _P, _I]),
     ("orc_powi_f32", _F, [_F, _I]),
 ]:
@@ -278,8 +281,10 @@ def get_focii(shape: OrcShape, cam, focus_points: int = 5, max_steps: int = 100,
 
 
 def sphere_trace(shape: OrcShape, origins, dirs, max_steps=256, eps_rel=1e-4, t_max=10.0, eps_abs=0.0,
-                 with_pos=False, threads=None):
-    """Sphere tracing (src/mesh/mod.rs:229-241; hit when d < eps_abs + eps_rel * t)."""
+                 with_pos=False, threads=None, marked=False):
+    """Sphere tracing (src/mesh/mod.rs:229-241; hit when d < eps_abs + eps_rel * t).
+    With marked=True also returns, last, a bool per ray: its march took the
+    final ln of a hard case (only under ln_mode("cr"))."""
     o = _f32(origins).reshape(-1, 3)
     dd = _f32(dirs).reshape(-1, 3)
     n = len(dd)
@@ -287,9 +292,27 @@ def sphere_trace(shape: OrcShape, origins, dirs, max_steps=256, eps_rel=1e-4, t_
     depth = np.empty(n, np.float32)
     steps = np.empty(n, np.uint32)
     pos = np.empty((n, 3), np.float32) if with_pos else None
-    _lib.orc_sphere_trace(C.byref(shape), _ptr(o), _ptr(dd), n, stride, max_steps, float(eps_abs), float(eps_rel),
-                          float(t_max), _ptr(depth), _ptr(steps), _ptr(pos), threads or default_threads())
-    return (depth, steps, pos) if with_pos else (depth, steps)
+    hard = np.zeros(n, np.uint8) if marked else None
+    _lib.orc_sphere_trace_marked(C.byref(shape), _ptr(o), _ptr(dd), n, stride, max_steps, float(eps_abs),
+                                 float(eps_rel), float(t_max), _ptr(depth), _ptr(steps), _ptr(pos), _ptr(hard),
+                                 threads or default_threads())
+    out = (depth, steps, pos) if with_pos else (depth, steps)
+    return out + (hard.astype(bool),) if marked else out
+
+
+@contextlib.contextmanager
+def ln_mode(mode: str):
+    """Within the block the final formula's ln is glibc's logf ("glibc", the
+    reference and the default) or the f64 log rounded to f32 ("cr": the
+    correctly rounded f32 ln outside hard cases, which is what the device
+    computes)."""
+    if mode not in ("glibc", "cr"):
+        raise ValueError("ln mode must be 'glibc' or 'cr'")
+    _lib.orc_set_ln_mode(1 if mode == "cr" else 0)
+    try:
+        yield
+    finally:
+        _lib.orc_set_ln_mode(0)
 
 
 def fd_diff(shape: OrcShape, pts, h: float, powers=None, threads=None) -> np.ndarray:

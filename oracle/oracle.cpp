@@ -10,7 +10,11 @@
 // `-ffp-contract=off -fno-fast-math` (see oracle/Makefile); x86-64 SSE gives the
 // same default-NaN (0xFFC00000, sign bit set) that the reference produces.
 // Transcendentals are glibc's (`logf`, `acosf`, `atan2f`, `sinf`, `cosf`,
-// `powf`), which is what Rust std links on x86_64-linux-gnu.
+// `powf`), which is what Rust std links on x86_64-linux-gnu.  One switch,
+// `orc_set_ln_mode(1)`, replaces the final formula's `logf` with the f64 log
+// rounded to f32 (the correctly rounded f32 ln outside hard cases, inputs
+// whose ln lies within 2^-18 ulps of a rounding boundary); the trace entry
+// `orc_sphere_trace_marked` then marks the rays whose march met a hard case.
 //
 // Citations are path:line in the cantucci source tree.
 //
@@ -37,6 +41,8 @@
 //   * sphere tracing         -- sphere: analytic ray/sphere hit.
 //   * dual-number gradients  -- FP64 central differences; sphere closed form;
 //                               0-iteration closed form.
+//   * ln mode 1, marked trace -- 50-digit decimal ln outside hard cases; a
+//                               march restated step by step marks the same rays.
 // Parity unpinned (no reference test or closed form fixes them; stated):
 //   * LLVM's lowering order of `powi` (the ladder below) -- at most 1 ulp/op.
 //   * cgmath's `normalize`, `lerp` operation order (vertex normals/positions).
@@ -112,7 +118,25 @@ inline float lib_cos(float v) { return cosf(v); }
 inline double lib_cos(double v) { return cos(v); }
 inline float lib_pow(float a, float b) { return powf(a, b); }
 inline double lib_pow(double a, double b) { return pow(a, b); }
-inline float lib_log(float v) { return logf(v); }
+// 0: glibc's logf (the reference).  1: the f64 log rounded to f32, with
+// t_ln_hard set when the f64 value lies within 2^-18 f32 ulps of a midpoint
+// between two f32 values, where an f64 evaluation with a relative error
+// below 2^-43 may round either way.
+std::atomic<int> g_ln_mode{0};
+thread_local bool t_ln_hard = false;
+
+inline float cr_log(float v) {
+    const double y = std::log((double)v);
+    const float f = (float)y;
+    if (std::isfinite(y) && (double)f != y) {
+        const float nb = std::nextafterf(f, y > (double)f ? INFINITY : -INFINITY);
+        const double mid = ((double)f + (double)nb) * 0.5;
+        const double gap = std::fabs((double)nb - (double)f);
+        if (std::fabs(y - mid) < 0x1p-18 * gap) t_ln_hard = true;
+    }
+    return f;
+}
+inline float lib_log(float v) { return g_ln_mode.load(std::memory_order_relaxed) == 1 ? cr_log(v) : logf(v); }
 inline double lib_log(double v) { return log(v); }
 
 // `Vec3::is_on_z_axis` (src/shape/mandelbulb.rs:421-426): PTEST of the x and y
@@ -1013,11 +1037,14 @@ void orc_focus_dirs(const float* cam, uint32_t fp, float* dirs) {
 //   miss if t > t_max                  -> depth = +inf (reference: no escape)
 // after max_steps without either -> depth = +inf, steps = max_steps.
 // hit_pos (nullable) receives the position after the last step.
-void orc_sphere_trace(const orc_shape* shape, const float* origins, const float* dirs, size_t n, int origin_stride,
-                      uint32_t max_steps, float eps_abs, float eps_rel, float t_max, float* depth, uint32_t* steps,
-                      float* hit_pos, int threads) {
+// orc_sphere_trace_marked also sets hard[i] (nullable) to 1 when ray i's
+// march took the final ln of a hard case (ln mode 1 only), else 0.
+void orc_sphere_trace_marked(const orc_shape* shape, const float* origins, const float* dirs, size_t n,
+                             int origin_stride, uint32_t max_steps, float eps_abs, float eps_rel, float t_max,
+                             float* depth, uint32_t* steps, float* hit_pos, uint8_t* hard, int threads) {
     const orc_shape s = *shape;
     parallel_for(n, threads, 64, [&](size_t i) {
+        t_ln_hard = false;
         const float* o = origins + (origin_stride ? 3 * i : 0);
         float px = o[0], py = o[1], pz = o[2];
         float dx = dirs[3 * i], dy = dirs[3 * i + 1], dz = dirs[3 * i + 2];
@@ -1044,8 +1071,19 @@ void orc_sphere_trace(const orc_shape* shape, const float* origins, const float*
             hit_pos[3 * i + 1] = py;
             hit_pos[3 * i + 2] = pz;
         }
+        if (hard) hard[i] = t_ln_hard ? 1 : 0;
     });
 }
+
+void orc_sphere_trace(const orc_shape* shape, const float* origins, const float* dirs, size_t n, int origin_stride,
+                      uint32_t max_steps, float eps_abs, float eps_rel, float t_max, float* depth, uint32_t* steps,
+                      float* hit_pos, int threads) {
+    orc_sphere_trace_marked(shape, origins, dirs, n, origin_stride, max_steps, eps_abs, eps_rel, t_max, depth, steps,
+                            hit_pos, nullptr, threads);
+}
+
+// Final-formula ln: 0 glibc's logf (default), 1 the f64 log rounded to f32.
+void orc_set_ln_mode(int mode) { g_ln_mode.store(mode == 1 ? 1 : 0); }
 
 // 6-tap central-difference gradient, f32 (the reference's normal stencil,
 // src/mesh/buffer.rs:256-266, with a caller-given step h per axis): returns

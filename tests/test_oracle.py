@@ -780,3 +780,43 @@ def test_cr_logf_against_decimal():
             side = up if ln > fd else lo
             mid = (fd + side) / 2
             assert abs(ln - mid) <= D(2.0 ** -18) * abs(side - fd), float(xi)
+
+
+def test_marked_trace_flags_hard_cases():
+    """The oracle's marked sphere trace under the correctly rounded ln: depths
+    and step counts equal a march restated here step by step from the
+    oracle's per-point DE (same formula, same ln mode), and a ray is marked
+    exactly when one of its steps took the ln of a hard case
+    (refmath.cr_logf on that step's final r).  Under glibc's logf nothing is
+    marked."""
+    from cantucci_amd import workloads as wl
+    tc = wl.TRACE_1080P
+    cam = wl.orbit_poses(4, 7, 320, 180, tc["fov_deg"])[1]
+    dirs = O.ray_dirs(cam, 320, 180)
+    sh = O.mandelbulb(15, 2.0)
+    with O.ln_mode("cr"):
+        depth, steps, marked = O.sphere_trace(sh, cam[:3][None], dirs, 256, 1e-4, 10.0, marked=True)
+        n = len(dirs)
+        pos = np.repeat(cam[:3][None], n, 0).astype(F32)
+        t = np.zeros(n, F32)
+        k = np.zeros(n, np.uint32)
+        out = np.full(n, np.inf, F32)
+        hard_any = np.zeros(n, bool)
+        act = np.ones(n, bool)
+        for _ in range(256):
+            i = np.nonzero(act)[0]
+            if len(i) == 0:
+                break
+            d, _, r, _ = O.de_points(sh, pos[i])
+            hard_any[i] |= R.cr_logf(r)[1]
+            pos[i] = pos[i] + dirs[i] * d[:, None]
+            t[i] = t[i] + d
+            k[i] += 1
+            hit = d < F32(0.0) + F32(1e-4) * t[i]
+            out[i[hit]] = t[i[hit]]
+            act[i[hit | (t[i] > F32(10.0))]] = False
+    assert (steps == k).all() and (depth.view(np.uint32) == out.view(np.uint32)).all()
+    assert (marked == hard_any).all()
+    _, _, marked_g = O.sphere_trace(sh, cam[:3][None], dirs, 256, 1e-4, 10.0, marked=True)
+    assert not marked_g.any()
+    print(f"{n} rays, {int(marked.sum())} marked")
