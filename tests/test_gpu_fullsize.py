@@ -94,7 +94,7 @@ def test_octree_workload_every_output(ctx, torch_cuda):
         a hard case among their taps."""
     from cantucci_amd import device as dev
     from cantucci_amd import workloads as wl
-    from test_gpu_mesh import _cr_distances, _cr_fd_normals
+    from test_gpu_mesh import _cr_distances, _cr_fd_normals, corner_points
     torch = torch_cuda
     res = wl.OCTREE["res"]
     shape, _, leaves, _, dists, _ = _octree_workload(ctx)
@@ -105,24 +105,12 @@ def test_octree_workload_every_output(ctx, torch_cuda):
     io = ioff.cpu().numpy().astype(np.int64)
     sh = O.mandelbulb(15, 2.0)
     n1 = res + 1
-    t = np.arange(n1, dtype=F32) / F32(res)
     tot = dict(corners=0, c_diff=0, c_hard=0, verts=0, d_diff=0, n_diff=0, hard=0)
     for a in range(0, len(lv), 40000):
         b = min(len(lv), a + 40000)
         L = lv[a:b]
-        # corners as the reference forms them (src/mesh/buffer.rs:64-83):
-        # s' = s + -(e - s) / res, e' = e + (e - s) / res, s' + (e' - s') * (i / res)
-        of = (L[:, 3:] - L[:, :3]) / F32(res)
-        s1 = L[:, :3] + -of
-        across = (L[:, 3:] + of) - s1
-        ax = s1[:, :, None] + across[:, :, None] * t[None, None, :]
-        P = np.empty((b - a, n1, n1, n1, 3), F32)
-        P[..., 0] = ax[:, 0, :, None, None]
-        P[..., 1] = ax[:, 1, None, :, None]
-        P[..., 2] = ax[:, 2, None, None, :]
         d_gpu = dists[a:b].cpu().numpy()
-        d_o, _, r, dr = O.de_points(sh, P.reshape(-1, 3))
-        del P
+        d_o, _, r, dr = O.de_points(sh, corner_points(L, res))
         dg = d_gpu.reshape(-1)
         assert (np.signbit(dg) == np.signbit(d_o)).all(), f"leaves {a}..{b}: a corner sign differs"
         ln_cr, hard = R.cr_logf(r)

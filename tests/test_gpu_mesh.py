@@ -179,6 +179,24 @@ def test_octree_leaf_mesh_parity(ctx, torch_cuda):
     assert_topology(voff, ioff, ix, voff_e, ioff_e, ix_e)
 
 
+def corner_points(spans, res):
+    """The corners of each span in GridTable order, formed as the reference
+    forms them (src/mesh/buffer.rs:64-83): s' = s + -(e - s) / res,
+    e' = e + (e - s) / res, corner = s' + (e' - s') * (i / res) per axis."""
+    L = np.asarray(spans, F32).reshape(-1, 6)
+    n1 = res + 1
+    t = np.arange(n1, dtype=F32) / F32(res)
+    of = (L[:, 3:] - L[:, :3]) / F32(res)
+    s1 = L[:, :3] + -of
+    across = (L[:, 3:] + of) - s1
+    ax = s1[:, :, None] + across[:, :, None] * t[None, None, :]
+    P = np.empty((len(L), n1, n1, n1, 3), F32)
+    P[..., 0] = ax[:, 0, :, None, None]
+    P[..., 1] = ax[:, 1, None, :, None]
+    P[..., 2] = ax[:, 2, None, None, :]
+    return P.reshape(-1, 3)
+
+
 def _cr_distances(shape_o, pts):
     """The reference's distances at pts from the oracle's loop state, with the
     final ln correctly rounded (as the kernels compute it) instead of glibc's
@@ -358,8 +376,11 @@ def test_mesh_edge_cases(ctx, torch_cuda):
 def test_shape_mesh_new_and_update(ctx, torch_cuda):
     """ShapeMesh::new and two updates (src/mesh/mod.rs:45-170) against the
     oracle: the 64 leaves of new(), the leaves the focus-point rule splits,
-    and every new leaf's mesh at resolution 64 on the device's corner
-    distances (topology and positions bit for bit)."""
+    and every new leaf's mesh at resolution 64: every corner distance equal
+    to the reference's formula with the ln correctly rounded (outside hard
+    cases), topology and positions bit for bit against the oracle's surface
+    nets on the device's corner distances, and every vertex distance and
+    6-tap normal equal to the reference's formulas (outside hard cases)."""
     import cantucci_amd as ca
     from cantucci_amd import device as dev
     from cantucci_amd.mesh import Camera, ShapeMesh
@@ -374,13 +395,25 @@ def test_shape_mesh_new_and_update(ctx, torch_cuda):
     def check_meshes(which):
         sp = spans_of()[which]
         d, _ = dev.sample_spans(shape, gpu(torch, sp), 64, with_cross=False)
-        voff_o, ioff_o, v_o, ix_o = O.mesh_batch(sh, sp, 64, dists=d.cpu().numpy())
+        dn = d.cpu().numpy()
+        d_cr, _, c_hard = _cr_distances(sh, corner_points(sp, 64))
+        c_same = same_bits(dn.reshape(-1), d_cr) | (np.isnan(dn.reshape(-1)) & np.isnan(d_cr))
+        assert (c_same | c_hard).all()
+        voff_o, ioff_o, v_o, ix_o = O.mesh_batch(sh, sp, 64, dists=dn)
         for k, i in enumerate(which):
             buf = sm.leaves[i][1]
             a, b = int(voff_o[k]), int(voff_o[k + 1])
             assert buf is not None and len(buf.vertices) == b - a
             assert (buf.indices == ix_o[int(ioff_o[k]):int(ioff_o[k + 1])]).all()
             assert same_bits(buf.vertices["position"], v_o[a:b, :3]).all()
+        V = np.concatenate([sm.leaves[i][1].vertices.view(F32).reshape(-1, 7) for i in which])
+        dv, _, dv_hard = _cr_distances(sh, V[:, :3])
+        nv, nv_hard = _cr_fd_normals(sh, sp, voff_o, V, 64)
+        dv_same = same_bits(V[:, 6], dv) | (np.isnan(V[:, 6]) & np.isnan(dv))
+        nv_same = (same_bits(V[:, 3:6], nv) | (np.isnan(V[:, 3:6]) & np.isnan(nv))).all(1)
+        assert (dv_same | dv_hard).all() and (nv_same | nv_hard).all()
+        print(f"{len(which)} leaves: {dn.size} corners ({int((~c_same).sum())} differ, {int(c_hard.sum())} hard), "
+              f"{len(V)} vertices ({int((~dv_same).sum())} distances and {int((~nv_same).sum())} normals differ)")
 
     cam = Camera((-1.6, 0.2, 0.1), (1.0, -0.1, -0.05), 1.0, 16 / 9, 0.00004, 10.0)
     sm.update(cam)  # nothing is meshed yet, so nothing splits; the 64 leaves are meshed
