@@ -25,6 +25,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "cantucci_hip.h"
 #include "log_table.h"
 
@@ -189,9 +191,12 @@ struct P8Exact {
     // the iteration cap after max_iters - 1 bodies and apply the last body's
     // dr update alone (see last_dr).
     static constexpr bool kLastDr = true;
+    // The work loops may use guard(), mag_tick() and body_guarded().
+    static constexpr bool kCarriedGuard = true;
     float px, py, pz, x, y, z, dr;
     float x2, y2, w2, z2;  // squares of the last magnitude, reused by rotate
     bool pos_nan;          // the orbit's first NaN came from acosf(+-inf)
+    bool fast = false;     // guard(): the next body may take the unscaled sequences
 
     __device__ __forceinline__ void start(float ax, float ay, float az, float, const MandelUniform&) {
         px = x = ax;
@@ -210,10 +215,36 @@ struct P8Exact {
         z2 = z * z;
         return sqrt_rn_wave(w2 + z2);
     }
+    // The wave test of body() (see there) on the current squares, kept for the
+    // next body_guarded().  The work loops call it after starting points.
+    __device__ __forceinline__ void guard(const MandelUniform& u) {
+        fast = u.bailout <= 16.0f && wave_mask(!(w2 >= 0x1p-15f)) == 0;
+    }
+    // mag() after a loop body, with the wave test of the next body made here
+    // and also used for this square root: m = w2 + z2 >= w2 >= 2^-15, and a
+    // point that was live before the body (r <= bailout <= 16) has every
+    // coordinate of its rotation below 2^46 (each is a degree-8 form in x, y,
+    // z whose denominator w8 cancels against the numerator's w^8 factor), so
+    // m < 2^94: the range of sqrt_rn_unscaled.  Lanes without a live point
+    // may be outside it; their results are dropped.
+    __device__ __forceinline__ float mag_tick(const MandelUniform& u) {
+        x2 = x * x;
+        y2 = y * y;
+        w2 = x2 + y2;
+        z2 = z * z;
+        guard(u);
+        const float m = w2 + z2;
+        return fast ? sqrt_rn_unscaled(m) : sqrt_rn_wave(m);
+    }
+    __device__ __forceinline__ void body(float r, const MandelUniform& u, bool live) {
+        body_with(r, live, u.bailout <= 16.0f && wave_mask(!(w2 >= 0x1p-15f)) == 0);
+    }
+    // body() with the wave test made by the last guard() or mag_tick().
+    __device__ __forceinline__ void body_guarded(float r, const MandelUniform&, bool live) { body_with(r, live, fast); }
     // One loop body for every lane of the wave, committed only where `live`
     // (lanes without a live point compute on stale state and discard it).
     // The z-axis branch is rare, so it runs behind a wave-uniform test.
-    __device__ __forceinline__ void body(float r, const MandelUniform& u, bool live) {
+    __device__ __forceinline__ void body_with(float r, bool live, bool unscaled) {
         // dr = r.powi(7) * 8.0 * dr + 1.0      (:73)
         float r2 = r * r;
         float r4 = r2 * r2;
@@ -234,8 +265,8 @@ struct P8Exact {
         // keeps a live lane's w2 in the square root's range [2^-96, 256]
         // (lanes without a live point may be outside: their results are
         // dropped), and it rules out the z axis (x = y = 0 gives w2 = 0).
+        // `unscaled` is that test: bailout <= 16 and every lane's w2 >= 2^-15.
         float q, sw2;
-        const bool unscaled = u.bailout <= 16.0f && wave_mask(!(w2 >= 0x1p-15f)) == 0;
         if (unscaled) {
             q = div_rn_unscaled(num, w8);
             sw2 = sqrt_rn_unscaled(w2);
@@ -314,7 +345,22 @@ struct P8Contract : P8Exact {
         z2 = z * z;
         return sqrt_rn_wave(w2 + z2);
     }
+    // P8Exact::mag_tick with contraction (the same ranges hold).
+    __device__ __forceinline__ float mag_tick(const MandelUniform& u) {
+#pragma clang fp contract(fast)
+        x2 = x * x;
+        y2 = y * y;
+        w2 = x2 + y2;
+        z2 = z * z;
+        guard(u);
+        const float m = w2 + z2;
+        return fast ? sqrt_rn_unscaled(m) : sqrt_rn_wave(m);
+    }
     __device__ __forceinline__ void body(float r, const MandelUniform& u, bool live) {
+        body_with(r, live, u.bailout <= 16.0f && wave_mask(!(w2 >= 0x1p-15f)) == 0);
+    }
+    __device__ __forceinline__ void body_guarded(float r, const MandelUniform&, bool live) { body_with(r, live, fast); }
+    __device__ __forceinline__ void body_with(float r, bool live, bool unscaled) {
 #pragma clang fp contract(fast)
         float r2 = r * r;
         float r4 = r2 * r2;
@@ -327,7 +373,6 @@ struct P8Contract : P8Exact {
         float num = ((z8 - (28.0f * z6) * w2) + (70.0f * z4) * w4) - (28.0f * z2) * w6;
         // The ranges of P8Exact::body: the guard depends on w2 only.
         float q, sw2;
-        const bool unscaled = u.bailout <= 16.0f && wave_mask(!(w2 >= 0x1p-15f)) == 0;
         if (unscaled) {
             q = div_rn_unscaled(num, w8);
             sw2 = sqrt_rn_unscaled(w2);
@@ -550,6 +595,34 @@ struct SphereEv {
     __device__ __forceinline__ void body(float, const MandelUniform&, bool) {}
     __device__ __forceinline__ float finish(float) const { return sphere_de(px, py, pz, c, radius); }
 };
+
+// The tick of a work loop through an evaluator.  Evaluators with
+// kCarriedGuard make the next body's wave test in the magnitude after a body
+// (mag_tick) and in guard(), which the loops call after starting points,
+// instead of in the body (body_guarded); the others keep mag() and body().
+template <class E, class = void>
+struct CarriesGuard : std::false_type {};
+template <class E>
+struct CarriesGuard<E, std::void_t<decltype(E::kCarriedGuard)>> : std::bool_constant<E::kCarriedGuard> {};
+
+template <class E>
+__device__ __forceinline__ void tick_guard(E& ev, const MandelUniform& u) {
+    if constexpr (CarriesGuard<E>::value) ev.guard(u);
+}
+template <class E>
+__device__ __forceinline__ void tick_body(E& ev, float r, const MandelUniform& u, bool live) {
+    if constexpr (CarriesGuard<E>::value)
+        ev.body_guarded(r, u, live);
+    else
+        ev.body(r, u, live);
+}
+template <class E>
+__device__ __forceinline__ float tick_mag(E& ev, const MandelUniform& u) {
+    if constexpr (CarriesGuard<E>::value)
+        return ev.mag_tick(u);
+    else
+        return ev.mag();
+}
 
 // Calls f(Ev{}) with the P8 evaluator of an arithmetic mode.
 template <class F>

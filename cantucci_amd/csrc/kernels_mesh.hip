@@ -893,18 +893,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
                 tally.body(now_m & ~zero_m);
             }
             if (live_m == 0) break;
+            tick_guard(ev, u);  // the started taps' squares
         }
         const bool live = __builtin_amdgcn_inverse_ballot_w64(live_m);
         if constexpr (Ev::kLastDr) {
             // as in refill_loop: a tap at the cap leaves after max_iters - 1
             // bodies and its last body's dr update is applied when it is held
-            ev.body(r, u, live);
+            tick_body(ev, r, u, live);
             ++tick;
             tally.body(live_m);
-            const float rn = ev.mag();
+            const float rn = tick_mag(ev, u);
             const uint64_t bail_m = wave_mask(rn > u.bailout) & live_m;
             tally.bail(bail_m);
-            if (live) r = rn;
+            // Every lane takes rn: a lane without a live point ran the body
+            // without committing it, so rn is its last magnitude again, bit
+            // for bit (a held point's r, the one its finish needs).
+            r = rn;
             const uint64_t cap_m = wave_mask(t0 == tick + 1u - u.max_iters) & live_m & ~bail_m;
             tally.body(cap_m);
             last_m |= cap_m;
@@ -912,10 +916,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
             hold_m |= cap_m | bail_m;
         } else {
             const uint64_t cap_m = wave_mask(t0 == tick + 1u - u.max_iters) & live_m;  // the max_iters-th body
-            ev.body(r, u, live);
+            tick_body(ev, r, u, live);
             ++tick;
             tally.body(live_m);
-            const float rn = ev.mag();
+            const float rn = tick_mag(ev, u);
             const uint64_t bail_m = wave_mask(rn > u.bailout) & live_m & ~cap_m;
             tally.bail(bail_m);
             // at the cap r stays the magnitude before the last body (src/shape/mandelbulb.rs:65-78)

@@ -32,6 +32,7 @@ __device__ __forceinline__ float grad_final(float r, float dr, float rq, float d
 struct P8Tangent : P8Exact {
     static constexpr int kWavesPerSimd = 1;  // the tangent state does not fit 64 VGPRs
     static constexpr bool kLastDr = false;   // the work loops run its last body in full
+    static constexpr bool kCarriedGuard = false;  // its body() also carries the tangents
     float T[3][3];    // dz/dp
     float tdr[3];     // d dr / dp
     float rq_prev[3]; // dr/dp of the magnitude entering the last body

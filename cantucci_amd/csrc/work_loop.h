@@ -204,17 +204,21 @@ __device__ __forceinline__ void refill_loop(const Src& src, const MandelUniform&
                 c_next += (nneed < avail) ? nneed : avail;
             }
             if (live_m == 0) break;
+            tick_guard(ev, u);  // the started points' squares
         }
         const bool live = __builtin_amdgcn_inverse_ballot_w64(live_m);
         uint64_t done_m;
         if constexpr (Ev::kLastDr) {
-            ev.body(r, u, live);
+            tick_body(ev, r, u, live);
             ++tick;
             tally.body(live_m);
-            const float rn = ev.mag();
+            const float rn = tick_mag(ev, u);
             const uint64_t bail_m = wave_mask(rn > u.bailout) & live_m;
             tally.bail(bail_m);
-            if (live) r = rn;
+            // Every lane takes rn: a lane without a live point ran the body
+            // without committing it, so rn is its last magnitude again, bit
+            // for bit (a held point's r, the one its finish needs).
+            r = rn;
             // Lanes that have now run max_iters - 1 bodies without bailing:
             // their max_iters-th body is the dr update alone (max_iters >= 2
             // here; max_iters == 1 ends at the start).
@@ -229,10 +233,10 @@ __device__ __forceinline__ void refill_loop(const Src& src, const MandelUniform&
         } else {
             // Lanes making their max_iters-th body in this tick (max_iters >= 1).
             const uint64_t cap_m = wave_mask(t0 == tick + 1u - u.max_iters) & live_m;
-            ev.body(r, u, live);
+            tick_body(ev, r, u, live);
             ++tick;
             tally.body(live_m);
-            const float rn = ev.mag();
+            const float rn = tick_mag(ev, u);
             const uint64_t bail_m = wave_mask(rn > u.bailout) & live_m & ~cap_m;
             tally.bail(bail_m);
             // At the iteration cap, r stays the magnitude computed before the
