@@ -31,6 +31,55 @@ def test_library_exports_every_declared_symbol():
     assert not missing, f"not exported: {missing}"
 
 
+def _prototypes():
+    """(return type, name, [parameter types]) of every cth_* prototype in the
+    public headers."""
+    out = []
+    for h in (ROOT / "include").glob("*.h"):
+        text = h.read_text()
+        text = re.sub(r"/\*.*?\*/", " ", text, flags=re.S)
+        text = re.sub(r"//[^\n]*", " ", text)
+        pat = r"\b((?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\**)\s*\b(cth_[a-z0-9_]+)\s*\(([^;{]*?)\)\s*;"
+        for ret, name, args in re.findall(pat, text, flags=re.S):
+            a = " ".join(args.split())
+            out.append((" ".join(ret.split()), name, [] if a in ("", "void") else [p.strip() for p in a.split(",")]))
+    return out
+
+
+def _c_kind(t):
+    """Kind of a C parameter or return type: 'ptr' for pointers and arrays,
+    else the scalar's width and class (size_t is 64-bit on this ABI)."""
+    t = " ".join(t.replace("const", " ").split())
+    if "*" in t or "[" in t:
+        return "ptr"
+    base = t.split()[0] if t.split()[0] != "unsigned" else t
+    return {"int": "i32", "int32_t": "i32", "float": "f32", "uint32_t": "u32", "size_t": "u64", "uint64_t": "u64",
+            "double": "f64", "void": "void"}.get(base, base)
+
+
+def _py_kind(a):
+    if a is None:
+        return "void"
+    if a in (C.c_void_p, C.c_char_p) or (isinstance(a, type) and issubclass(a, C._Pointer)):
+        return "ptr"
+    return {C.c_int: "i32", C.c_float: "f32", C.c_uint32: "u32", C.c_uint64: "u64", C.c_double: "f64"}.get(a, repr(a))
+
+
+def test_ctypes_signatures_match_header():
+    """Every prototype of the header has a ctypes signature in the Python
+    binding with the same number of parameters and the same kind of each
+    parameter and of the result (a mismatch would pass wrong values to the
+    device entry points)."""
+    from cantucci_amd import _lib
+    protos = _prototypes()
+    assert len(protos) == len(declared_symbols())
+    assert set(_lib._SIGS) == {name for _, name, _ in protos}
+    for ret, name, params in protos:
+        res, args = _lib._SIGS[name]
+        assert [_c_kind(p) for p in params] == [_py_kind(a) for a in args], name
+        assert _c_kind(ret) == _py_kind(res), name
+
+
 def test_header_has_no_torch_or_cpp_types():
     text = (ROOT / "include" / "cantucci_hip.h").read_text()
     for bad in ("torch", "at::", "std::", "Tensor"):
