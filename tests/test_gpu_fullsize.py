@@ -1,5 +1,5 @@
 """The benchmark workloads at their full sizes, checked through properties
-that do not depend on size plus seeded samples against the oracle.
+that do not depend on size and, output by output, against the oracle.
 
 * octree workload: level counts and leaves identical to the oracle's
   refinement; crossing words of span sampling and of the mesh count pass
@@ -8,10 +8,11 @@ that do not depend on size plus seeded samples against the oracle.
   distances, vertices and indices) against the oracle.
 * 32 x 1080p sphere tracing: one seeded row per pose traced by the oracle,
   and every ray against the oracle's march with the correctly rounded ln.
-* 2^22 shell points: tangent (and 6-tap FD) normals against the FP64 dual
-  gradient on a seeded sample, also without the points whose stencil changes
-  the iteration count.
-* 64 mixed jobs: three seeded jobs against the oracle's grid.
+* 2^22 shell points: every point's tangent (and 6-tap FD) normal against the
+  FP64 dual gradient, also without the points whose stencil changes the
+  iteration count.
+* 64 mixed jobs: every point of every job against the oracle and the FP64
+  dual gradient.
 """
 import numpy as np
 import pytest
@@ -215,8 +216,8 @@ def test_camera_trace_every_ray(ctx, torch_cuda):
 
 def test_shell_normals_full_size(ctx, torch_cuda):
     """2^22 shell points, power U[6,10] (the normals workload at full size):
-    the forward-mode normals against the FP64 dual-number gradient on 20,000
-    seeded points, with the 6-tap central difference (h = 1e-3) beside them.
+    the forward-mode normals of every point against the FP64 dual-number
+    gradient, with the 6-tap central difference (h = 1e-3) beside them.
     Points whose 6-tap stencil changes the iteration count (the distance
     estimate has a kink there, so no difference quotient can follow the
     gradient) are counted and the errors are also given without them."""
@@ -231,7 +232,7 @@ def test_shell_normals_full_size(ctx, torch_cuda):
     _, n_tan = dev.normals(shape, P, W)
     _, n_fd = dev.normals_fd(shape, P, sc["fd_h"], W)
     torch.cuda.synchronize()
-    sel = np.random.default_rng(2).choice(sc["n"], 20000, replace=False)
+    sel = np.arange(sc["n"])
     sel_t = torch.from_numpy(sel).cuda()
     orc = O.mandelbulb_trig(15, 2.0)
     _, g64, _ = O.de_grad_f64(orc, pts[sel], powers[sel])
@@ -265,6 +266,9 @@ def test_shell_normals_full_size(ctx, torch_cuda):
 
 
 def test_mixed_jobs_full_size(ctx, torch_cuda):
+    """The 64 mixed jobs in one persistent launch (56.4 M points): every job's
+    completion stamp set, and every point of every job against the oracle and
+    the FP64 dual gradient (the bar of check_trig_job)."""
     import cantucci_amd as ca
     from cantucci_amd import _lib
     from cantucci_amd import device as dev
@@ -282,12 +286,10 @@ def test_mixed_jobs_full_size(ctx, torch_cuda):
     assert js.total == int((np.asarray(N, np.int64) ** 3).sum())
     ticks = done.cpu().numpy().astype(np.uint64)
     assert (ticks > np.uint64(start.cpu().numpy()[0])).all()
-    # distances and normals of 6 seeded jobs, 20,000 seeded points each,
-    # against the oracle and the FP64 dual gradient (the bar of check_trig_job)
     from test_gpu_jobs import check_trig_job
-    for i in np.random.default_rng(4).choice(len(jobs), 6, replace=False):
-        j = jobs[int(i)]
-        a, b = js.offsets[int(i)], js.offsets[int(i) + 1]
-        check_trig_job(d[a:b].cpu().numpy(), nr[a:b].cpu().numpy(), j, sample=20000, seed=int(i),
-                       tag=f"job {int(i)}: ")
+    d_all = d.cpu().numpy()
+    nr_all = nr.cpu().numpy()
+    for i, j in enumerate(jobs):
+        a, b = js.offsets[i], js.offsets[i + 1]
+        check_trig_job(d_all[a:b], nr_all[a:b], j, tag=f"job {i}: ")
     js.close()
