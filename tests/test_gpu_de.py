@@ -424,3 +424,40 @@ def test_trig_integral_powers_zero_and_negative(ctx, torch_cuda):
     d0 = dev.de_points(ca.MandelbulbTrig(15, 2.0, 0.0, ctx=ctx), gp).cpu().numpy()
     d0_o = O.de_points(O.mandelbulb_trig(15, 2.0, 0.0), pts)[0]
     assert_de_parity(d0, d0_o)
+
+
+def test_every_entry_point_takes_empty_input(ctx, torch_cuda):
+    """Zero points, spans, leaves, cameras or rays (the reference's batch
+    methods return an empty Vec for an empty slice, src/shape/util.rs:1-18):
+    every device entry point returns at once with empty outputs (the mesh's
+    offsets a single 0), and so do the host trait batches."""
+    import cantucci_amd as ca
+    from cantucci_amd import device as dev
+    torch = torch_cuda
+    p8 = ca.Mandelbulb.classic(15, 2.0, ctx=ctx)
+    trig = ca.MandelbulbTrig(15, 2.0, 7.5, ctx=ctx)
+    sph = ca.Sphere((0.0, 0.0, 0.0), 1.0, ctx=ctx)
+    z = lambda *s, dt=torch.float32: torch.zeros(s, dtype=dt, device="cuda")
+    for shape in (p8, trig, sph):
+        assert dev.de_points(shape, z(0, 3)).numel() == 0
+        assert shape.batch_min_distance_from(np.zeros((0, 3), F32)).size == 0
+        assert shape.batch_contains(np.zeros((0, 3), F32)).size == 0
+    assert sph.batch_max_distance_from(np.zeros((0, 3), F32)).size == 0
+    assert all(t.numel() == 0 for t in dev.de_state(p8, z(0, 3)))
+    d, cb = dev.sample_spans(p8, z(0, 6), 8)
+    assert d.shape == (0, 729) and cb.shape == (0, 8)
+    voff, ioff, v, ix = dev.mesh(p8, z(0, 6), z(0, 729), 8, "fd6")
+    assert voff.cpu().tolist() == [0] and ioff.cpu().tolist() == [0] and v.shape == (0, 7) and ix.numel() == 0
+    assert dev.mesh_vertices(p8, z(0, 7)).shape == (0, 7)
+    depth, steps = dev.sphere_trace(p8, z(1, 3), z(0, 3))
+    assert depth.numel() == 0 and steps.numel() == 0
+    depth, steps = dev.render_depth(p8, z(0, 14), 16, 9)
+    assert depth.shape == (0, 9, 16) and steps.shape == (0, 9, 16)
+    f, fd, fs = dev.get_focii(p8, z(0, 10), 5)
+    assert f.shape == (0, 25, 3) and fd.shape == (0, 25)
+    for fn in (dev.de_vjp, dev.de_grad, dev.normals):
+        assert all(t.numel() == 0 for t in fn(trig, z(0, 3)))
+    assert all(t.numel() == 0 for t in dev.normals_fd(trig, z(0, 3), 1e-3))
+    assert dev.focus_split(ctx, z(0, 6), z(0, 3), (0.0, 0.0, 3.0)).numel() == 0
+    assert dev.split_leaves(ctx, z(0, 6), z(0, dt=torch.int32)).shape == (0, 6)
+    torch.cuda.synchronize()
