@@ -9,8 +9,9 @@
 //     and square root, denormals kept, the same association order.  The loop
 //     state (r, dr, iteration count) is therefore bit-identical to the
 //     reference; only the final `ln` (correctly rounded here, glibc's logf in
-//     the reference) may differ by an ulp.  NaN results are canonicalised to the x86 default NaN (sign bit set)
-//     so that `is_sign_positive` classifies them exactly as the reference does.
+//     the reference) may differ by an ulp.  NaN results are canonicalised to
+//     the NaN x86 would give (canon_nan), so that `is_sign_positive`
+//     classifies them as the reference does.
 //
 //   * Fast evaluators (perf mode) compute the same map with explicit FMAs,
 //     shared subexpressions, hardware reciprocal / square root and the
@@ -95,18 +96,33 @@ __device__ __forceinline__ float sqrt_rn_wave(float x) {
 }
 
 // NaN sign of the reference.  x86 SSE arithmetic creates the default NaN
-// 0xFFC00000 (sign set) and propagates NaN operands unchanged; glibc's acosf
-// returns +NaN (0x7FC00000) for arguments outside [-1, 1].  In the distance
-// estimator acosf(z / r) sees +-inf only when r underflows to 0 with z != 0;
-// that NaN is then the orbit's first and every later NaN of the orbit is a
-// propagation of it.  A NaN distance carries the orbit's sign when r or dr is
-// NaN at the end; otherwise it was created by the final formula itself
-// (ln 0 * 0) and is the default NaN.  `is_sign_positive` then classifies
-// every NaN exactly as the reference does.
+// 0xFFC00000 (sign set) and propagates NaN operands unchanged (where two NaN
+// operands meet, the first source operand's); glibc's acosf returns +NaN
+// (0x7FC00000) for arguments outside [-1, 1].  An orbit's NaNs have one
+// source:
+//  * a NaN input coordinate.  The first magnitude (x*x + y*y) + z*z meets the
+//    coordinates in x, y, z order, so r carries the first NaN coordinate's
+//    sign, and every later NaN of the orbit is a propagation of r or of that
+//    coordinate (the orbit adds the point back each body);
+//  * acosf(z / r) = acosf(+-inf), which happens only when r underflows to 0
+//    with z != 0 and the input has no NaN: +NaN.
+// A NaN distance carries the orbit's sign when r or dr is NaN at the end;
+// otherwise it was created by the final formula itself (ln 0 * 0) and is the
+// default NaN.  `is_sign_positive` then classifies these NaNs exactly as the
+// reference does.  (A point with a NaN coordinate and also an infinite or
+// overflowing one is not covered: there the rotation creates a default NaN
+// (0 * inf, inf - inf) that meets the input's, and which one survives
+// depends on the compiled operand order.)
 __device__ __forceinline__ float canon_nan(float d, bool pos) {
     return (d != d) ? __uint_as_float(pos ? 0x7fc00000u : 0xffc00000u) : d;
 }
 __device__ __forceinline__ float canon_nan(float d) { return canon_nan(d, false); }
+// The orbit's NaN sign at the start: that of the first NaN coordinate in
+// x, y, z order, positive only if there is one and its sign bit is clear.
+__device__ __forceinline__ bool input_pos_nan(float x, float y, float z) {
+    const float f = (x != x) ? x : ((y != y) ? y : z);
+    return f != f && !__builtin_signbit(f);
+}
 // ln for the final formula, correctly rounded to f32 (up to a ~2^-18 chance
 // per input of a double-rounding miss).  glibc's logf, which the reference
 // calls, returns the correctly rounded value for 99.6% of f32 inputs in
@@ -195,7 +211,7 @@ struct P8Exact {
     static constexpr bool kCarriedGuard = true;
     float px, py, pz, x, y, z, dr;
     float x2, y2, w2, z2;  // squares of the last magnitude, reused by rotate
-    bool pos_nan;          // the orbit's first NaN came from acosf(+-inf)
+    bool pos_nan;          // the orbit's NaNs are positive (canon_nan)
     bool fast = false;     // guard(): the next body may take the unscaled sequences
 
     __device__ __forceinline__ void start(float ax, float ay, float az, float, const MandelUniform&) {
@@ -203,7 +219,7 @@ struct P8Exact {
         py = y = ay;
         pz = z = az;
         dr = 1.0f;
-        pos_nan = false;
+        pos_nan = input_pos_nan(ax, ay, az);
     }
     // r = sqrt((x*x + y*y) + z*z) (:412-418).  x*x, y*y, x*x + y*y are the same
     // IEEE operations as the rotate's x2, y2, rxy2 (:153-168), so they are
@@ -429,7 +445,7 @@ struct TrigExact {
         pz = z = az;
         dr = 1.0f;
         P = power;
-        pos_nan = false;
+        pos_nan = input_pos_nan(ax, ay, az);
         if (u.integral) {
             Pi = u.ipower;
             ladder = true;
@@ -570,7 +586,7 @@ struct P8Fast {
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float sphere_de(float px, float py, float pz, const float* c, float radius) {
     float dx = c[0] - px, dy = c[1] - py, dz = c[2] - pz;
-    return sqrt_rn((dx * dx + dy * dy) + dz * dz) - radius;
+    return canon_nan(sqrt_rn((dx * dx + dy * dy) + dz * dz) - radius, input_pos_nan(px, py, pz));
 }
 
 // The sphere as an evaluator of the lane-refill loops: it has no loop body,

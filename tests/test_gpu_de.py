@@ -9,6 +9,7 @@ import pytest
 
 import oracle as O
 import refmath as R
+from test_oracle import NAN_SHAPES, nan_input_points
 
 pytestmark = pytest.mark.gpu
 
@@ -461,3 +462,43 @@ def test_every_entry_point_takes_empty_input(ctx, torch_cuda):
     assert dev.focus_split(ctx, z(0, 6), z(0, 3), (0.0, 0.0, 3.0)).numel() == 0
     assert dev.split_leaves(ctx, z(0, 6), z(0, dt=torch.int32)).shape == (0, 6)
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("name,mi,_s", NAN_SHAPES, ids=[s[0] + f"-{s[1]}" for s in NAN_SHAPES])
+def test_nan_inputs_carry_their_sign(ctx, torch_cuda, name, mi, _s):
+    """Points with NaN coordinates of either sign, and NaNs with payloads:
+    iteration counts equal the oracle's, r and dr are NaN where the oracle's
+    are, and every distance is a NaN with the oracle's sign (the first NaN
+    coordinate's, in x, y, z order).  The distance is the canonical NaN of
+    that sign; the oracle carries an input payload through, so payloads are
+    compared by sign only."""
+    import cantucci_amd as ca
+    from cantucci_amd import device as dev
+    torch = torch_cuda
+    pay = np.array([0x7FC12345, 0xFFD00001, 0x7FFFFFFF], np.uint32).view(F32)
+    extra = np.array([[pay[0], 0.3, -0.2], [0.5, pay[1], 0.1], [0.0, 0.0, pay[2]], [pay[1], pay[0], 0.0]], F32)
+    pts = np.concatenate([nan_input_points(), extra])
+    if name == "sphere":
+        shape, orc = ca.Sphere((0.1, -0.2, 0.3), 1.0, ctx=ctx), O.sphere((0.1, -0.2, 0.3), 1.0)
+    elif name == "trig7.5":
+        shape, orc = ca.MandelbulbTrig(mi, 2.0, 7.5, ctx=ctx), O.mandelbulb_trig(mi, 2.0, 7.5)
+    else:
+        P = 8 if name == "p8" else int(name[4:])
+        shape, orc = ca.Mandelbulb(mi, 2.0, P, ctx=ctx), O.mandelbulb(mi, 2.0, P)
+    print(f"{name} max_iters {mi}: {len(pts)} points", flush=True)
+    g = gpu(torch, pts)
+    d, it = dev.de_points(shape, g, with_iters=True)
+    d, it = d.cpu().numpy(), it.cpu().numpy().astype(np.uint32)
+    d_o, it_o, r_o, dr_o = O.de_points(orc, pts)
+    assert np.isnan(d_o).all() and np.isnan(d).all()
+    assert (np.signbit(d) == np.signbit(d_o)).all()
+    canon = np.where(np.signbit(d_o), np.uint32(0xFFC00000), np.uint32(0x7FC00000))
+    assert (d.view(np.uint32) == canon).all()
+    assert (it == it_o).all()
+    if name != "sphere":
+        r, dr, it2 = (t.cpu().numpy() for t in dev.de_state(shape, g))
+        assert (it2.astype(np.uint32) == it_o).all()
+        assert (np.isnan(r) == np.isnan(r_o)).all() and (np.isnan(dr) == np.isnan(dr_o)).all()
+    # the host trait batch goes through the same kernels
+    dh = shape.batch_min_distance_from(pts)
+    assert (np.asarray(dh, F32).view(np.uint32) == canon).all()

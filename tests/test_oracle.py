@@ -140,6 +140,49 @@ def test_z_axis_branch_recurrence_and_origin_nan():
     assert (dm == dp[0]).all()
 
 
+def nan_input_points(signs=(1, -1)):
+    """Every point over {+NaN, -NaN, 0, -0, 0.3, -0.7, 1.25}^3 with at least one
+    NaN coordinate whose NaNs take the given signs (the other coordinates finite
+    and small, so no step of an orbit creates a NaN of its own)."""
+    import itertools
+    nan = [F32(np.nan) if s > 0 else -F32(np.nan) for s in signs]
+    vals = nan + [F32(0.0), F32(-0.0), F32(0.3), F32(-0.7), F32(1.25)]
+    pts = [p for p in itertools.product(vals, repeat=3) if any(np.isnan(v) for v in p)]
+    return np.array(pts, F32)
+
+
+def first_nan_positive(pts):
+    """Whether the first NaN coordinate of each point, in x, y, z order, is +NaN."""
+    nan = np.isnan(pts)
+    first = np.argmax(nan, axis=1)
+    return ~np.signbit(pts[np.arange(len(pts)), first])
+
+
+NAN_SHAPES = ([("p8", mi, O.mandelbulb(mi, 2.0)) for mi in (1, 2, 15)]
+              + [(f"trig{P}", mi, O.mandelbulb(mi, 2.0, P)) for P in (2, 3, 9) for mi in (1, 15)]
+              + [("trig7.5", 15, O.mandelbulb_trig(15, 2.0, 7.5)), ("sphere", 0, O.sphere((0.1, -0.2, 0.3), 1.0))])
+
+
+@pytest.mark.parametrize("name,mi,shape", NAN_SHAPES, ids=[s[0] + f"-{s[1]}" for s in NAN_SHAPES])
+def test_nan_input_sign_propagates(name, mi, shape):
+    """A NaN input coordinate reaches the distance with its sign.  With NaNs of
+    one sign the result is fixed by IEEE 754 alone: x86 returns a NaN operand
+    unchanged, and with the other coordinates finite and small no operation
+    of the orbit meets two non-NaN operands it cannot represent, so the
+    default NaN (sign set) never appears.  With NaNs of both signs x86 keeps
+    the first source operand's, and the magnitude (x*x + y*y) + z*z meets the
+    coordinates in x, y, z order: the first NaN coordinate's sign wins."""
+    for signs in ((1,), (-1,)):
+        pts = nan_input_points(signs)
+        d = O.de_points(shape, pts)[0]
+        assert np.isnan(d).all()
+        assert (np.signbit(d) == (signs[0] < 0)).all(), name
+    pts = nan_input_points()
+    d = O.de_points(shape, pts)[0]
+    assert np.isnan(d).all()
+    assert (~np.signbit(d) == first_nan_positive(pts)).all(), name
+
+
 @pytest.mark.parametrize("P", [3, 5])
 def test_trig_integer_power_vs_textbook(P):
     """Mandelbulb<P != 8> uses the trig map (src/shape/mandelbulb.rs:128-146)."""
