@@ -143,6 +143,7 @@ _SIGS = {
     "cth_sphere_trace_dev": (_I, [_P, _SH, _P, _I, _P, _S, _U, _F, _F, _F, _P, _P, _P, _P]),
     "cth_sphere_trace": (_I, [_P, _SH, _P, _I, _P, _S, _U, _F, _F, _F, _P, _P, _P]),
     "cth_de_vjp_dev": (_I, [_P, _SH, _P, _P, _S, _P, _P, _P, _P, _P]),
+    "cth_de_vjp_split_dev": (_I, [_P, _SH, _P, _P, _S, _P, _P, _P, _P, _P, _P]),
     "cth_de_vjp_noalloc_dev": (_I, [_P, _SH, _P, _P, _S, _P, _P, _P, _P, _P]),
     "cth_normals_dev": (_I, [_P, _SH, _P, _P, _S, _P, _P, _P]),
     "cth_de_grad_dev": (_I, [_P, _SH, _P, _P, _S, _P, _P, _P, _P, _P]),

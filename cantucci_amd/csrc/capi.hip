@@ -406,6 +406,24 @@ int cth_de_vjp_dev(cth_ctx* c, const cth_shape* s, const float* pts, const float
     return launch_adjoint(*c, *s, pts, powers, (uint32_t)n, grad_d, d, grad_p, gp, nullptr, st);
 }
 
+int cth_de_vjp_split_dev(cth_ctx* c, const cth_shape* s, const float* pts, const float* powers, size_t n,
+                         const float* grad_d, float* d, float* grad_z0, float* grad_c, float* grad_power,
+                         void* stream) {
+    if (!c) return fail(CTH_E_INVALID, "null context");
+    if (int rc = check_shape(s)) return rc;
+    if (n && !pts) return fail(CTH_E_INVALID, "null buffer");
+    if (n >= (size_t(1) << 31)) return fail(CTH_E_INVALID, "batch larger than 2^31");
+    hipStream_t st = pick_stream(c, stream);
+    float* gp = grad_power;
+    if (grad_power && s->kind != CTH_SHAPE_MANDELBULB_TRIG) {
+        int rc = launch_fill_u32(reinterpret_cast<uint32_t*>(grad_power), n, 0u, st);
+        if (rc) return rc;
+        gp = nullptr;
+    }
+    return launch_adjoint(*c, *s, pts, powers, (uint32_t)n, grad_d, d, nullptr, gp, nullptr, st, nullptr, 0, grad_z0,
+                          grad_c);
+}
+
 int cth_de_vjp_noalloc_dev(cth_ctx* c, const cth_shape* s, const float* pts, const float* powers, size_t n,
                            const float* grad_d, float* d, float* grad_p, float* grad_power, void* stream) {
     if (!c) return fail(CTH_E_INVALID, "null context");

@@ -149,7 +149,8 @@ int launch_trace_rays(Ctx& c, const cth_shape& s, const float* origins, int per_
 // caller's ws_bytes of device memory (at least vjp_workspace_bytes).
 int launch_adjoint(Ctx& c, const cth_shape& s, const float* pts, const float* powers, uint32_t n,
                    const float* grad_d, float* d, float* grad_p, float* grad_power, float* normals, hipStream_t st,
-                   void* workspace = nullptr, size_t ws_bytes = 0);
+                   void* workspace = nullptr, size_t ws_bytes = 0, float* grad_z0 = nullptr,
+                   float* grad_c = nullptr);
 // The allocation-free VJP's device scratch (kernels_adjoint.hip): its size,
 // the per-device set-up (hipGetSymbolAddress with `device` current, done by
 // cth_ctx_create) and the device address it recorded.

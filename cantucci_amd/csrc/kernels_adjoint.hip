@@ -248,9 +248,23 @@ struct AdjOut {
     float* grad_p;        // n x 3
     float* grad_power;    // nullable
     float* normals;       // nullable: grad_p / |grad_p| instead of grad_p
+    float* grad_z0;       // nullable: the part of grad_p through the orbit's start z0
+    float* grad_c;        // nullable: the part through the constant c of every body
 };
 
-__device__ __forceinline__ void adj_write(const AdjOut& out, uint32_t idx, float gx, float gy, float gz, float Pb) {
+// grad_p = (gx, gy, gz) = z_bar_0 + c_bar (z_0 = c = p), and its two parts.
+__device__ __forceinline__ void adj_write(const AdjOut& out, uint32_t idx, float gx, float gy, float gz, float zx,
+                                          float zy, float zz, float cx, float cy, float cz, float Pb) {
+    if (out.grad_z0) {
+        out.grad_z0[3 * (size_t)idx] = zx;
+        out.grad_z0[3 * (size_t)idx + 1] = zy;
+        out.grad_z0[3 * (size_t)idx + 2] = zz;
+    }
+    if (out.grad_c) {
+        out.grad_c[3 * (size_t)idx] = cx;
+        out.grad_c[3 * (size_t)idx + 1] = cy;
+        out.grad_c[3 * (size_t)idx + 2] = cz;
+    }
     if (out.normals) {
         // cgmath normalize: v * (1 / |v|)
         float m = __builtin_sqrtf(__builtin_fmaf(gx, gx, __builtin_fmaf(gy, gy, gz * gz)));
@@ -312,8 +326,9 @@ __global__ __launch_bounds__(256) void k_adj_reverse(SrcPoints src, MandelUnifor
                         int K = (int)rc.k;
                         float ir = 1.0f / rf;
                         if (K == 0) {
-                            // r_f = |c|: no loop body
-                            adj_write(out, idx, rbar * px * ir, rbar * py * ir, rbar * pz * ir, 0.0f);
+                            // r_f = |z_0|: no loop body, so c does not reach d
+                            const float gx = rbar * px * ir, gy = rbar * py * ir, gz = rbar * pz * ir;
+                            adj_write(out, idx, gx, gy, gz, gx, gy, gz, 0.0f, 0.0f, 0.0f, 0.0f);
                         } else {
                             if ((uint32_t)K == u.max_iters) {
                                 // iteration cap: r_f = |z_{K-1}|, z_K unused
@@ -343,8 +358,7 @@ __global__ __launch_bounds__(256) void k_adj_reverse(SrcPoints src, MandelUnifor
                 rev_step_p8(s.x, s.y, s.z, s.w, rb_extra, zbx, zby, zbz, drb, cbx, cby, cbz);
             rb_extra = 0.0f;
             if (--j < 0) {
-                // z_0 = c: grad_p = c_bar + z_bar_0
-                adj_write(out, idx, cbx + zbx, cby + zby, cbz + zbz, Pb);
+                adj_write(out, idx, cbx + zbx, cby + zby, cbz + zbz, zbx, zby, zbz, cbx, cby, cbz, Pb);
                 live = false;
             }
         }
@@ -438,7 +452,7 @@ size_t vjp_workspace_bytes(uint32_t max_iters, size_t n) {
 
 int launch_adjoint(Ctx& c, const cth_shape& s, const float* pts, const float* powers, uint32_t n,
                    const float* grad_d, float* d, float* grad_p, float* grad_power, float* normals, hipStream_t st,
-                   void* workspace, size_t ws_bytes) {
+                   void* workspace, size_t ws_bytes, float* grad_z0, float* grad_c) {
     if (n == 0) return CTH_OK;
     if (s.kind == CTH_SHAPE_SPHERE) return fail(CTH_E_UNSUPPORTED, "use the closed form for a sphere");
     MandelUniform u;
@@ -482,8 +496,9 @@ int launch_adjoint(Ctx& c, const cth_shape& s, const float* pts, const float* po
         rc = check_hip(hipGetLastError(), "adjoint forward launch");
         if (rc) break;
         // kernel 2: reverse sweep
-        AdjOut out{grad_d ? grad_d + off : nullptr, grad_p ? grad_p + 3 * off : nullptr,
-                   grad_power ? grad_power + off : nullptr, normals ? normals + 3 * off : nullptr};
+        AdjOut out{grad_d ? grad_d + off : nullptr,         grad_p ? grad_p + 3 * off : nullptr,
+                   grad_power ? grad_power + off : nullptr, normals ? normals + 3 * off : nullptr,
+                   grad_z0 ? grad_z0 + 3 * off : nullptr,   grad_c ? grad_c + 3 * off : nullptr};
         WorkQueue q2 = make_queue(c, m, c.shape_adj, blocks, st);
         if (family == FAM_P8)
             hipLaunchKernelGGL((k_adj_reverse<false>), dim3(blocks), dim3(c.shape_adj.block), 0, st, src, u, orbit, rec,

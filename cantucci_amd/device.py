@@ -272,6 +272,25 @@ def de_vjp(shape: Shape, pts: torch.Tensor, powers: torch.Tensor | None = None, 
     return (d, gp, gpow) if with_power else (d, gp)
 
 
+def de_vjp_split(shape: Shape, pts: torch.Tensor, powers: torch.Tensor | None = None,
+                 grad_d: torch.Tensor | None = None, with_power: bool = False, ctx: Context | None = None):
+    """(d, grad_z0, grad_c[, grad_power]): the reverse sweep's gradient with the
+    point's two roles apart, the orbit's start z0 and the constant c of every
+    body (grad_p = grad_z0 + grad_c)."""
+    ctx = ctx or shape.ctx
+    pts = _points(pts)
+    n = pts.shape[0]
+    powers = _per_point(powers, n, "powers")
+    grad_d = _per_point(grad_d, n, "grad_d")
+    d = torch.empty(n, dtype=torch.float32, device=pts.device)
+    gz = torch.empty((n, 3), dtype=torch.float32, device=pts.device)
+    gc = torch.empty((n, 3), dtype=torch.float32, device=pts.device)
+    gpow = torch.empty(n, dtype=torch.float32, device=pts.device) if with_power else None
+    check(lib.cth_de_vjp_split_dev(ctx.handle, C.byref(shape.desc), _ptr(pts), _ptr(powers), n, _ptr(grad_d),
+                                   _ptr(d), _ptr(gz), _ptr(gc), _ptr(gpow), _stream()))
+    return (d, gz, gc, gpow) if with_power else (d, gz, gc)
+
+
 def de_grad(shape: Shape, pts: torch.Tensor, powers: torch.Tensor | None = None, grad_d: torch.Tensor | None = None,
             with_power: bool = False, ctx: Context | None = None):
     """(d, grad_p[, grad_power]) in forward (tangent-linear) mode."""

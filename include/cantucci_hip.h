@@ -341,6 +341,20 @@ int cth_render_depth_dev(cth_ctx* ctx, const cth_shape* shape, const cth_camera*
 int cth_de_vjp_dev(cth_ctx* ctx, const cth_shape* shape, const float* pts, const float* powers, size_t n,
                    const float* grad_d, float* d, float* grad_p, float* grad_power, void* stream);
 
+/* cth_de_vjp_dev with the point's two roles apart.  The reference's orbit
+ * starts at z_0 = p and adds c = p in every body (src/shape/mandelbulb.rs:
+ * 60-74), so dd/dp = dd/dz0 + dd/dc; for fitting the additive constant as a
+ * shape parameter the parts are wanted alone:
+ *   grad_z0[i] = grad_d[i] * dd/dz0   (n x 3, nullable)
+ *   grad_c[i]  = grad_d[i] * dd/dc    (n x 3, nullable; 0 when no loop body
+ *                                      reaches the distance)
+ * grad_z0 + grad_c, summed in f32, is cth_de_vjp_dev's grad_p bit for bit (up to
+ * the sign of a zero where no loop body runs).
+ * d, grad_power, grad_d and the workspace as for cth_de_vjp_dev. */
+int cth_de_vjp_split_dev(cth_ctx* ctx, const cth_shape* shape, const float* pts, const float* powers, size_t n,
+                         const float* grad_d, float* d, float* grad_z0, float* grad_c, float* grad_power,
+                         void* stream);
+
 /* cth_de_vjp_dev without any allocation.  No counterpart in the reference:
  * for callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid it).  The orbit

@@ -21,7 +21,7 @@ K_MANDELBULB_TRIG = 1
 K_SPHERE = 2
 
 __all__ = [
-    "OrcShape", "mandelbulb", "mandelbulb_trig", "sphere", "de_points", "de_points_f64", "de_grad_f64",
+    "OrcShape", "mandelbulb", "mandelbulb_trig", "sphere", "de_points", "de_points_f64", "de_grad_f64", "de_grad_split_f64",
     "de_grid", "sample_spans", "surface_nets", "mesh_batch", "focus_split", "create_spans", "refine", "ray_dirs", "focus_dirs", "get_focii", "sphere_trace", "fd_diff",
     "powi", "build", "K_MANDELBULB", "K_MANDELBULB_TRIG", "K_SPHERE", "default_threads", "ln_mode",
 ]
@@ -61,6 +61,7 @@ for _name, _res, _args in [
     ("orc_de_points_f32", None, [_SH, _P, _P, _S, _P, _P, _P, _P, _I]),
     ("orc_de_points_f64", None, [_SH, _P, _P, _S, _P, _P, _I]),
     ("orc_de_grad_f64", None, [_SH, _P, _P, _S, _P, _P, _P, _I]),
+    ("orc_de_grad_split_f64", None, [_SH, _P, _P, _S, _P, _P, _P, _I]),
     ("orc_de_grid_f32", None, [_SH, _P, _P, _U, _F, _P, _P, _I]),
     ("orc_sample_spans", None, [_SH, _P, _S, _U, _P, _P, _P, _I]),
     ("orc_surface_nets", None, [_SH, _P, _U, _P, _P, _S, _P, _S, C.POINTER(_S), C.POINTER(_S)]),
@@ -155,6 +156,20 @@ def de_grad_f64(shape: OrcShape, pts, powers=None, threads=None):
     it = np.empty(n, np.uint32)
     _lib.orc_de_grad_f64(C.byref(shape), _ptr(pts), _ptr(pw), n, _ptr(d), _ptr(g), _ptr(it),
                          threads or default_threads())
+    return d, g, it
+
+
+def de_grad_split_f64(shape: OrcShape, pts, powers=None, threads=None):
+    """FP64 dual numbers with the point's two roles apart:
+    (d, grad[n,7] = dd/dz0 xyz, dd/dpower, dd/dc xyz, iters); dd/dp = dd/dz0 + dd/dc."""
+    pts = np.ascontiguousarray(np.asarray(pts, dtype=np.float64)).reshape(-1, 3)
+    n = len(pts)
+    pw = None if powers is None else _f32(powers)
+    d = np.empty(n, np.float64)
+    g = np.empty((n, 7), np.float64)
+    it = np.empty(n, np.uint32)
+    _lib.orc_de_grad_split_f64(C.byref(shape), _ptr(pts), _ptr(pw), n, _ptr(d), _ptr(g), _ptr(it),
+                               threads or default_threads())
     return d, g, it
 
 

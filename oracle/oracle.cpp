@@ -41,6 +41,10 @@
 //   * sphere tracing         -- sphere: analytic ray/sphere hit.
 //   * dual-number gradients  -- FP64 central differences; sphere closed form;
 //                               0-iteration closed form.
+//   * dd/dz0 vs dd/dc        -- central differences of an FP64 orbit with z0
+//                               and c apart; one-body closed form (bailout at
+//                               |z1|: dd/dc = 0.5 (ln r1 + 1) z1 / (r1 dr1));
+//                               dd/dc = 0 at the cap of one body and at 0 bodies.
 //   * ln mode 1, marked trace -- 50-digit decimal ln outside hard cases; a
 //                               march restated step by step marks the same rays.
 // Parity unpinned (no reference test or closed form fixes them; stated):
@@ -313,7 +317,10 @@ inline float shape_de(const orc_shape& s, float px, float py, float pz, uint32_t
 // Forward-mode dual numbers (FP64) for the gradient oracle
 // ---------------------------------------------------------------------------
 
-constexpr int NG = 4;  // d/dpx, d/dpy, d/dpz, d/dpower
+// Tangent slots: d/dz0 (x, y, z), d/dpower, d/dc (x, y, z).  The point p is
+// both the orbit's start z0 and the constant c added in every body
+// (src/shape/mandelbulb.rs:60-74), so d/dp = d/dz0 + d/dc.
+constexpr int NG = 7;
 
 struct Dual {
     double v;
@@ -392,19 +399,24 @@ inline Dual dpow_or_powi(const Dual& r, const Dual& P, int Pi, bool integral) {
 }
 
 // The reference DE evaluated in FP64 forward mode; P carries d/dpower when the
-// family is trig with real power.  Returns d with gradient (dd/dp, dd/dP).
+// family is trig with real power.  Returns d with gradient (dd/dz0, dd/dP,
+// dd/dc) in the slots above.
 inline Dual mandel_de_dual(double px, double py, double pz, const MandelParams& m, uint32_t* iters) {
-    Dual Px = dconst(px), Py = dconst(py), Pz = dconst(pz);
-    Px.g[0] = 1.0;
-    Py.g[1] = 1.0;
-    Pz.g[2] = 1.0;
+    Dual Px = dconst(px), Py = dconst(py), Pz = dconst(pz);  // c
+    Px.g[4] = 1.0;
+    Py.g[5] = 1.0;
+    Pz.g[6] = 1.0;
+    Dual Zx = dconst(px), Zy = dconst(py), Zz = dconst(pz);  // z0
+    Zx.g[0] = 1.0;
+    Zy.g[1] = 1.0;
+    Zz.g[2] = 1.0;
     Dual P = dconst(m.P);
     if (m.real_power) P.g[3] = 1.0;
     // The real-power kind is differentiated as powf throughout (an integral
     // power only changes the f32 forward's lowering, not the function).
     const bool integral = m.integral && !m.real_power;
     Dual one = dconst(1.0);
-    Dual x = Px, y = Py, z = Pz, dr = one, r = dconst(0.0);
+    Dual x = Zx, y = Zy, z = Zz, dr = one, r = dconst(0.0);
     uint32_t k = 0;
     for (uint64_t i = 0; i < m.max_iters; ++i) {
         r = dmag(x, y, z);
@@ -744,7 +756,33 @@ void orc_de_grad_f64(const orc_shape* shape, const double* pts, const float* pow
         uint32_t it = 0;
         Dual r = mandel_de_dual(px, py, pz, m, &it);
         d[i] = r.v;
-        for (int k = 0; k < 4; ++k) grad[4 * i + k] = r.g[k];
+        for (int k = 0; k < 3; ++k) grad[4 * i + k] = r.g[k] + r.g[4 + k];
+        grad[4 * i + 3] = r.g[3];
+        if (iters) iters[i] = it;
+    });
+}
+
+// The gradient with the point's two roles apart: grad[n x 7] = dd/dz0 (the
+// orbit's start), dd/dpower, dd/dc (the constant of every body).  A sphere
+// has no orbit: its gradient goes to dd/dz0 and dd/dc is 0.
+void orc_de_grad_split_f64(const orc_shape* shape, const double* pts, const float* powers, size_t n, double* d,
+                           double* grad, uint32_t* iters, int threads) {
+    const orc_shape s = *shape;
+    std::vector<double> g4(4 * n);
+    if (s.kind == K_SPHERE) {
+        orc_de_grad_f64(shape, pts, powers, n, d, g4.data(), iters, threads);
+        for (size_t i = 0; i < n; ++i) {
+            for (int k = 0; k < 4; ++k) grad[7 * i + k] = g4[4 * i + k];
+            for (int k = 4; k < 7; ++k) grad[7 * i + k] = 0.0;
+        }
+        return;
+    }
+    parallel_for(n, threads, 64, [&](size_t i) {
+        MandelParams m = mandel_params(s, powers ? powers[i] : s.power);
+        uint32_t it = 0;
+        Dual r = mandel_de_dual(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], m, &it);
+        d[i] = r.v;
+        for (int k = 0; k < 7; ++k) grad[7 * i + k] = r.g[k];
         if (iters) iters[i] = it;
     });
 }

@@ -59,12 +59,15 @@ def trig_map(x, y, z, P):
     return R * np.sin(P * t) * np.cos(P * f), R * np.sin(P * t) * np.sin(P * f), R * np.cos(P * t)
 
 
-def de_f64(p, mapfn, P, max_iters, bailout):
+def de_f64(p, mapfn, P, max_iters, bailout, c=None):
     """Distance estimate 0.5 ln(r) r / dr with dr' = P r^(P-1) dr + 1, in FP64,
-    for an arbitrary map (vectorised, with the reference's loop structure)."""
+    for an arbitrary map (vectorised, with the reference's loop structure).
+    The orbit starts at p and adds c in every body; c defaults to p, as in the
+    reference."""
     p = np.asarray(p, np.float64)
-    px, py, pz = p[:, 0], p[:, 1], p[:, 2]
-    x, y, z = px.copy(), py.copy(), pz.copy()
+    x, y, z = p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy()
+    c = p if c is None else np.asarray(c, np.float64)
+    px, py, pz = c[:, 0], c[:, 1], c[:, 2]
     dr = np.ones_like(px)
     r = np.zeros_like(px)
     active = np.ones(px.shape, bool)

@@ -165,3 +165,58 @@ def test_autograd_function(ctx, torch_cuda, sweep):
     assert np.median(ang) < 1e-5 and np.mean(ang <= 1e-3) > 0.97
     with pytest.raises(ValueError):
         dev.distance(shape, pts, powers, sweep="sideways")
+
+
+@pytest.mark.parametrize("kind", ["p8", "trig"])
+def test_vjp_split_z0_and_c(ctx, torch_cuda, kind):
+    """The VJP with the point's two roles apart (the orbit's start z0 and the
+    constant c of every body): the parts sum, in f32, to cth_de_vjp_dev's
+    grad_p, and each part agrees with the FP64 dual-number oracle's to the
+    bar of the VJP itself.  c does not reach the distance when no body runs
+    or when the only body is at the iteration cap: its part is exactly 0."""
+    import cantucci_amd as ca
+    from cantucci_amd import device as dev
+    from cantucci_amd.workloads import shell_points
+    torch = torch_cuda
+    if kind == "p8":
+        pts, _ = shell_points(30000, seed=2, r_lo=0.6, r_hi=1.25)
+        powers = None
+        shape, orc = ca.Mandelbulb.classic(15, 2.0, ctx=ctx), O.mandelbulb(15, 2.0)
+    else:
+        pts, powers = shell_points(30000, seed=3)
+        shape, orc = ca.MandelbulbTrig(15, 2.0, 8.0, ctx=ctx), O.mandelbulb_trig(15, 2.0)
+    gpts = gpu(torch, pts)
+    gpw = None if powers is None else gpu(torch, powers)
+    gd = gpu(torch, np.random.default_rng(32).uniform(0.5, 2.0, len(pts)).astype(F32))
+    d, gz, gc, gpow = dev.de_vjp_split(shape, gpts, gpw, gd, with_power=True)
+    d1, gp, gpow1 = dev.de_vjp(shape, gpts, gpw, gd, with_power=True)
+    torch.cuda.synchronize()
+    assert torch.equal(d.view(torch.int32), d1.view(torch.int32))
+    assert torch.equal(gpow.view(torch.int32), gpow1.view(torch.int32))
+    s = (gz + gc).cpu().numpy()
+    gp = gp.cpu().numpy()
+    assert ((s == gp) | (np.isnan(s) & np.isnan(gp))).all()
+    gz, gc, g = gz.cpu().numpy(), gc.cpu().numpy(), gd.cpu().numpy().astype(np.float64)
+    _, g64, it64 = O.de_grad_split_f64(orc, pts, powers)
+    it_o = O.de_points(orc, pts, powers)[1]
+    ok = (it_o == it64) & (it_o >= 1) & np.isfinite(g64).all(1)
+    for part, ref in ((gz, g64[:, :3]), (gc, g64[:, 4:])):
+        # an angle that is not defined (a zero or non-finite vector on either
+        # side) counts as a miss
+        with np.errstate(invalid="ignore"):
+            ang = np.nan_to_num(angle(part[ok], ref[ok]), nan=np.pi)
+            rel = np.nan_to_num(np.linalg.norm(part[ok] - g[ok, None] * ref[ok], axis=1)
+                                / np.linalg.norm(g[ok, None] * ref[ok], axis=1), nan=np.inf)
+        print(f"{kind}: {int((~np.isfinite(part[ok]).all(1)).sum())} of {int(ok.sum())} not finite; angle p50 "
+              f"{np.median(ang):.2e} p98 {np.percentile(ang, 98):.2e}; magnitude rel p50 {np.median(rel):.2e}")
+        assert np.median(ang) < 1e-5 and np.mean(ang <= 1e-3) > 0.98
+        assert np.median(rel) < 1e-4
+    # c does not reach the distance: zero bodies, or one body at the cap
+    far = np.random.default_rng(33).normal(size=(4000, 3))
+    far = (far / np.linalg.norm(far, axis=1, keepdims=True) * 3.0).astype(F32)
+    _, gzf, gcf = dev.de_vjp_split(shape, gpu(torch, far), None if powers is None else gpu(torch, powers[:4000]))
+    assert (gcf.cpu().numpy() == 0).all() and (gzf.cpu().numpy() != 0).any(axis=1).all()
+    one = ca.Mandelbulb.classic(1, 2.0, ctx=ctx) if kind == "p8" else ca.MandelbulbTrig(1, 2.0, 8.0, ctx=ctx)
+    _, _, gc1 = dev.de_vjp_split(one, gpts, gpw)
+    torch.cuda.synchronize()
+    assert (gc1.cpu().numpy() == 0).all()

@@ -456,7 +456,7 @@ def test_every_entry_point_takes_empty_input(ctx, torch_cuda):
     assert depth.shape == (0, 9, 16) and steps.shape == (0, 9, 16)
     f, fd, fs = dev.get_focii(p8, z(0, 10), 5)
     assert f.shape == (0, 25, 3) and fd.shape == (0, 25)
-    for fn in (dev.de_vjp, dev.de_grad, dev.normals):
+    for fn in (dev.de_vjp, dev.de_vjp_split, dev.de_grad, dev.normals):
         assert all(t.numel() == 0 for t in fn(trig, z(0, 3)))
     assert all(t.numel() == 0 for t in dev.normals_fd(trig, z(0, 3), 1e-3))
     assert dev.focus_split(ctx, z(0, 6), z(0, 3), (0.0, 0.0, 3.0)).numel() == 0

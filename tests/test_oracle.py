@@ -381,6 +381,69 @@ def test_dual_gradient_matches_fp64_central_differences():
         assert np.percentile(err[ok], 99) < 1e-4
 
 
+def test_dual_gradient_split_z0_and_c():
+    """The point's two roles apart: dd/dz0 (the orbit's start) and dd/dc (the
+    constant of every body) against central differences of an FP64 orbit that
+    takes z0 and c separately (the P8 map in its complex form, the trig map in
+    its textbook form), and their sum against dd/dp."""
+    p = _rng_points(3000, -1.3, 1.3, 21)
+    cases = ((O.mandelbulb(15, 2.0), None, lambda x, y, z, P: R.p8_map_complex(x, y, z), 8.0),
+             (O.mandelbulb_trig(15, 2.0), np.random.default_rng(22).uniform(6, 10, len(p)).astype(F32), R.trig_map,
+              None))
+    for shape, powers, fmap, P in cases:
+        P = powers.astype(np.float64) if P is None else P
+        d, g, it = O.de_grad_split_f64(shape, p, powers)
+        _, gp, _ = O.de_grad_f64(shape, p, powers)
+        assert np.allclose(g[:, :3] + g[:, 4:], gp[:, :3], rtol=1e-12, atol=0)
+        assert (g[:, 3] == gp[:, 3]).all()
+        p64 = p.astype(np.float64)
+        h = 1e-6
+        for lo, start in ((0, True), (4, False)):
+            fd = np.zeros((len(p), 3))
+            stable = np.ones(len(p), bool)
+            for a in range(3):
+                e = np.zeros(3)
+                e[a] = h
+                if start:
+                    dp, itp = R.de_f64(p64 + e, fmap, P, 15, 2.0, c=p64)
+                    dm, itm = R.de_f64(p64 - e, fmap, P, 15, 2.0, c=p64)
+                else:
+                    dp, itp = R.de_f64(p64, fmap, P, 15, 2.0, c=p64 + e)
+                    dm, itm = R.de_f64(p64, fmap, P, 15, 2.0, c=p64 - e)
+                fd[:, a] = (dp - dm) / (2 * h)
+                stable &= (itp == it) & (itm == it)
+            gs = g[:, lo:lo + 3]
+            err = np.linalg.norm(gs - fd, axis=1) / np.maximum(np.linalg.norm(gs, axis=1), 1e-12)
+            ok = stable & (it <= 6) & (it >= 1)
+            assert ok.sum() > 1000
+            assert np.percentile(err[ok], 99) < 1e-4, (lo, np.percentile(err[ok], 99))
+        # a plausible mistake: the roles swapped
+        ok = (it >= 2) & (it <= 6)
+        assert np.median(np.linalg.norm(g[ok, :3] - g[ok, 4:], axis=1) / np.linalg.norm(g[ok, 4:], axis=1)) > 0.1
+
+
+def test_dual_gradient_split_closed_forms():
+    """dd/dc = 0 where c never reaches the distance: no body (|p| > bailout)
+    and one body at the iteration cap (the distance takes |z0|).  After one
+    body that bails (|z0| <= bailout < |z1|, z1 = F(z0) + c, dr1 = 8 |z0|^7 + 1):
+    dd/dc = 0.5 (ln r1 + 1) z1 / (r1 dr1)."""
+    p = _rng_points(4000, -2.5, 2.5, 23)
+    _, g, it = O.de_grad_split_f64(O.mandelbulb(15, 2.0), p)
+    out = np.linalg.norm(p.astype(np.float64), axis=1) > 2.0
+    assert out.sum() > 500 and (it[out] == 0).all() and (g[out, 4:] == 0).all()
+    _, g1, it1 = O.de_grad_split_f64(O.mandelbulb(1, 2.0), p)
+    assert (it1[~out] == 1).all() and (g1[:, 4:] == 0).all()
+    assert (np.linalg.norm(g1[~out, :3], axis=1) > 0).all()
+    one = it == 1
+    assert one.sum() > 300
+    p64 = p[one].astype(np.float64)
+    r0 = np.linalg.norm(p64, axis=1, keepdims=True)
+    z1 = np.stack(R.p8_map_complex(p64[:, 0], p64[:, 1], p64[:, 2]), 1) + p64
+    r1 = np.linalg.norm(z1, axis=1, keepdims=True)
+    expect = 0.5 * (np.log(r1) + 1) * z1 / (r1 * (8 * r0 ** 7 + 1))
+    assert np.allclose(g[one, 4:], expect, rtol=1e-9, atol=1e-12)
+
+
 def test_dual_gradient_power_matches_central_difference():
     p = _rng_points(2000, -1.3, 1.3, 17)
     powers = np.random.default_rng(18).uniform(6, 10, len(p)).astype(np.float64)
