@@ -30,9 +30,9 @@ SPACE = {
     "dynamic": [0, 1],
 }
 
-# The trace kernel applies finished evaluations' steps in the iteration
-# they finish in, and refill_min only batches the fetch of new rays, which
-# pays off at small batches.
+# The trace kernel batches finished evaluations' steps by a fixed threshold
+# of its own; refill_min only batches the fetch of new rays, which pays off
+# at small batches.
 SPACES = {
     "trace": {
         "block": [256],
