@@ -29,6 +29,9 @@ CTH_NORMALS_FD6 = 0
 CTH_NORMALS_TANGENT = 1
 CTH_NORMALS_DEFERRED = 2
 
+CTH_SWEEP_FORWARD = 0
+CTH_SWEEP_REVERSE = 1
+
 
 class CthShape(C.Structure):
     _fields_ = [
@@ -144,6 +147,7 @@ _SIGS = {
     "cth_sphere_trace": (_I, [_P, _SH, _P, _I, _P, _S, _U, _F, _F, _F, _P, _P, _P]),
     "cth_de_vjp_dev": (_I, [_P, _SH, _P, _P, _S, _P, _P, _P, _P, _P]),
     "cth_de_vjp_split_dev": (_I, [_P, _SH, _P, _P, _S, _P, _P, _P, _P, _P, _P]),
+    "cth_de_vjp_batch": (_I, [_P, _SH, _P, _P, _S, _P, _P, _P, _P, _I]),
     "cth_de_vjp_noalloc_dev": (_I, [_P, _SH, _P, _P, _S, _P, _P, _P, _P, _P]),
     "cth_normals_dev": (_I, [_P, _SH, _P, _P, _S, _P, _P, _P]),
     "cth_de_grad_dev": (_I, [_P, _SH, _P, _P, _S, _P, _P, _P, _P, _P]),

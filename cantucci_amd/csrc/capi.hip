@@ -821,4 +821,36 @@ int cth_contains_batch(cth_ctx* c, const cth_shape* s, const cth_point3* pts, si
     return host_query(c, s, pts, n, out, Q_CONTAINS);
 }
 
+int cth_de_vjp_batch(cth_ctx* c, const cth_shape* s, const cth_point3* pts, const float* powers, size_t n,
+                     const float* grad_d, float* d, float* grad_p, float* grad_power, int sweep) {
+    if (!c) return fail(CTH_E_INVALID, "null context");
+    if (int rc = check_shape(s)) return rc;
+    if (sweep != CTH_SWEEP_FORWARD && sweep != CTH_SWEEP_REVERSE)
+        return fail(CTH_E_INVALID, "sweep must be CTH_SWEEP_FORWARD or CTH_SWEEP_REVERSE");
+    if (n == 0) return CTH_OK;
+    if (!pts) return fail(CTH_E_INVALID, "null buffer");
+    if (n >= (size_t(1) << 31)) return fail(CTH_E_INVALID, "batch larger than 2^31");
+    void* b[6];  // points, powers, grad_d, d, grad_p, grad_power
+    int rc = staging(*c, WS_HOST_IN, {n * 12, powers ? n * 4 : 0, grad_d ? n * 4 : 0, n * 4, n * 12, n * 4}, b);
+    if (rc) return rc;
+    rc = check_hip(hipMemcpyAsync(b[0], pts, n * 12, hipMemcpyHostToDevice, c->stream), "H2D");
+    if (!rc && powers) rc = check_hip(hipMemcpyAsync(b[1], powers, n * 4, hipMemcpyHostToDevice, c->stream), "H2D");
+    if (!rc && grad_d) rc = check_hip(hipMemcpyAsync(b[2], grad_d, n * 4, hipMemcpyHostToDevice, c->stream), "H2D");
+    if (!rc) {
+        const float* dp = (const float*)b[0];
+        const float* dpw = powers ? (const float*)b[1] : nullptr;
+        const float* dg = grad_d ? (const float*)b[2] : nullptr;
+        float* gpw = grad_power ? (float*)b[5] : nullptr;
+        rc = sweep == CTH_SWEEP_FORWARD
+                 ? cth_de_grad_dev(c, s, dp, dpw, n, dg, (float*)b[3], (float*)b[4], gpw, c->stream)
+                 : cth_de_vjp_dev(c, s, dp, dpw, n, dg, (float*)b[3], (float*)b[4], gpw, c->stream);
+    }
+    if (!rc && d) rc = check_hip(hipMemcpyAsync(d, b[3], n * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+    if (!rc && grad_p) rc = check_hip(hipMemcpyAsync(grad_p, b[4], n * 12, hipMemcpyDeviceToHost, c->stream), "D2H");
+    if (!rc && grad_power)
+        rc = check_hip(hipMemcpyAsync(grad_power, b[5], n * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+    int rs = check_hip(hipStreamSynchronize(c->stream), "sync");
+    return rc ? rc : rs;
+}
+
 }  // extern "C"

@@ -212,6 +212,52 @@ class Sphere : public Shape {
     }
 };
 
+// A differentiable distance op with the contract of torch.autograd.Function:
+// forward(points[, powers]) returns Shape::batch_min_distance_from and keeps
+// its inputs (save_for_backward); backward(grad_d) returns the gradients of
+// sum(grad_d * d) with respect to the points and the per-point powers.  The
+// reference has no derivative; this differentiates its min_distance_from
+// (src/shape/mandelbulb.rs:59-79).  The sweep is the forward-mode Jacobian
+// (the faster one) or the reverse sweep; both are the device kernels'.
+class DistanceOp {
+  public:
+    struct Grads {
+        std::vector<Point3> points;  // dd/dp * grad_d
+        std::vector<float> powers;   // dd/dpower * grad_d (0 except for MandelbulbTrig)
+    };
+
+    explicit DistanceOp(const Shape& shape, int sweep = CTH_SWEEP_FORWARD) : shape_(shape), sweep_(sweep) {}
+
+    std::vector<float> forward(const std::vector<Point3>& points, const std::vector<float>& powers = {}) {
+        if (!powers.empty() && powers.size() != points.size())
+            throw Error(CTH_E_INVALID, "DistanceOp: one power per point");
+        points_ = points;
+        powers_ = powers;
+        std::vector<float> d(points.size());
+        if (powers.empty()) return shape_.batch_min_distance_from(points);
+        check(cth_de_vjp_batch(shape_.context().get(), &shape_.desc(),
+                               reinterpret_cast<const cth_point3*>(points.data()), powers.data(), points.size(),
+                               nullptr, d.data(), nullptr, nullptr, sweep_));
+        return d;
+    }
+
+    Grads backward(const std::vector<float>& grad_d) const {
+        if (grad_d.size() != points_.size()) throw Error(CTH_E_INVALID, "DistanceOp: one grad_d per point");
+        Grads g{std::vector<Point3>(points_.size()), std::vector<float>(points_.size())};
+        check(cth_de_vjp_batch(shape_.context().get(), &shape_.desc(),
+                               reinterpret_cast<const cth_point3*>(points_.data()),
+                               powers_.empty() ? nullptr : powers_.data(), points_.size(), grad_d.data(), nullptr,
+                               reinterpret_cast<float*>(g.points.data()), g.powers.data(), sweep_));
+        return g;
+    }
+
+  private:
+    const Shape& shape_;
+    int sweep_;
+    std::vector<Point3> points_;
+    std::vector<float> powers_;
+};
+
 // `Vertex` (src/mesh/mod.rs:255-261).
 struct Vertex {
     float position[3];

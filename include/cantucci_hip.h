@@ -202,6 +202,19 @@ int cth_bounded_distance_batch(cth_ctx* ctx, const cth_shape* shape, const cth_p
  * |c - p|^2 <= r^2, src/shape/sphere.rs:23-26).  One byte (0/1) per point. */
 int cth_contains_batch(cth_ctx* ctx, const cth_shape* shape, const cth_point3* pts, size_t n, uint8_t* out);
 
+/* The backward of a differentiable distance op whose forward is
+ * cth_min_distance_batch (host buffers, synchronous), for a host-side op with
+ * an autograd function's contract (the forward keeps its inputs, the backward
+ * maps the output's gradient to the inputs').  The reference has no derivative: this differentiates its min_distance_from
+ * (src/shape/mandelbulb.rs:59-79).  With sweep CTH_SWEEP_FORWARD the results
+ * are those of cth_de_grad_dev, with CTH_SWEEP_REVERSE those of
+ * cth_de_vjp_dev, bit for bit.  powers (per-point powers of
+ * CTH_SHAPE_MANDELBULB_TRIG) and grad_d (NULL: ones) are nullable inputs;
+ * d, grad_p (n x 3) and grad_power are nullable outputs. */
+enum { CTH_SWEEP_FORWARD = 0, CTH_SWEEP_REVERSE = 1 };
+int cth_de_vjp_batch(cth_ctx* ctx, const cth_shape* shape, const cth_point3* pts, const float* powers, size_t n,
+                     const float* grad_d, float* d, float* grad_p, float* grad_power, int sweep);
+
 /* ---- device-pointer entry points (asynchronous on `stream`) ------------------ */
 
 /* Distances of n AoS points (3 floats each).  `powers` (nullable) gives a

@@ -6,6 +6,7 @@
 // update (split rule, create_spans, the leaves' meshes).
 // Needs a gfx950 device.  Prints one line per check; exits non-zero on the
 // first failure.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -32,6 +33,8 @@ void orc_mesh_batch(const cth_shape* shape, const float* spans, size_t nspans, u
 void orc_sphere_trace(const cth_shape* shape, const float* origins, const float* dirs, size_t n, int origin_stride,
                       uint32_t max_steps, float eps_abs, float eps_rel, float t_max, float* depth, uint32_t* steps,
                       float* hit_pos, int threads);
+void orc_de_grad_f64(const cth_shape* shape, const double* pts, const float* powers, size_t n, double* d,
+                     double* grad, uint32_t* iters, int threads);
 }
 
 namespace {
@@ -316,6 +319,112 @@ int run() {
         bool all_meshed = true;
         for (const auto& leaf : sm.leaves()) all_meshed &= leaf.mesh.has_value();
         expect(all_meshed && meshes_match(children), "update 2: the new leaves meshed as the oracle meshes them");
+    }
+
+    // DistanceOp: forward / backward over cth_de_vjp_batch, against the
+    // oracle's FP64 dual-number gradient (shell 0.9 < |p| < 1.3).
+    {
+        std::mt19937 rg(11);
+        std::normal_distribution<float> N(0.0f, 1.0f);
+        std::uniform_real_distribution<float> R(0.9f, 1.3f), W(6.0f, 10.0f);
+        std::vector<Point3> sp(4000);
+        std::vector<float> pw(sp.size());
+        for (size_t i = 0; i < sp.size(); ++i) {
+            float v[3] = {N(rg), N(rg), N(rg)};
+            float m = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]), r = R(rg);
+            sp[i] = {v[0] / m * r, v[1] / m * r, v[2] / m * r};
+            pw[i] = W(rg);
+        }
+        std::vector<double> p64(3 * sp.size());
+        for (size_t i = 0; i < sp.size(); ++i) {
+            p64[3 * i] = sp[i].x;
+            p64[3 * i + 1] = sp[i].y;
+            p64[3 * i + 2] = sp[i].z;
+        }
+        auto angle = [](Point3 a, const double* b) {
+            double dot = a.x * b[0] + a.y * b[1] + a.z * b[2];
+            double na = std::sqrt((double)a.x * a.x + (double)a.y * a.y + (double)a.z * a.z);
+            double nb = std::sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+            double c = dot / (na * nb);
+            return std::acos(std::max(-1.0, std::min(1.0, c)));
+        };
+        // the share of points within 1e-3 rad, and the median angle, over
+        // points whose f32 and FP64 orbits take the same number of bodies
+        auto score = [&](const std::vector<Point3>& g, const std::vector<double>& g64, const std::vector<uint32_t>& it64,
+                         const std::vector<uint32_t>& it32, double& within, double& median) {
+            std::vector<double> a;
+            for (size_t i = 0; i < g.size(); ++i)
+                if (it64[i] == it32[i]) a.push_back(angle(g[i], &g64[4 * i]));
+            std::sort(a.begin(), a.end());
+            within = a.empty() ? 0.0 : (double)(std::upper_bound(a.begin(), a.end(), 1e-3) - a.begin()) / a.size();
+            median = a.empty() ? 1.0 : a[a.size() / 2];
+        };
+        for (int sweep : {CTH_SWEEP_FORWARD, CTH_SWEEP_REVERSE}) {
+            const char* name = sweep == CTH_SWEEP_FORWARD ? "forward sweep" : "reverse sweep";
+            DistanceOp op(m8, sweep);
+            auto d = op.forward(sp);
+            auto ref = m8.batch_min_distance_from(sp);
+            bool same = true;
+            for (size_t i = 0; i < sp.size(); ++i) same &= bits(d[i]) == bits(ref[i]);
+            std::vector<float> ones(sp.size(), 1.0f), twos(sp.size(), 2.0f);
+            auto g1 = op.backward(ones);
+            auto g2 = op.backward(twos);
+            bool lin = true;
+            for (size_t i = 0; i < sp.size(); ++i)
+                lin &= bits(g2.points[i].x) == bits(2.0f * g1.points[i].x) &&
+                       bits(g2.points[i].y) == bits(2.0f * g1.points[i].y) &&
+                       bits(g2.points[i].z) == bits(2.0f * g1.points[i].z) && g1.powers[i] == 0.0f;
+            std::vector<double> d64(sp.size()), g64(4 * sp.size());
+            std::vector<uint32_t> it64(sp.size()), it32(sp.size());
+            orc_de_grad_f64(&m8.desc(), p64.data(), nullptr, sp.size(), d64.data(), g64.data(), it64.data(), 4);
+            orc_de_points_f32(&m8.desc(), reinterpret_cast<const float*>(sp.data()), nullptr, sp.size(), d.data(),
+                              it32.data(), nullptr, nullptr, 4);
+            double within, median;
+            score(g1.points, g64, it64, it32, within, median);
+            std::printf("DistanceOp P8 %s: %.4f within 1e-3 rad of the FP64 gradient, median %.2e rad\n", name,
+                        within, median);
+            expect(same, "DistanceOp::forward == batch_min_distance_from");
+            expect(lin, "DistanceOp::backward is linear in grad_d (bit for bit), no power gradient for P8");
+            expect(within > 0.98 && median < 1e-5, "DistanceOp::backward vs the FP64 dual gradient (P8)");
+        }
+        {
+            MandelbulbTrig mt(ctx, 15, 2.0f, 8.0f);
+            DistanceOp op(mt);
+            auto d = op.forward(sp, pw);
+            std::vector<float> ones(sp.size(), 1.0f);
+            auto g = op.backward(ones);
+            std::vector<double> d64(sp.size()), g64(4 * sp.size());
+            std::vector<uint32_t> it64(sp.size()), it32(sp.size());
+            std::vector<float> d32(sp.size());
+            orc_de_grad_f64(&mt.desc(), p64.data(), pw.data(), sp.size(), d64.data(), g64.data(), it64.data(), 4);
+            orc_de_points_f32(&mt.desc(), reinterpret_cast<const float*>(sp.data()), pw.data(), sp.size(),
+                              d32.data(), it32.data(), nullptr, nullptr, 4);
+            double within, median;
+            score(g.points, g64, it64, it32, within, median);
+            std::vector<double> rel;
+            size_t close = 0;
+            for (size_t i = 0; i < sp.size(); ++i) {
+                if (it64[i] != it32[i]) continue;
+                rel.push_back(std::fabs(g.powers[i] - g64[4 * i + 3]) / std::max(std::fabs(g64[4 * i + 3]), 1e-12));
+                close += std::fabs((double)d[i] - d64[i]) <= 1e-4 * std::fabs(d64[i]);
+            }
+            std::sort(rel.begin(), rel.end());
+            double rmed = rel.empty() ? 1.0 : rel[rel.size() / 2];
+            std::printf("DistanceOp trig, per-point powers: %.4f within 1e-3 rad, median %.2e rad; "
+                        "d/dpower median rel %.2e; %zu of %zu distances within 1e-4 of FP64\n",
+                        within, median, rmed, close, rel.size());
+            expect(within > 0.98 && median < 1e-5 && rmed < 1e-4 && close >= rel.size() * 90 / 100,
+                   "DistanceOp with per-point powers vs the FP64 dual gradient (trig)");
+            expect(throws_status(CTH_E_INVALID, [&] { op.backward(std::vector<float>(3, 1.0f)); }),
+                   "DistanceOp::backward refuses a grad_d of the wrong length");
+        }
+        {
+            Sphere sph(ctx, {0.0f, 0.0f, 0.0f}, 1.0f);
+            DistanceOp op(sph);
+            op.forward(sp);
+            expect(throws_status(CTH_E_UNSUPPORTED, [&] { op.backward(std::vector<float>(sp.size(), 1.0f)); }),
+                   "DistanceOp on a sphere refuses (its gradient is the closed form)");
+        }
     }
 
     std::printf("%s: %d failure(s)\n", failures ? "FAILED" : "PASSED", failures);

@@ -220,3 +220,30 @@ def test_vjp_split_z0_and_c(ctx, torch_cuda, kind):
     _, _, gc1 = dev.de_vjp_split(one, gpts, gpw)
     torch.cuda.synchronize()
     assert (gc1.cpu().numpy() == 0).all()
+
+
+def test_host_vjp_batch_equals_device_entries(ctx, torch_cuda):
+    """cth_de_vjp_batch (host buffers) gives cth_de_grad_dev's results with
+    the forward sweep and cth_de_vjp_dev's with the reverse sweep, bit for
+    bit, with per-point powers and a grad_d; a bad sweep is refused."""
+    import ctypes as C
+    import cantucci_amd as ca
+    from cantucci_amd import _lib
+    from cantucci_amd import device as dev
+    from cantucci_amd.workloads import shell_points
+    torch = torch_cuda
+    pts, powers = shell_points(20000, seed=4)
+    g = np.random.default_rng(41).uniform(-2, 2, len(pts)).astype(F32)
+    shape = ca.MandelbulbTrig(15, 2.0, 8.0, ctx=ctx)
+    P = lambda a: a.ctypes.data_as(C.c_void_p)
+    for sweep, fn in ((_lib.CTH_SWEEP_FORWARD, dev.de_grad), (_lib.CTH_SWEEP_REVERSE, dev.de_vjp)):
+        d, gp, gw = np.empty(len(pts), F32), np.empty((len(pts), 3), F32), np.empty(len(pts), F32)
+        _lib.check(_lib.lib.cth_de_vjp_batch(ctx.handle, C.byref(shape.desc), P(pts), P(powers), len(pts), P(g),
+                                             P(d), P(gp), P(gw), sweep))
+        d1, gp1, gw1 = (t.cpu().numpy() for t in fn(shape, gpu(torch, pts), gpu(torch, powers), gpu(torch, g),
+                                                     with_power=True))
+        for a, b in ((d, d1), (gp, gp1), (gw, gw1)):
+            assert (a.view(np.uint32) == b.view(np.uint32)).all(), sweep
+    with pytest.raises(ca.CthError):
+        _lib.check(_lib.lib.cth_de_vjp_batch(ctx.handle, C.byref(shape.desc), P(pts), None, len(pts), None, P(d),
+                                             None, None, 7))
