@@ -218,7 +218,8 @@ class Sphere : public Shape {
 // sum(grad_d * d) with respect to the points and the per-point powers.  The
 // reference has no derivative; this differentiates its min_distance_from
 // (src/shape/mandelbulb.rs:59-79).  The sweep is the forward-mode Jacobian
-// (the faster one) or the reverse sweep; both are the device kernels'.
+// (the faster one) or the reverse sweep; both are the device kernels'.  The
+// op refers to its shape, which must outlive it.
 class DistanceOp {
   public:
     struct Grads {
@@ -233,8 +234,10 @@ class DistanceOp {
             throw Error(CTH_E_INVALID, "DistanceOp: one power per point");
         points_ = points;
         powers_ = powers;
-        std::vector<float> d(points.size());
         if (powers.empty()) return shape_.batch_min_distance_from(points);
+        // per-point powers: the distances of the gradient kernel, which equal
+        // the distance kernel's bit for bit
+        std::vector<float> d(points.size());
         check(cth_de_vjp_batch(shape_.context().get(), &shape_.desc(),
                                reinterpret_cast<const cth_point3*>(points.data()), powers.data(), points.size(),
                                nullptr, d.data(), nullptr, nullptr, sweep_));

@@ -37,9 +37,10 @@ def test_trig_vjp_matches_dual_oracle(ctx, torch_cuda):
     d_fwd = dev.de_points(shape, gpu(torch, pts), gpu(torch, powers)).cpu().numpy()
     assert (d.view(np.uint32) == d_fwd.view(np.uint32)).all()
     # forward-mode (tangent) gradient: same quantity, other evaluation order
-    _, gt, gtp = dev.de_grad(shape, gpu(torch, pts), gpu(torch, powers), with_power=True)
+    dt, gt, gtp = dev.de_grad(shape, gpu(torch, pts), gpu(torch, powers), with_power=True)
     torch.cuda.synchronize()
-    gt, gtp = gt.cpu().numpy(), gtp.cpu().numpy()
+    dt, gt, gtp = dt.cpu().numpy(), gt.cpu().numpy(), gtp.cpu().numpy()
+    assert (dt.view(np.uint32) == d_fwd.view(np.uint32)).all()
     same = it_o == it64
     ang = angle(gp[same], g64[same, :3])
     relp = np.abs(gpow[same] - g64[same, 3]) / np.maximum(np.abs(g64[same, 3]), 1e-12)
