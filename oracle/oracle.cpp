@@ -24,6 +24,11 @@
 //                               trig form on the y=0 plane, FP64 agreement.
 //   * DE, z-axis branch      -- 1-D recurrence z <- |z|^8 + pz (brute force);
 //                               origin gives a sign-negative NaN.
+//   * DE, NaN inputs         -- NaN coordinates of one sign give a NaN
+//                               distance of that sign (IEEE 754 propagation;
+//                               P8, trig and sphere); with both signs the
+//                               first NaN coordinate's (x86 keeps the first
+//                               operand's NaN).
 //   * DE, trig family        -- FP64 textbook formula, y-reflection equivariance.
 //   * powi ladder            -- exact integer powers of small integers.
 //   * Sphere                 -- closed forms.
